@@ -1,0 +1,212 @@
+/* hz.h -- C ABI of the MI355X 3D Helmholtz solver hot path.
+ *
+ * The reference (dklyuchinskiy/Iterative-Helmholtz-solver-with-PARDISO, paths
+ * below relative to "Iterative Helholtz solver with PARDISO/Source/") exposes
+ * its hot path as free C++ functions with MKL PARDISO handles threaded through;
+ * each entry point here names the reference function it replaces.
+ *
+ * Conventions
+ *  - complex numbers are interleaved (re, im) doubles: hz_c128;
+ *  - every vector argument is a DEVICE pointer holding nrhs consecutive
+ *    vectors of N = nx*ny*nz points, layout l = i + nx*j + nx*ny*k
+ *    (functions.cpp:225-230);
+ *  - all calls are ordered on the problem's HIP stream; calls that return
+ *    host-visible results (hz_fgmres, hz_bicgstab, norms, misfits) synchronise
+ *    that stream before returning;
+ *  - every function returns an HZ_* status; hz_last_error() gives the message
+ *    of the last failure on the calling thread;
+ *  - a problem handle is not thread-safe: one host thread per handle.
+ */
+#ifndef HZ_H
+#define HZ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  double re, im;
+} hz_c128;
+
+enum {
+  HZ_OK = 0,
+  HZ_ERR_ARG = 1,           /* invalid argument or size */
+  HZ_ERR_HIP = 2,           /* HIP runtime error */
+  HZ_ERR_STATE = 3,         /* call out of order (e.g. apply before factor) */
+  HZ_ERR_NOMEM = 4,         /* device allocation failed */
+  HZ_ERR_NOT_CONVERGED = 5, /* Krylov driver hit its iteration limit */
+  HZ_ERR_BREAKDOWN = 6,     /* Krylov breakdown before the tolerance */
+  HZ_ERR_SINGULAR = 7,      /* zero pivot in a plane factorisation */
+  HZ_ERR_NONFINITE = 8      /* NaN/Inf met in a residual */
+};
+
+/* operator modes */
+enum {
+  HZ_MODE_REF = 0, /* x/y PML, spectral periodic z, sponge delta-L: A = L0 - diag(deltaL) */
+  HZ_MODE_S7 = 1   /* 7-point PML stencil on every face (beta3D, functions.cpp:2242-2303) */
+};
+/* z symbol of the preconditioner planes */
+enum {
+  HZ_KZ_REFERENCE = 0, /* FGMRES.cpp:182-190 map (k < nz/2-1 -> k, else nz-k) */
+  HZ_KZ_SPECTRAL = 1,  /* min(k, nz-k) */
+  HZ_KZ_FD = 2         /* (2 - 2cos(2 pi k/nz))/hz^2 */
+};
+/* background speed of the preconditioner */
+enum {
+  HZ_C0_REFERENCE = 0, /* SetSoundSpeed2D running sum, functions.cpp:1716-1728 */
+  HZ_C0_HARMONIC = 1   /* per-column harmonic mean over the physical z range */
+};
+/* preconditioner precision */
+enum {
+  HZ_PREC_FP64 = 0, /* complex128 factors, FFT and apply */
+  HZ_PREC_C64 = 1   /* complex64 factors (FP32 arithmetic), FFT and apply */
+};
+/* Krylov options */
+enum { HZ_DRIVER_FGMRES = 0, HZ_DRIVER_GMRES_REF = 1 };
+enum { HZ_ORTHO_REF_CONJ = 0, HZ_ORTHO_MGS = 1 };
+enum { HZ_MASK_FULL = 0, HZ_MASK_PHYS = 1 };
+
+/* Problem descriptor.  Replaces the per-axis size_m structs (definitions.h:56-68)
+ * and the compile-time macros omega, c_z, nu, HOMO (definitions.h:163-329). */
+typedef struct {
+  int64_t n[3];     /* nodes per axis including layers */
+  double h[3];      /* spacing per axis (node i at (i+1) h) */
+  double l[3];      /* domain length per axis (z length sets the spectral symbol) */
+  int32_t pml[3];   /* PML nodes per side (alpha profile, functions.cpp:1839-1878) */
+  int32_t spg[3];   /* sponge nodes per side (REF: z only; delta-L taper and phys mask) */
+  double omega;     /* angular frequency */
+  double beta;      /* complex shift of the preconditioner (beta_eq) */
+  double pml_c;     /* d(x) = pml_c x^2 (1000 in functions.cpp:893-906) */
+  int32_t mode;     /* HZ_MODE_* */
+  int32_t kz_symbol;/* HZ_KZ_* */
+  int32_t c0_rule;  /* HZ_C0_* */
+  int32_t reserved;
+  double c_bg;      /* > 0: constant background speed of the planes (reference HOMO) */
+} hz_desc;
+
+typedef struct hz_problem hz_problem;
+
+typedef struct {
+  int32_t driver;       /* HZ_DRIVER_* */
+  int32_t m;            /* restart length (FGMRES niter, main.cpp:150) */
+  int32_t max_restarts; /* restart count (main.cpp:151) */
+  int32_t ortho;        /* GMRES_REF: HZ_ORTHO_* */
+  int32_t mask;         /* HZ_MASK_* residual box */
+  int32_t monitor;      /* GMRES_REF: the reference's per-iteration u = L0^{-1} x (FGMRES.cpp:601) */
+  double tol;           /* relative residual target */
+  double breakdown;     /* |h_{j+1,j}| threshold (FGMRES.cpp:522) */
+  double* hist;         /* optional host array of per-iteration residuals (may be NULL) */
+  int32_t hist_cap;
+  int32_t prec;         /* HZ_PREC_* used by the driver's preconditioner applies */
+} hz_krylov_opts;
+
+typedef struct {
+  int32_t iters;     /* inner iterations (one preconditioned operator apply each) */
+  int32_t restarts;  /* restart cycles started */
+  int32_t converged;
+  int32_t breakdown;
+  double rel_res;    /* final true relative residual over the mask */
+  double res;        /* final absolute residual over the mask */
+  double diff_sol;   /* GMRES_REF monitor ||u_k - u_{k-1}||/||u_{k-1}|| (FGMRES.cpp:644) */
+  int32_t n_hist;
+  int32_t reserved;
+  double t_solve_s;  /* wall time of the Krylov loop */
+} hz_krylov_stats;
+
+/* ---------------------------------------------------------------- library */
+const char* hz_last_error(void);
+int hz_version(void);
+
+/* ---------------------------------------------------------------- problem */
+/* Set up one frequency: PML tables (alpha, functions.cpp:1839-1878;
+ * beta2D_pml / beta3D, functions.cpp:2242-2402), background c0 (SetSoundSpeed2D,
+ * functions.cpp:1686-1738), delta-L (GenerateDeltaL, functions.cpp:1740-1824)
+ * and the device copies of the medium.  c_host: N sound speeds (host memory).
+ * Replaces the setup block of FGMRES, FGMRES.cpp:72-101. */
+int hz_problem_create(const hz_desc* desc, const double* c_host, hz_problem** out);
+int hz_problem_destroy(hz_problem* p);
+/* Use an external HIP stream (hipStream_t) for every later call; NULL = own stream. */
+int hz_problem_set_stream(hz_problem* p, void* hip_stream);
+int hz_problem_sync(hz_problem* p);
+/* The HIP stream (hipStream_t) every call of this problem is ordered on. */
+void* hz_problem_stream(const hz_problem* p);
+/* dims[0..3] = nx, ny, nz, N; *factor_bytes = bytes held by the plane factors. */
+int hz_problem_info(const hz_problem* p, int64_t* dims, double* factor_bytes);
+
+/* ------------------------------------------------------- preconditioner */
+/* Factor every kz plane M_k = D_xy + kappa - kz_k^2 as a block-tridiagonal
+ * Schur recursion over y-lines with explicit nx x nx inverses (Gauss-Jordan,
+ * partial pivoting).  Replaces the per-plane PARDISO phases 11+22,
+ * FGMRES.cpp:176-298 (and DirFactFastDiagStructOnline,
+ * HODLR/BinaryTreesHODLR.cpp:2870-2969). */
+int hz_precond_factor(hz_problem* p, int prec);
+/* out = L0^{-1} in = F_z^{-1} diag(M_k^{-1}) F_z in, in precision `prec`
+ * (must have been factored).  Replaces Solve3DSparseUsingFT_CSR,
+ * functions.cpp:5837-6061. */
+int hz_precond_apply(hz_problem* p, int prec, int nrhs, const hz_c128* in, hz_c128* out);
+/* out = (L0^{-1})^H in, with the conjugate-transposed plane factors
+ * (no reference counterpart: adjoint solves of the misfit gradient). */
+int hz_precond_apply_adjoint(hz_problem* p, int prec, int nrhs, const hz_c128* in, hz_c128* out);
+
+/* ---------------------------------------------------------- operators */
+/* out = L0 in.  Replaces Multiply3DSparseUsingFT, functions.cpp:6247-6318. */
+int hz_L0_apply(hz_problem* p, int nrhs, const hz_c128* in, hz_c128* out);
+/* out = A in, the true operator: REF = L0 - diag(deltaL); S7 = 7-point PML
+ * stencil + omega^2/c^2 (beta3D, functions.cpp:2242-2303; ComputeResidual,
+ * functions.cpp:6446-6504). */
+int hz_op_apply(hz_problem* p, int nrhs, const hz_c128* in, hz_c128* out);
+/* out = A^H in. */
+int hz_op_apply_adjoint(hz_problem* p, int nrhs, const hz_c128* in, hz_c128* out);
+/* out = A L0^{-1} in (REF: in - deltaL * L0^{-1} in).  Replaces
+ * ApplyCoeffMatrixA_CSR, functions.cpp:5554-5575. */
+int hz_op_apply_preconditioned(hz_problem* p, int prec, int nrhs, const hz_c128* in, hz_c128* out);
+/* r = f - A u and rel[q] = ||mask(r_q)|| / ||f_q|| (host array of nrhs). */
+int hz_residual(hz_problem* p, int nrhs, const hz_c128* f, const hz_c128* u, hz_c128* r, int mask,
+                double* rel);
+
+/* ------------------------------------------------------ Krylov drivers */
+/* Solve A u = f for nrhs right-hand sides (stats: host array of nrhs).
+ *  HZ_DRIVER_FGMRES:    flexible GMRES, MGS, complex Givens, FP64 true-operator
+ *                       matvec and residual, preconditioner in opts->prec;
+ *  HZ_DRIVER_GMRES_REF: the reference's restarted GMRES on (I - deltaL L0^{-1}),
+ *                       zgels least squares, per-iteration true residual over
+ *                       the mask, u = L0^{-1} x (FGMRES.cpp:6-725).
+ * Replaces FGMRES (templates.h:59). */
+int hz_fgmres(hz_problem* p, int nrhs, const hz_c128* f, hz_c128* u, const hz_krylov_opts* opts,
+              hz_krylov_stats* stats);
+/* Right-preconditioned BiCGStab with shadow residual 2 r0; stops when the
+ * residual over the mask (absolute when tol_abs, else relative) < tol.
+ * Replaces BCGSTAB (templates.h:60; BcGSTAB.cpp:6-666). */
+int hz_bicgstab(hz_problem* p, int prec, int nrhs, const hz_c128* f, hz_c128* u, int maxit, double tol,
+                int tol_abs, int mask, hz_krylov_stats* stats);
+
+/* ------------------------------------------------------------- adjoint */
+/* Receiver misfit J = 1/2 sum_s ||P u_s - d_s||^2 (S7 mode) for nsrc sources
+ * f (device, nsrc*N), receivers rec (host, nrec flat indices) and data d
+ * (host, nsrc*nrec).  Forward fields are kept on the device when
+ * store_fields != 0, else recomputed by hz_misfit_backward.  No reference
+ * counterpart. */
+int hz_misfit_forward(hz_problem* p, int nsrc, const hz_c128* f, int nrec, const int64_t* rec,
+                      const hz_c128* data, const hz_krylov_opts* opts, int store_fields, double* J);
+/* grad (device, N doubles) = dJ/dm, m = 1/c^2:
+ *   A^H lambda_s = P^T (P u_s - d_s),  grad = -omega^2 sum_s Re(conj(u_s) lambda_s). */
+int hz_misfit_backward(hz_problem* p, double* grad);
+
+/* ------------------------------------------------------ stage hooks */
+/* z-DFT of every (x,y) line: sign -1 forward (unscaled), +1 backward (x 1/nz),
+ * computed in `prec`; in/out device c128.  The DftiCompute loops of
+ * functions.cpp:5876-5881, 6041-6046. */
+int hz_fft_z(hz_problem* p, int prec, int sign, int nrhs, const hz_c128* in, hz_c128* out);
+/* In-place plane solves M_k x_k = b_k (adjoint: M_k^H) on z-transformed data,
+ * x: nrhs * N complex of the factor precision (complex128 for HZ_PREC_FP64,
+ * complex64 for HZ_PREC_C64).  The per-plane PARDISO phase-33 loop of
+ * functions.cpp:5941-6028. */
+int hz_plane_solve(hz_problem* p, int prec, int adjoint, int nrhs, void* x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HZ_H */

@@ -1,0 +1,143 @@
+"""ctypes binding of libhz.so (include/hz.h).  Loading fails loudly when the
+library has not been built: there is no CPU fallback."""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhz.so")
+
+HZ_OK = 0
+ERRORS = {1: "HZ_ERR_ARG", 2: "HZ_ERR_HIP", 3: "HZ_ERR_STATE", 4: "HZ_ERR_NOMEM", 5: "HZ_ERR_NOT_CONVERGED",
+          6: "HZ_ERR_BREAKDOWN", 7: "HZ_ERR_SINGULAR", 8: "HZ_ERR_NONFINITE"}
+PREC_FP64, PREC_C64 = 0, 1
+DRIVER_FGMRES, DRIVER_GMRES_REF = 0, 1
+ORTHO_REF_CONJ, ORTHO_MGS = 0, 1
+MASK_FULL, MASK_PHYS = 0, 1
+
+
+class HzDesc(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64 * 3),
+        ("h", C.c_double * 3),
+        ("l", C.c_double * 3),
+        ("pml", C.c_int32 * 3),
+        ("spg", C.c_int32 * 3),
+        ("omega", C.c_double),
+        ("beta", C.c_double),
+        ("pml_c", C.c_double),
+        ("mode", C.c_int32),
+        ("kz_symbol", C.c_int32),
+        ("c0_rule", C.c_int32),
+        ("reserved", C.c_int32),
+        ("c_bg", C.c_double),
+    ]
+
+
+class KrylovOpts(C.Structure):
+    _fields_ = [
+        ("driver", C.c_int32),
+        ("m", C.c_int32),
+        ("max_restarts", C.c_int32),
+        ("ortho", C.c_int32),
+        ("mask", C.c_int32),
+        ("monitor", C.c_int32),
+        ("tol", C.c_double),
+        ("breakdown", C.c_double),
+        ("hist", C.POINTER(C.c_double)),
+        ("hist_cap", C.c_int32),
+        ("prec", C.c_int32),
+    ]
+
+
+class KrylovStats(C.Structure):
+    _fields_ = [
+        ("iters", C.c_int32),
+        ("restarts", C.c_int32),
+        ("converged", C.c_int32),
+        ("breakdown", C.c_int32),
+        ("rel_res", C.c_double),
+        ("res", C.c_double),
+        ("diff_sol", C.c_double),
+        ("n_hist", C.c_int32),
+        ("reserved", C.c_int32),
+        ("t_solve_s", C.c_double),
+    ]
+
+
+# every extern "C" entry point of include/hz.h with its argument types
+_vp, _i, _i64, _d = C.c_void_p, C.c_int, C.c_int64, C.c_double
+_pp = C.POINTER(C.c_void_p)
+SIGNATURES = {
+    "hz_last_error": (C.c_char_p, []),
+    "hz_version": (_i, []),
+    "hz_problem_create": (_i, [C.POINTER(HzDesc), C.POINTER(C.c_double), _pp]),
+    "hz_problem_destroy": (_i, [_vp]),
+    "hz_problem_set_stream": (_i, [_vp, _vp]),
+    "hz_problem_sync": (_i, [_vp]),
+    "hz_problem_stream": (_vp, [_vp]),
+    "hz_problem_info": (_i, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
+    "hz_precond_factor": (_i, [_vp, _i]),
+    "hz_precond_apply": (_i, [_vp, _i, _i, _vp, _vp]),
+    "hz_precond_apply_adjoint": (_i, [_vp, _i, _i, _vp, _vp]),
+    "hz_L0_apply": (_i, [_vp, _i, _vp, _vp]),
+    "hz_op_apply": (_i, [_vp, _i, _vp, _vp]),
+    "hz_op_apply_adjoint": (_i, [_vp, _i, _vp, _vp]),
+    "hz_op_apply_preconditioned": (_i, [_vp, _i, _i, _vp, _vp]),
+    "hz_residual": (_i, [_vp, _i, _vp, _vp, _vp, _i, C.POINTER(C.c_double)]),
+    "hz_fgmres": (_i, [_vp, _i, _vp, _vp, C.POINTER(KrylovOpts), C.POINTER(KrylovStats)]),
+    "hz_bicgstab": (_i, [_vp, _i, _i, _vp, _vp, _i, _d, _i, _i, C.POINTER(KrylovStats)]),
+    "hz_misfit_forward": (_i, [_vp, _i, _vp, _i, C.POINTER(C.c_int64), _vp, C.POINTER(KrylovOpts), _i,
+                               C.POINTER(C.c_double)]),
+    "hz_misfit_backward": (_i, [_vp, _vp]),
+    "hz_fft_z": (_i, [_vp, _i, _i, _i, _vp, _vp]),
+    "hz_plane_solve": (_i, [_vp, _i, _i, _i, _vp]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libhz.so (built by build.py)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libhz.so is not built (run __graft_entry__.build() or "
+                               "python iterative-helmholtz-solver-with-pardiso_amd/build.py)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+class HzError(RuntimeError):
+    def __init__(self, code, where):
+        self.code = code
+        msg = lib().hz_last_error().decode(errors="replace")
+        super().__init__("%s: %s (%s)" % (where, ERRORS.get(code, code), msg))
+
+
+def check(rc, where):
+    if rc != HZ_OK:
+        raise HzError(rc, where)
+    return rc
+
+
+def make_desc(d):
+    o = HzDesc()
+    for a in range(3):
+        o.n[a] = int(d.n[a])
+        o.h[a] = float(d.h[a])
+        o.l[a] = float(d.l[a])
+        o.pml[a] = int(d.pml[a])
+        o.spg[a] = int(d.spg[a])
+    o.omega = float(d.omega)
+    o.beta = float(d.beta)
+    o.pml_c = float(d.pml_c)
+    o.mode = int(d.mode)
+    o.kz_symbol = int(d.kz_symbol)
+    o.c0_rule = int(d.c0_rule)
+    o.c_bg = float(d.c_bg)
+    return o

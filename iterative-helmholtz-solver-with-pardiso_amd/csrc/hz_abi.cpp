@@ -1,0 +1,813 @@
+// hz_abi.cpp -- the extern "C" entry points of include/hz.h and the Krylov
+// drivers that sequence the device kernels.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdarg>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "hz_internal.h"
+
+namespace hz {
+
+static thread_local std::string g_err;
+
+void set_error(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+
+void DevBuf::alloc(size_t b) {
+  release();
+  if (b == 0) return;
+  hipError_t e = hipMalloc(&ptr, b);
+  if (e != hipSuccess) {
+    ptr = nullptr;
+    set_error("hipMalloc(%zu bytes): %s", b, hipGetErrorString(e));
+    throw Error{HZ_ERR_NOMEM};
+  }
+  bytes = b;
+}
+
+void DevBuf::release() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+
+typedef std::complex<double> zc;
+
+static inline double2 d2(zc v) { return make_double2(v.real(), v.imag()); }
+static inline zc zz(double2 v) { return zc(v.x, v.y); }
+
+static void grow(DevBuf& b, size_t bytes) {
+  if (b.bytes < bytes) b.alloc(bytes);
+}
+
+static void ensure_work(hz_problem* p, int nrhs) {
+  const size_t b = (size_t)p->N * nrhs * sizeof(double2);
+  grow(p->w_a, b);
+  grow(p->w_b, b);
+  grow(p->w_c, b);
+  grow(p->scal, 4096 * sizeof(double2));
+}
+
+static void sync(hz_problem* p) { HZ_HIP(hipStreamSynchronize(p->stream)); }
+
+// ------------------------------------------------------------ operators
+// out = L0^{-1} in (or its adjoint).  epi: REF right-preconditioned operator
+// out = in - deltaL * L0^{-1} in fused into the backward transform.
+static void precond_apply(hz_problem* p, int prec, int nrhs, const double2* in, double2* out, bool adjoint,
+                          bool epi) {
+  HZ_CHECK(prec == HZ_PREC_FP64 || prec == HZ_PREC_C64, HZ_ERR_ARG, "bad precision %d", prec);
+  HZ_CHECK(p->fac[prec].ready, HZ_ERR_STATE, "hz_precond_factor(prec=%d) has not run", prec);
+  ensure_work(p, nrhs);
+  const double2* dL = p->d_dL.as<double2>();
+  if (prec == HZ_PREC_FP64) {
+    launch_fft_z(p, prec, -1, nrhs, in, 0, p->w_a.ptr, 0, 0, nullptr, nullptr, p->stream);
+    launch_plane_solve(p, prec, nrhs, p->w_a.ptr, adjoint, p->stream);
+    launch_fft_z(p, prec, +1, nrhs, p->w_a.ptr, 0, out, 0, epi ? 1 : 0, in, dL, p->stream);
+  } else {
+    launch_fft_z(p, prec, -1, nrhs, in, 0, p->w_a.ptr, 1, 0, nullptr, nullptr, p->stream);
+    launch_plane_solve(p, prec, nrhs, p->w_a.ptr, adjoint, p->stream);
+    launch_fft_z(p, prec, +1, nrhs, p->w_a.ptr, 1, out, 0, epi ? 1 : 0, in, dL, p->stream);
+  }
+}
+
+static void L0_apply(hz_problem* p, int nrhs, const double2* in, double2* out, bool adjoint) {
+  ensure_work(p, nrhs);
+  launch_fft_z(p, HZ_PREC_FP64, -1, nrhs, in, 0, p->w_a.ptr, 0, 0, nullptr, nullptr, p->stream);
+  launch_plane_matvec(p, nrhs, p->w_a.as<double2>(), p->w_b.as<double2>(), adjoint, p->stream);
+  launch_fft_z(p, HZ_PREC_FP64, +1, nrhs, p->w_b.ptr, 0, out, 0, 0, nullptr, nullptr, p->stream);
+}
+
+// out = A in (true operator)
+static void op_apply(hz_problem* p, int nrhs, const double2* in, double2* out, bool adjoint) {
+  if (p->d.mode == HZ_MODE_S7) {
+    launch_stencil(p, nrhs, in, out, adjoint, nullptr, p->stream);
+    return;
+  }
+  L0_apply(p, nrhs, in, out, adjoint);
+  // out -= dL .* in  (conj(dL) for the adjoint)
+  const double2* dL = adjoint ? p->d_dLc.as<double2>() : p->d_dL.as<double2>();
+  for (int q = 0; q < nrhs; ++q)
+    launch_w_minus_dl(p->N, out + (size_t)q * p->N, dL, in + (size_t)q * p->N, out + (size_t)q * p->N, p->stream);
+}
+
+// r = f - A u
+static void residual(hz_problem* p, int nrhs, const double2* f, const double2* u, double2* r, bool adjoint) {
+  if (p->d.mode == HZ_MODE_S7) {
+    launch_stencil(p, nrhs, u, r, adjoint, f, p->stream);
+    return;
+  }
+  op_apply(p, nrhs, u, r, adjoint);
+  launch_sub((int64_t)p->N * nrhs, f, r, r, p->stream);
+}
+
+// out = A L0^{-1} in
+static void op_pre_apply(hz_problem* p, int prec, int nrhs, const double2* in, double2* out) {
+  if (p->d.mode == HZ_MODE_REF) {
+    precond_apply(p, prec, nrhs, in, out, false, true);
+    return;
+  }
+  ensure_work(p, nrhs);
+  precond_apply(p, prec, nrhs, in, p->w_c.as<double2>(), false, false);
+  launch_stencil(p, nrhs, p->w_c.as<double2>(), out, false, nullptr, p->stream);
+}
+
+// ------------------------------------------------------------ reductions
+static double2* dscal(hz_problem* p, int i) { return p->scal.as<double2>() + i; }
+
+static zc dot_host(hz_problem* p, const double2* x, const double2* y, int mask = HZ_MASK_FULL) {
+  launch_dot(p, p->N, x, y, mask, dscal(p, 0), p->stream);
+  double2 h;
+  HZ_HIP(hipMemcpyAsync(&h, dscal(p, 0), sizeof(double2), hipMemcpyDeviceToHost, p->stream));
+  sync(p);
+  return zz(h);
+}
+
+static double nrm_host(hz_problem* p, const double2* x, int mask = HZ_MASK_FULL) {
+  launch_dot(p, p->N, x, nullptr, mask, dscal(p, 0), p->stream);
+  double2 h;
+  HZ_HIP(hipMemcpyAsync(&h, dscal(p, 0), sizeof(double2), hipMemcpyDeviceToHost, p->stream));
+  sync(p);
+  return h.x;
+}
+
+// Householder least squares min ||b - H y|| for an m x n (m >= n) column-major
+// H with leading dimension ld (the zgels step of FGMRES.cpp:563-569).
+static void lsq(int m, int n, std::vector<zc>& H, int ld, std::vector<zc>& b) {
+  for (int j = 0; j < n; ++j) {
+    double s2 = 0.0;
+    for (int i = j; i < m; ++i) s2 += std::norm(H[i + ld * j]);
+    double nr = std::sqrt(s2);
+    if (nr == 0.0) continue;
+    zc x0 = H[j + ld * j];
+    zc ph = std::abs(x0) == 0.0 ? zc(1.0, 0.0) : x0 / std::abs(x0);
+    std::vector<zc> v(m - j);
+    for (int i = j; i < m; ++i) v[i - j] = H[i + ld * j];
+    v[0] += ph * nr;  // v = x - alpha e1 with alpha = -ph * nr
+    double vn = 0.0;
+    for (auto& e : v) vn += std::norm(e);
+    if (vn == 0.0) continue;
+    for (int c = j; c < n; ++c) {
+      zc s(0, 0);
+      for (int i = j; i < m; ++i) s += std::conj(v[i - j]) * H[i + ld * c];
+      s *= 2.0 / vn;
+      for (int i = j; i < m; ++i) H[i + ld * c] -= s * v[i - j];
+    }
+    zc s(0, 0);
+    for (int i = j; i < m; ++i) s += std::conj(v[i - j]) * b[i];
+    s *= 2.0 / vn;
+    for (int i = j; i < m; ++i) b[i] -= s * v[i - j];
+  }
+  for (int j = n - 1; j >= 0; --j) {
+    zc s = b[j];
+    for (int c = j + 1; c < n; ++c) s -= H[j + ld * c] * b[c];
+    b[j] = s / H[j + ld * j];
+  }
+}
+
+static void ensure_krylov(hz_problem* p, int m) {
+  const size_t vb = (size_t)p->N * sizeof(double2);
+  grow(p->kV, vb * (m + 1));
+  grow(p->kZ, vb * m);
+  grow(p->kr, vb);
+  grow(p->kw, vb);
+  grow(p->kx, vb);
+  grow(p->kaux, vb);
+  grow(p->scal, std::max<size_t>(4096, (size_t)(m + 2) * (m + 2) + 256) * sizeof(double2));
+}
+
+struct HistW {
+  double* buf;
+  int cap;
+  int n = 0;
+  void push(double v) {
+    if (buf && n < cap) buf[n] = v;
+    ++n;
+  }
+};
+
+// Flexible GMRES (MGS, complex Givens) on A u = f, right preconditioner
+// L0^{-1} in `prec`, FP64 Krylov vectors; x (device) holds the solution.
+static void fgmres_one(hz_problem* p, const double2* f, double2* x, const hz_krylov_opts& o, hz_krylov_stats& st,
+                       bool adjoint) {
+  const int m = o.m;
+  const int64_t N = p->N;
+  ensure_krylov(p, m);
+  double2* V = p->kV.as<double2>();
+  double2* Z = p->kZ.as<double2>();
+  double2* w = p->kw.as<double2>();
+  double2* r = p->kr.as<double2>();
+  double2* dH = dscal(p, 64);  // (m+1) x m column-major
+  const int ldh = m + 1;
+  double2* dY = dH + (size_t)ldh * m;
+  HistW hist{o.hist, o.hist_cap};
+  memset(&st, 0, sizeof(st));
+  auto t0 = std::chrono::steady_clock::now();
+  const double norm_f = (nrm_host(p, f, HZ_MASK_FULL));
+  launch_fill(N, make_double2(0, 0), x, p->stream);
+  if (norm_f == 0.0) {
+    st.converged = 1;
+    sync(p);
+    return;
+  }
+  std::vector<zc> H((size_t)ldh * m), cs(m), sn(m), g(m + 1), y(m);
+  std::vector<double2> hcol(m + 2), yd(m);
+  double rel = 0.0, res = 0.0;
+  int total = 0;
+  bool conv = false, brk = false;
+  for (int cyc = 0; cyc < o.max_restarts; ++cyc) {
+    residual(p, 1, f, x, r, adjoint);
+    res = (nrm_host(p, r, o.mask));
+    rel = res / norm_f;
+    if (!std::isfinite(rel)) {
+      set_error("non-finite residual in FGMRES");
+      throw Error{HZ_ERR_NONFINITE};
+    }
+    if (rel < o.tol) { conv = true; break; }
+    st.restarts++;
+    launch_dot(p, N, r, nullptr, HZ_MASK_FULL, dscal(p, 1), p->stream);  // beta^2 (sqrt on final)
+    double2 hb;
+    HZ_HIP(hipMemcpyAsync(&hb, dscal(p, 1), sizeof(double2), hipMemcpyDeviceToHost, p->stream));
+    sync(p);
+    const double beta = hb.x;  // k_dot_final returns the norm in mode 1
+    launch_scale_inv_dev(N, dscal(p, 1), r, V, p->stream);
+    std::fill(g.begin(), g.end(), zc(0, 0));
+    g[0] = beta;
+    int jj = 0;
+    for (int j = 0; j < m; ++j) {
+      double2* Vj = V + (size_t)N * j;
+      double2* Zj = Z + (size_t)N * j;
+      precond_apply(p, o.prec, 1, Vj, Zj, adjoint, false);
+      op_apply(p, 1, Zj, w, adjoint);
+      total++;
+      for (int i = 0; i <= j; ++i) {
+        double2* hij = dH + i + (size_t)ldh * j;
+        launch_dot(p, N, V + (size_t)N * i, w, HZ_MASK_FULL, hij, p->stream);
+        launch_axpy_dev(N, hij, true, false, V + (size_t)N * i, w, p->stream);
+      }
+      double2* hn_d = dH + (j + 1) + (size_t)ldh * j;
+      launch_dot(p, N, w, nullptr, HZ_MASK_FULL, hn_d, p->stream);
+      HZ_HIP(hipMemcpyAsync(hcol.data(), dH + (size_t)ldh * j, (j + 2) * sizeof(double2), hipMemcpyDeviceToHost,
+                            p->stream));
+      sync(p);
+      for (int i = 0; i <= j + 1; ++i) H[i + (size_t)ldh * j] = zz(hcol[i]);
+      const double hn = hcol[j + 1].x;
+      for (int i = 0; i < j; ++i) {
+        zc a = H[i + (size_t)ldh * j], b = H[i + 1 + (size_t)ldh * j];
+        H[i + (size_t)ldh * j] = cs[i] * a + sn[i] * b;
+        H[i + 1 + (size_t)ldh * j] = -std::conj(sn[i]) * a + cs[i] * b;
+      }
+      zc a = H[j + (size_t)ldh * j], b = H[j + 1 + (size_t)ldh * j];
+      const double aa = std::abs(a), bb = std::abs(b);
+      if (bb == 0.0) { cs[j] = 1.0; sn[j] = 0.0; }
+      else if (aa == 0.0) { cs[j] = 0.0; sn[j] = std::conj(b) / bb; }
+      else {
+        const double t = std::sqrt(aa * aa + bb * bb);
+        cs[j] = aa / t;
+        sn[j] = (a / aa) * std::conj(b) / t;
+      }
+      H[j + (size_t)ldh * j] = cs[j] * a + sn[j] * b;
+      H[j + 1 + (size_t)ldh * j] = 0.0;
+      g[j + 1] = -std::conj(sn[j]) * g[j];
+      g[j] = cs[j] * g[j];
+      jj = j + 1;
+      const double est = std::abs(g[j + 1]) / norm_f;
+      hist.push(est);
+      if (!std::isfinite(est)) {
+        set_error("non-finite Arnoldi coefficient in FGMRES");
+        throw Error{HZ_ERR_NONFINITE};
+      }
+      if (est < o.tol) break;
+      if (hn <= o.breakdown) { brk = true; break; }
+      launch_scale_inv_dev(N, hn_d, w, V + (size_t)N * (j + 1), p->stream);
+    }
+    for (int i = jj - 1; i >= 0; --i) {
+      zc s = g[i];
+      for (int c = i + 1; c < jj; ++c) s -= H[i + (size_t)ldh * c] * y[c];
+      y[i] = s / H[i + (size_t)ldh * i];
+    }
+    for (int i = 0; i < jj; ++i) yd[i] = d2(y[i]);
+    HZ_HIP(hipMemcpyAsync(dY, yd.data(), jj * sizeof(double2), hipMemcpyHostToDevice, p->stream));
+    launch_lincomb(N, jj, dY, Z, N, x, true, p->stream);
+    sync(p);
+    if (brk) {
+      st.breakdown = 1;
+      residual(p, 1, f, x, r, adjoint);
+      res = (nrm_host(p, r, o.mask));
+      rel = res / norm_f;
+      conv = rel < o.tol;
+      break;
+    }
+  }
+  if (!conv && !brk) {
+    residual(p, 1, f, x, r, adjoint);
+    res = (nrm_host(p, r, o.mask));
+    rel = res / norm_f;
+    conv = rel < o.tol;
+  }
+  st.iters = total;
+  st.converged = conv;
+  st.rel_res = rel;
+  st.res = res;
+  st.n_hist = hist.n;
+  st.t_solve_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// The reference's restarted GMRES on (I - deltaL L0^{-1}) w = f
+// (FGMRES.cpp:458-661), u = L0^{-1} w at the end.
+static void gmres_ref_one(hz_problem* p, const double2* f, double2* u, const hz_krylov_opts& o,
+                          hz_krylov_stats& st) {
+  const int m = o.m;
+  const int64_t N = p->N;
+  ensure_krylov(p, m);
+  double2* V = p->kV.as<double2>();
+  double2* w = p->kw.as<double2>();
+  double2* x0 = p->kx.as<double2>();
+  double2* xi = p->kaux.as<double2>();
+  double2* xs = p->kZ.as<double2>();   // monitor u_k (slot 0), u_{k-1} (slot 1) when m >= 2
+  double2* dH = dscal(p, 64);
+  const int ldh = m + 1;
+  double2* dY = dH + (size_t)ldh * m;
+  HistW hist{o.hist, o.hist_cap};
+  memset(&st, 0, sizeof(st));
+  auto t0 = std::chrono::steady_clock::now();
+  launch_fill(N, make_double2(0, 0), xi, p->stream);
+  launch_fill(N, make_double2(0, 0), x0, p->stream);
+  const double norm_f = (nrm_host(p, f, HZ_MASK_FULL));
+  std::vector<zc> H((size_t)ldh * m, zc(0, 0)), Hg, eb(m + 1);
+  std::vector<double2> hcol(m + 2), yd(m);
+  double Res = 0.0, RelRes = 0.0, diff = 0.0;
+  bool finish = false, have_prev = false;
+  int total = 0;
+  for (int rs = 0; rs < o.max_restarts && !finish; ++rs) {
+    st.restarts++;
+    HZ_HIP(hipMemcpyAsync(x0, xi, N * sizeof(double2), hipMemcpyDeviceToDevice, p->stream));
+    op_pre_apply(p, o.prec, 1, x0, w);
+    launch_sub(N, f, w, w, p->stream);  // r0 = f - A x0
+    launch_dot(p, N, w, nullptr, HZ_MASK_FULL, dscal(p, 1), p->stream);
+    double2 hb;
+    HZ_HIP(hipMemcpyAsync(&hb, dscal(p, 1), sizeof(double2), hipMemcpyDeviceToHost, p->stream));
+    sync(p);
+    const double beta = hb.x;
+    launch_scale_inv_dev(N, dscal(p, 1), w, V, p->stream);
+    for (int j = 0; j < m; ++j) {
+      op_pre_apply(p, o.prec, 1, V + (size_t)N * j, w);
+      total++;
+      for (int i = 0; i <= j; ++i) {
+        double2* hij = dH + i + (size_t)ldh * j;
+        if (o.ortho == HZ_ORTHO_REF_CONJ)
+          launch_dot(p, N, w, V + (size_t)N * i, HZ_MASK_FULL, hij, p->stream);
+        else
+          launch_dot(p, N, V + (size_t)N * i, w, HZ_MASK_FULL, hij, p->stream);
+        launch_axpy_dev(N, hij, true, false, V + (size_t)N * i, w, p->stream);
+      }
+      double2* hn_d = dH + (j + 1) + (size_t)ldh * j;
+      launch_dot(p, N, w, nullptr, HZ_MASK_FULL, hn_d, p->stream);
+      HZ_HIP(hipMemcpyAsync(hcol.data(), dH + (size_t)ldh * j, (j + 2) * sizeof(double2), hipMemcpyDeviceToHost,
+                            p->stream));
+      sync(p);
+      for (int i = 0; i <= j + 1; ++i) H[i + (size_t)ldh * j] = zz(hcol[i]);
+      const double hn = hcol[j + 1].x;
+      if (std::fabs(hn) < o.breakdown) {
+        st.breakdown = 1;
+        break;
+      }
+      launch_scale_inv_dev(N, hn_d, w, V + (size_t)N * (j + 1), p->stream);
+      std::fill(eb.begin(), eb.end(), zc(0, 0));
+      eb[0] = beta;
+      Hg = H;
+      lsq(j + 2, j + 1, Hg, ldh, eb);
+      for (int i = 0; i <= j; ++i) yd[i] = d2(eb[i]);
+      HZ_HIP(hipMemcpyAsync(dY, yd.data(), (j + 1) * sizeof(double2), hipMemcpyHostToDevice, p->stream));
+      HZ_HIP(hipMemcpyAsync(x0, xi, N * sizeof(double2), hipMemcpyDeviceToDevice, p->stream));
+      launch_lincomb(N, j + 1, dY, V, N, x0, true, p->stream);
+      op_pre_apply(p, o.prec, 1, x0, w);
+      launch_sub(N, w, f, w, p->stream);
+      Res = (nrm_host(p, w, o.mask));
+      RelRes = Res / norm_f;
+      hist.push(RelRes);
+      if (o.monitor && m >= 2) {
+        double2* cur = xs + (size_t)N * (total % 2);
+        double2* prev = xs + (size_t)N * ((total + 1) % 2);
+        precond_apply(p, o.prec, 1, x0, cur, false, false);
+        if (have_prev) {
+          launch_sub(N, cur, prev, w, p->stream);
+          const double a = (nrm_host(p, w, HZ_MASK_PHYS));
+          const double b = (nrm_host(p, prev, HZ_MASK_PHYS));
+          diff = a / b;
+        }
+        have_prev = true;
+      }
+      if (!std::isfinite(RelRes)) {
+        set_error("non-finite residual in GMRES");
+        throw Error{HZ_ERR_NONFINITE};
+      }
+      if (RelRes < o.tol) {
+        finish = true;
+        break;
+      }
+    }
+    HZ_HIP(hipMemcpyAsync(xi, x0, N * sizeof(double2), hipMemcpyDeviceToDevice, p->stream));
+  }
+  precond_apply(p, o.prec, 1, x0, u, false, false);
+  sync(p);
+  st.iters = total;
+  st.converged = finish;
+  st.rel_res = RelRes;
+  st.res = Res;
+  st.diff_sol = diff;
+  st.n_hist = hist.n;
+  st.t_solve_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// BiCGStab of BcGSTAB.cpp:397-568 (w-space recurrences on A L0^{-1}, x in
+// u-space).  The residual uses the true operator A x - f directly.
+static void bicgstab_one(hz_problem* p, int prec, const double2* f, double2* x, int maxit, double tol, int tol_abs,
+                         int mask, hz_krylov_stats& st) {
+  const int64_t N = p->N;
+  ensure_krylov(p, 8);
+  double2* V = p->kV.as<double2>();
+  double2 *p0 = V, *s0 = V + N, *yn = V + 2 * N, *zn = V + 3 * N, *Ayn = V + 4 * N, *Azn = V + 5 * N,
+          *r0 = V + 6 * N, *rst = V + 7 * N, *w = p->kw.as<double2>();
+  memset(&st, 0, sizeof(st));
+  st.restarts = 1;
+  auto t0 = std::chrono::steady_clock::now();
+  const double norm_f = (nrm_host(p, f, HZ_MASK_FULL));
+  launch_fill(N, make_double2(0, 0), x, p->stream);
+  HZ_HIP(hipMemcpyAsync(r0, f, N * sizeof(double2), hipMemcpyDeviceToDevice, p->stream));  // f - A_pre(0)
+  HZ_HIP(hipMemcpyAsync(p0, r0, N * sizeof(double2), hipMemcpyDeviceToDevice, p->stream));
+  launch_fill(N, make_double2(0, 0), rst, p->stream);
+  launch_axpby(N, make_double2(2.0, 0), r0, make_double2(1.0, 0), rst, p->stream);
+  const bool ref = p->d.mode == HZ_MODE_REF;
+  auto apply_pre = [&](const double2* in, double2* pre, double2* out) {
+    precond_apply(p, prec, 1, in, pre, false, false);
+    if (ref) launch_w_minus_dl(N, in, p->d_dL.as<double2>(), pre, out, p->stream);
+    else launch_stencil(p, 1, pre, out, false, nullptr, p->stream);
+  };
+  double Res = 0, RelRes = 0;
+  int it = 0;
+  bool conv = false;
+  for (int j = 0; j < maxit; ++j) {
+    apply_pre(p0, yn, Ayn);
+    zc alpha = -(dot_host(p, rst, r0) / dot_host(p, rst, Ayn));
+    launch_axpbypcz(N, r0, d2(alpha), Ayn, make_double2(0, 0), Ayn, s0, p->stream);
+    apply_pre(s0, zn, Azn);
+    zc om = -(dot_host(p, Azn, s0) / dot_host(p, Azn, Azn));
+    // r_{n+1} = s0 + om Azn  (into w)
+    launch_axpbypcz(N, s0, d2(om), Azn, make_double2(0, 0), Azn, w, p->stream);
+    alpha = -alpha;
+    om = -om;
+    launch_axpbypcz(N, x, d2(alpha), yn, d2(om), zn, x, p->stream);
+    zc beta = (alpha / om) * (dot_host(p, rst, w) / dot_host(p, rst, r0));
+    om = -om;
+    // p = rn + beta (p + om Ayn)
+    launch_axpby(N, d2(om * beta), Ayn, d2(beta), p0, p->stream);
+    launch_axpby(N, make_double2(1, 0), w, make_double2(1, 0), p0, p->stream);
+    HZ_HIP(hipMemcpyAsync(r0, w, N * sizeof(double2), hipMemcpyDeviceToDevice, p->stream));
+    it++;
+    residual(p, 1, f, x, w, false);
+    Res = (nrm_host(p, w, mask));
+    RelRes = Res / norm_f;
+    if (!std::isfinite(RelRes)) {
+      set_error("non-finite residual in BiCGStab");
+      throw Error{HZ_ERR_NONFINITE};
+    }
+    if ((tol_abs ? Res : RelRes) < tol) {
+      conv = true;
+      break;
+    }
+  }
+  sync(p);
+  st.iters = it;
+  st.converged = conv;
+  st.res = Res;
+  st.rel_res = RelRes;
+  st.t_solve_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace hz
+
+using namespace hz;
+
+#define HZ_API_BEGIN try {
+#define HZ_API_END                                    \
+  }                                                   \
+  catch (const hz::Error& e) {                        \
+    return e.code;                                    \
+  }                                                   \
+  catch (const std::exception& e) {                   \
+    hz::set_error("%s", e.what());                    \
+    return HZ_ERR_ARG;                                \
+  }                                                   \
+  return HZ_OK;
+
+extern "C" {
+
+const char* hz_last_error(void) { return g_err.c_str(); }
+int hz_version(void) { return 1; }
+
+int hz_problem_create(const hz_desc* d, const double* c_host, hz_problem** out) {
+  HZ_API_BEGIN
+  HZ_CHECK(d && c_host && out, HZ_ERR_ARG, "null argument");
+  for (int a = 0; a < 3; ++a) {
+    HZ_CHECK(d->n[a] >= 1 && d->h[a] > 0.0, HZ_ERR_ARG, "axis %d: n=%lld h=%g", a, (long long)d->n[a], d->h[a]);
+    HZ_CHECK(d->pml[a] >= 0 && d->spg[a] >= 0 && 2 * (d->pml[a] + d->spg[a]) < d->n[a], HZ_ERR_ARG,
+             "axis %d: layers (pml %d, sponge %d) do not fit n=%lld", a, d->pml[a], d->spg[a], (long long)d->n[a]);
+  }
+  HZ_CHECK(d->mode == HZ_MODE_REF || d->mode == HZ_MODE_S7, HZ_ERR_ARG, "bad mode %d", d->mode);
+  HZ_CHECK(d->n[0] <= 512, HZ_ERR_ARG, "nx=%lld exceeds the 512-node block limit", (long long)d->n[0]);
+  HZ_CHECK(d->omega > 0.0, HZ_ERR_ARG, "omega must be positive");
+  if (d->mode == HZ_MODE_REF) HZ_CHECK(d->spg[2] >= 1, HZ_ERR_ARG, "REF mode needs a z sponge (spg[2] >= 1)");
+  hz_problem* p = new hz_problem();
+  try {
+    p->d = *d;
+    p->nx = d->n[0]; p->ny = d->n[1]; p->nz = d->n[2];
+    p->n2 = p->nx * p->ny;
+    p->N = p->n2 * p->nz;
+    HZ_HIP(hipGetDevice(&p->device));
+    HZ_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+    p->own_stream = true;
+    setup_tables(p, c_host);
+    if (d->mode == HZ_MODE_REF) {
+      // conjugated delta-L for the adjoint operator
+      std::vector<double2> dl(p->N);
+      HZ_HIP(hipMemcpy(dl.data(), p->d_dL.ptr, p->N * sizeof(double2), hipMemcpyDeviceToHost));
+      for (auto& e : dl) e.y = -e.y;
+      p->d_dLc.alloc(p->N * sizeof(double2));
+      HZ_HIP(hipMemcpy(p->d_dLc.ptr, dl.data(), p->N * sizeof(double2), hipMemcpyHostToDevice));
+    }
+    grow(p->scal, 4096 * sizeof(double2));
+  } catch (...) {
+    if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+    throw;
+  }
+  *out = p;
+  HZ_API_END
+}
+
+int hz_problem_destroy(hz_problem* p) {
+  if (!p) return HZ_OK;
+  (void)hipStreamSynchronize(p->stream);
+  if (p->own_stream) (void)hipStreamDestroy(p->stream);
+  delete p;
+  return HZ_OK;
+}
+
+int hz_problem_set_stream(hz_problem* p, void* s) {
+  HZ_API_BEGIN
+  HZ_CHECK(p, HZ_ERR_ARG, "null problem");
+  HZ_HIP(hipStreamSynchronize(p->stream));
+  if (p->own_stream) HZ_HIP(hipStreamDestroy(p->stream));
+  p->own_stream = false;
+  p->stream = (hipStream_t)s;
+  if (!s) {
+    HZ_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+    p->own_stream = true;
+  }
+  HZ_API_END
+}
+
+int hz_problem_sync(hz_problem* p) {
+  HZ_API_BEGIN
+  HZ_CHECK(p, HZ_ERR_ARG, "null problem");
+  sync(p);
+  HZ_API_END
+}
+
+void* hz_problem_stream(const hz_problem* p) { return p ? (void*)p->stream : nullptr; }
+
+int hz_plane_solve(hz_problem* p, int prec, int adjoint, int nrhs, void* x) {
+  HZ_API_BEGIN
+  HZ_CHECK(p && x && nrhs >= 1, HZ_ERR_ARG, "bad arguments");
+  HZ_CHECK(prec == HZ_PREC_FP64 || prec == HZ_PREC_C64, HZ_ERR_ARG, "bad precision %d", prec);
+  launch_plane_solve(p, prec, nrhs, x, adjoint != 0, p->stream);
+  HZ_API_END
+}
+
+int hz_problem_info(const hz_problem* p, int64_t* dims, double* factor_bytes) {
+  HZ_API_BEGIN
+  HZ_CHECK(p, HZ_ERR_ARG, "null problem");
+  if (dims) { dims[0] = p->nx; dims[1] = p->ny; dims[2] = p->nz; dims[3] = p->N; }
+  if (factor_bytes) *factor_bytes = (double)(p->fac[0].G.bytes + p->fac[1].G.bytes);
+  HZ_API_END
+}
+
+int hz_precond_factor(hz_problem* p, int prec) {
+  HZ_API_BEGIN
+  HZ_CHECK(p, HZ_ERR_ARG, "null problem");
+  HZ_CHECK(prec == HZ_PREC_FP64 || prec == HZ_PREC_C64, HZ_ERR_ARG, "bad precision %d", prec);
+  launch_factor(p, prec, p->stream);
+  HZ_API_END
+}
+
+int hz_precond_apply(hz_problem* p, int prec, int nrhs, const hz_c128* in, hz_c128* out) {
+  HZ_API_BEGIN
+  HZ_CHECK(p && in && out && nrhs >= 1, HZ_ERR_ARG, "bad arguments");
+  precond_apply(p, prec, nrhs, (const double2*)in, (double2*)out, false, false);
+  HZ_API_END
+}
+
+int hz_precond_apply_adjoint(hz_problem* p, int prec, int nrhs, const hz_c128* in, hz_c128* out) {
+  HZ_API_BEGIN
+  HZ_CHECK(p && in && out && nrhs >= 1, HZ_ERR_ARG, "bad arguments");
+  precond_apply(p, prec, nrhs, (const double2*)in, (double2*)out, true, false);
+  HZ_API_END
+}
+
+int hz_L0_apply(hz_problem* p, int nrhs, const hz_c128* in, hz_c128* out) {
+  HZ_API_BEGIN
+  HZ_CHECK(p && in && out && nrhs >= 1, HZ_ERR_ARG, "bad arguments");
+  L0_apply(p, nrhs, (const double2*)in, (double2*)out, false);
+  HZ_API_END
+}
+
+int hz_op_apply(hz_problem* p, int nrhs, const hz_c128* in, hz_c128* out) {
+  HZ_API_BEGIN
+  HZ_CHECK(p && in && out && nrhs >= 1, HZ_ERR_ARG, "bad arguments");
+  HZ_CHECK(in != out, HZ_ERR_ARG, "hz_op_apply is out of place");
+  op_apply(p, nrhs, (const double2*)in, (double2*)out, false);
+  HZ_API_END
+}
+
+int hz_op_apply_adjoint(hz_problem* p, int nrhs, const hz_c128* in, hz_c128* out) {
+  HZ_API_BEGIN
+  HZ_CHECK(p && in && out && nrhs >= 1, HZ_ERR_ARG, "bad arguments");
+  HZ_CHECK(in != out, HZ_ERR_ARG, "hz_op_apply_adjoint is out of place");
+  op_apply(p, nrhs, (const double2*)in, (double2*)out, true);
+  HZ_API_END
+}
+
+int hz_op_apply_preconditioned(hz_problem* p, int prec, int nrhs, const hz_c128* in, hz_c128* out) {
+  HZ_API_BEGIN
+  HZ_CHECK(p && in && out && nrhs >= 1, HZ_ERR_ARG, "bad arguments");
+  HZ_CHECK(in != out, HZ_ERR_ARG, "hz_op_apply_preconditioned is out of place");
+  op_pre_apply(p, prec, nrhs, (const double2*)in, (double2*)out);
+  HZ_API_END
+}
+
+int hz_residual(hz_problem* p, int nrhs, const hz_c128* f, const hz_c128* u, hz_c128* r, int mask, double* rel) {
+  HZ_API_BEGIN
+  HZ_CHECK(p && f && u && r && nrhs >= 1, HZ_ERR_ARG, "bad arguments");
+  residual(p, nrhs, (const double2*)f, (const double2*)u, (double2*)r, false);
+  if (rel) {
+    for (int q = 0; q < nrhs; ++q) {
+      const double2* fq = (const double2*)f + (size_t)q * p->N;
+      const double2* rq = (const double2*)r + (size_t)q * p->N;
+      const double a = (nrm_host(p, rq, mask));
+      const double b = (nrm_host(p, fq, HZ_MASK_FULL));
+      rel[q] = a / b;
+    }
+  }
+  HZ_API_END
+}
+
+int hz_fgmres(hz_problem* p, int nrhs, const hz_c128* f, hz_c128* u, const hz_krylov_opts* o, hz_krylov_stats* st) {
+  HZ_API_BEGIN
+  HZ_CHECK(p && f && u && o && st && nrhs >= 1, HZ_ERR_ARG, "bad arguments");
+  HZ_CHECK(o->m >= 1 && o->m <= 2000 && o->max_restarts >= 1, HZ_ERR_ARG, "bad restart parameters");
+  int rc = HZ_OK;
+  for (int q = 0; q < nrhs; ++q) {
+    const double2* fq = (const double2*)f + (size_t)q * p->N;
+    double2* uq = (double2*)u + (size_t)q * p->N;
+    hz_krylov_opts oq = *o;
+    if (q > 0) { oq.hist = nullptr; oq.hist_cap = 0; }
+    if (o->driver == HZ_DRIVER_GMRES_REF) gmres_ref_one(p, fq, uq, oq, st[q]);
+    else fgmres_one(p, fq, uq, oq, st[q], false);
+    if (!st[q].converged && rc == HZ_OK) {
+      rc = st[q].breakdown ? HZ_ERR_BREAKDOWN : HZ_ERR_NOT_CONVERGED;
+      set_error("Krylov solve %d stopped at relative residual %.3e after %d iterations", q, st[q].rel_res,
+                st[q].iters);
+    }
+  }
+  return rc;
+  HZ_API_END
+}
+
+int hz_bicgstab(hz_problem* p, int prec, int nrhs, const hz_c128* f, hz_c128* u, int maxit, double tol, int tol_abs,
+                int mask, hz_krylov_stats* st) {
+  HZ_API_BEGIN
+  HZ_CHECK(p && f && u && st && nrhs >= 1 && maxit >= 1, HZ_ERR_ARG, "bad arguments");
+  int rc = HZ_OK;
+  for (int q = 0; q < nrhs; ++q) {
+    bicgstab_one(p, prec, (const double2*)f + (size_t)q * p->N, (double2*)u + (size_t)q * p->N, maxit, tol, tol_abs,
+                 mask, st[q]);
+    if (!st[q].converged && rc == HZ_OK) {
+      rc = HZ_ERR_NOT_CONVERGED;
+      set_error("BiCGStab %d stopped at residual %.3e after %d iterations", q, st[q].res, st[q].iters);
+    }
+  }
+  return rc;
+  HZ_API_END
+}
+
+int hz_fft_z(hz_problem* p, int prec, int sign, int nrhs, const hz_c128* in, hz_c128* out) {
+  HZ_API_BEGIN
+  HZ_CHECK(p && in && out && nrhs >= 1 && (sign == 1 || sign == -1), HZ_ERR_ARG, "bad arguments");
+  HZ_CHECK(prec == HZ_PREC_FP64 || prec == HZ_PREC_C64, HZ_ERR_ARG, "bad precision %d", prec);
+  launch_fft_z(p, prec, sign, nrhs, in, 0, out, 0, 0, nullptr, nullptr, p->stream);
+  HZ_API_END
+}
+
+// ------------------------------------------------------------- misfit
+static void scatter_rec(hz_problem* p, const std::vector<double2>& vals, double2* dst) {
+  // dst = 0; dst[rec[q]] += vals[q]  (receivers are distinct after the check in forward)
+  launch_fill(p->N, make_double2(0, 0), dst, p->stream);
+  for (size_t q = 0; q < p->rec.size(); ++q)
+    HZ_HIP(hipMemcpyAsync(dst + p->rec[q], &vals[q], sizeof(double2), hipMemcpyHostToDevice, p->stream));
+  sync(p);
+}
+
+static void gather_rec(hz_problem* p, const double2* src, std::vector<double2>& vals) {
+  vals.resize(p->rec.size());
+  for (size_t q = 0; q < p->rec.size(); ++q)
+    HZ_HIP(hipMemcpyAsync(&vals[q], src + p->rec[q], sizeof(double2), hipMemcpyDeviceToHost, p->stream));
+  sync(p);
+}
+
+int hz_misfit_forward(hz_problem* p, int nsrc, const hz_c128* f, int nrec, const int64_t* rec, const hz_c128* data,
+                      const hz_krylov_opts* o, int store_fields, double* J) {
+  HZ_API_BEGIN
+  HZ_CHECK(p && f && rec && data && o && J && nsrc >= 1 && nrec >= 1, HZ_ERR_ARG, "bad arguments");
+  HZ_CHECK(p->d.mode == HZ_MODE_S7, HZ_ERR_STATE, "the misfit gradient is defined for HZ_MODE_S7");
+  std::vector<int64_t> r(rec, rec + nrec);
+  for (auto v : r) HZ_CHECK(v >= 0 && v < p->N, HZ_ERR_ARG, "receiver index %lld out of range", (long long)v);
+  std::vector<int64_t> sorted = r;
+  std::sort(sorted.begin(), sorted.end());
+  HZ_CHECK(std::adjacent_find(sorted.begin(), sorted.end()) == sorted.end(), HZ_ERR_ARG, "duplicate receivers");
+  p->rec = r;
+  p->nsrc = nsrc;
+  p->data.assign((const double2*)data, (const double2*)data + (size_t)nsrc * nrec);
+  p->mopts = *o;
+  p->mopts.driver = HZ_DRIVER_FGMRES;
+  p->mopts.hist = nullptr;
+  p->mopts.hist_cap = 0;
+  const size_t vb = (size_t)p->N * sizeof(double2);
+  p->src_f.alloc(vb * nsrc);
+  HZ_HIP(hipMemcpyAsync(p->src_f.ptr, f, vb * nsrc, hipMemcpyDeviceToDevice, p->stream));
+  p->fields_stored = store_fields != 0;
+  if (p->fields_stored) p->fields_u.alloc(vb * nsrc);
+  else grow(p->fields_u, vb);
+  double Jt = 0.0;
+  std::vector<double2> pu;
+  for (int s = 0; s < nsrc; ++s) {
+    double2* us = p->fields_u.as<double2>() + (p->fields_stored ? (size_t)p->N * s : 0);
+    hz_krylov_stats st;
+    fgmres_one(p, p->src_f.as<double2>() + (size_t)p->N * s, us, p->mopts, st, false);
+    HZ_CHECK(st.converged, HZ_ERR_NOT_CONVERGED, "forward solve %d: relative residual %.3e", s, st.rel_res);
+    gather_rec(p, us, pu);
+    for (int q = 0; q < nrec; ++q) {
+      const double2 d = p->data[(size_t)s * nrec + q];
+      const double rx = pu[q].x - d.x, ry = pu[q].y - d.y;
+      Jt += 0.5 * (rx * rx + ry * ry);
+    }
+  }
+  p->misfit_ready = true;
+  *J = Jt;
+  HZ_API_END
+}
+
+int hz_misfit_backward(hz_problem* p, double* grad) {
+  HZ_API_BEGIN
+  HZ_CHECK(p && grad, HZ_ERR_ARG, "bad arguments");
+  HZ_CHECK(p->misfit_ready, HZ_ERR_STATE, "hz_misfit_forward has not run");
+  const int64_t N = p->N;
+  const int nrec = (int)p->rec.size();
+  const double w2 = p->d.omega * p->d.omega;
+  grow(p->m_adj, (size_t)N * sizeof(double2) * 2);
+  double2* rhs = p->m_adj.as<double2>();
+  double2* lam = rhs + N;
+  launch_fill_real(N, 0.0, grad, p->stream);
+  std::vector<double2> pu, res(nrec);
+  for (int s = 0; s < p->nsrc; ++s) {
+    double2* us = p->fields_u.as<double2>() + (p->fields_stored ? (size_t)N * s : 0);
+    hz_krylov_stats st;
+    if (!p->fields_stored) {
+      fgmres_one(p, p->src_f.as<double2>() + (size_t)N * s, us, p->mopts, st, false);
+      HZ_CHECK(st.converged, HZ_ERR_NOT_CONVERGED, "recomputed forward solve %d did not converge", s);
+    }
+    gather_rec(p, us, pu);
+    for (int q = 0; q < nrec; ++q) {
+      const double2 d = p->data[(size_t)s * nrec + q];
+      res[q] = make_double2(pu[q].x - d.x, pu[q].y - d.y);
+    }
+    scatter_rec(p, res, rhs);
+    fgmres_one(p, rhs, lam, p->mopts, st, true);
+    HZ_CHECK(st.converged, HZ_ERR_NOT_CONVERGED, "adjoint solve %d: relative residual %.3e", s, st.rel_res);
+    launch_grad_acc(N, -w2, us, lam, grad, p->stream);
+  }
+  sync(p);
+  HZ_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,177 @@
+// hz_internal.h -- internal state and kernel entry points of the hz library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/hz.h"
+
+namespace hz {
+
+// ------------------------------------------------------------ errors
+void set_error(const char* fmt, ...);
+struct Error {
+  int code;
+};
+
+#define HZ_HIP(call)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess) {                                                            \
+      ::hz::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+      throw ::hz::Error{HZ_ERR_HIP};                                                   \
+    }                                                                                  \
+  } while (0)
+
+#define HZ_CHECK(cond, code, ...)       \
+  do {                                  \
+    if (!(cond)) {                      \
+      ::hz::set_error(__VA_ARGS__);     \
+      throw ::hz::Error{code};          \
+    }                                   \
+  } while (0)
+
+// ------------------------------------------------------------ device buffers
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  void alloc(size_t b);
+  void release();
+  ~DevBuf() { release(); }
+  template <class T>
+  T* as() const { return reinterpret_cast<T*>(ptr); }
+};
+
+// FFT plan for one length: radix sequence and twiddle tables.
+struct FftPlan {
+  int n = 0;
+  int nstages = 0;
+  int radix[24];
+  DevBuf tw64;  // double2[n]: exp(-2 pi i t / n)
+  DevBuf tw32;  // float2[n]
+};
+
+// Plane factors of one precision.
+struct Factors {
+  bool ready = false;
+  DevBuf G;        // nz * ny * nx * nx complex, [k][j] blocks, column-major nx x nx
+  size_t elem = 0; // bytes per complex element
+};
+
+}  // namespace hz
+
+struct hz_problem {
+  hz_desc d;
+  int64_t nx, ny, nz, n2, N;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int device = 0;
+
+  // host tables
+  std::vector<double2> cm[3], cp[3], cdg[3];  // stencil coefficients per axis
+  std::vector<double> lamz;                    // z symbol per plane
+  std::vector<double2> kappa;                  // (nx*ny) (w/cb)^2 (1 + i beta)
+  std::vector<double2> c0;                     // (nx*ny) background speed
+
+  // device tables: coefficient block [3 axes][cm, cp, cdg, acm, acp] (double2)
+  hz::DevBuf d_tab;      // stencil tables (forward and adjoint)
+  hz::DevBuf d_lamz;     // double[nz]
+  hz::DevBuf d_kappa;    // double2[n2]
+  hz::DevBuf d_mass;     // double[N] = omega^2 / c^2 (S7)
+  hz::DevBuf d_dL;       // double2[N] delta-L (REF)
+  hz::DevBuf d_dLc;      // double2[N] conj(delta-L) (REF adjoint)
+  hz::DevBuf d_sig;      // double2[ny] Schur coupling cm_y(j) cp_y(j-1)
+
+  hz::FftPlan fft;
+  hz::Factors fac[2];
+
+  // work buffers (grown on demand)
+  hz::DevBuf w_a, w_b, w_c;    // c128 N * nrhs each
+  hz::DevBuf red;              // reduction partials
+  hz::DevBuf scal;             // device scalars (double2)
+
+  // Krylov storage
+  hz::DevBuf kV, kZ, kx, kr, kw, kaux;
+  int k_m = 0, k_nrhs = 0;
+
+  // misfit state
+  std::vector<int64_t> rec;
+  std::vector<double2> data;
+  int nsrc = 0;
+  hz::DevBuf fields_u;  // nsrc * N (stored forward fields)
+  hz::DevBuf src_f;     // nsrc * N copy of the sources (recompute policy)
+  hz::DevBuf m_adj;     // adjoint right-hand side and multiplier (2 N)
+  hz_krylov_opts mopts;
+  bool fields_stored = false;
+  bool misfit_ready = false;
+
+  // offsets into d_tab
+  const double2* tab(int ax, int which) const {
+    // which: 0 cm, 1 cp, 2 cdg, 3 adjoint cm (conj cp(i-1)), 4 adjoint cp (conj cm(i+1))
+    const double2* base = d_tab.as<double2>();
+    int64_t off = 0;
+    for (int a = 0; a < ax; ++a) off += 5 * d.n[a];
+    return base + off + which * d.n[ax];
+  }
+};
+
+namespace hz {
+
+// ------------------------------------------------------------ setup (host)
+void setup_tables(hz_problem* p, const double* c_host);
+
+// ------------------------------------------------------------ kernels
+// z-FFT over every (x,y) line.  in/out: complex arrays of `in_prec`/`out_prec`
+// (0: double2, 1: float2); compute precision `prec`.  sign -1 forward
+// (unscaled), +1 backward (x 1/nz).  epi = 1 (backward only): out = w - dL*res
+// with w (double2) and dL (double2) read at the same index.
+void launch_fft_z(hz_problem* p, int prec, int sign, int nrhs, const void* in, int in_prec, void* out,
+                  int out_prec, int epi, const double2* w, const double2* dL, hipStream_t s);
+
+// S7 stencil y = A u (adjoint: A^H u); epi 1: y = f - A u.
+void launch_stencil(hz_problem* p, int nrhs, const double2* u, double2* y, bool adjoint, const double2* f,
+                    hipStream_t s);
+// plane 5-point matvec on z-transformed data: y_k = M_k x_k (adjoint: M_k^H x_k).
+void launch_plane_matvec(hz_problem* p, int nrhs, const double2* x, double2* y, bool adjoint, hipStream_t s);
+
+// block-tridiagonal factorisation of every plane, precision prec.
+void launch_factor(hz_problem* p, int prec, hipStream_t s);
+// in-place plane solves on z-transformed data (prec 0: double2, 1: float2).
+void launch_plane_solve(hz_problem* p, int prec, int nrhs, void* x, bool adjoint, hipStream_t s);
+
+// ---- vector kernels (FP64 complex)
+int reduce_blocks(int64_t n);
+// partial sums of conj(x) . y  (or |x|^2 when y == nullptr) over a masked or full range,
+// nv vectors laid out with stride N; result: out[v] (double2) on device.
+void launch_dot(hz_problem* p, int64_t n, const double2* x, const double2* y, int mask, double2* out,
+                hipStream_t s);
+void launch_dot_multi(hz_problem* p, int64_t n, int nv, const double2* x, int64_t xstride, const double2* y,
+                      int64_t ystride, int mask, double2* out, hipStream_t s);
+// y = y + a * x where a = coef[0] * (neg ? -1 : 1) (device scalar)
+void launch_axpy_dev(int64_t n, const double2* coef, bool neg, bool conj_coef, const double2* x, double2* y,
+                     hipStream_t s);
+// y = x / |coef| (coef holds a norm in .x)
+void launch_scale_inv_dev(int64_t n, const double2* coef, const double2* x, double2* y, hipStream_t s);
+// y = sum_i c[i] * X_i  + (accumulate ? y : 0)
+void launch_lincomb(int64_t n, int nv, const double2* c, const double2* X, int64_t xstride, double2* y,
+                    bool accumulate, hipStream_t s);
+// y = a*x + b*y with host scalars
+void launch_axpby(int64_t n, double2 a, const double2* x, double2 b, double2* y, hipStream_t s);
+// z = x + a*y + b*w (host scalars)
+void launch_axpbypcz(int64_t n, const double2* x, double2 a, const double2* y, double2 b, const double2* w,
+                     double2* z, hipStream_t s);
+// out = a - b (elementwise)
+void launch_sub(int64_t n, const double2* a, const double2* b, double2* out, hipStream_t s);
+// out = w - dL .* x
+void launch_w_minus_dl(int64_t n, const double2* w, const double2* dL, const double2* x, double2* out,
+                       hipStream_t s);
+// grad[l] += scale * Re(conj(u[l]) * lam[l])
+void launch_grad_acc(int64_t n, double scale, const double2* u, const double2* lam, double* grad, hipStream_t s);
+void launch_fill(int64_t n, double2 v, double2* y, hipStream_t s);
+void launch_fill_real(int64_t n, double v, double* y, hipStream_t s);
+
+}  // namespace hz

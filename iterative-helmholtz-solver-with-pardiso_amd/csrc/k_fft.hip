@@ -1,0 +1,204 @@
+// k_fft.hip -- batched z-DFT of every (x,y) line (length nz, stride nx*ny).
+//
+// One workgroup owns LINES consecutive lines: it stages the LINES x nz tile in
+// LDS (coalesced LINES*sizeof(complex) segments per z), runs a mixed-radix
+// Stockham transform (radices 4, 2, 3, 5 and any small prime) ping-ponging
+// between two LDS buffers, and writes the tile back, optionally converting
+// between complex128 and complex64 and fusing the reference's
+// g = w - deltaL * (L0^{-1} w) epilogue into the backward pass
+// (functions.cpp:5571-5574).  Forward is unscaled with e^{-2 pi i jk/n},
+// backward scales by 1/n (DFTI settings of functions.cpp:5864-5869).
+#include "hz_internal.h"
+
+namespace hz {
+namespace {
+
+template <typename T> struct Cx;
+template <> struct Cx<float> { typedef float2 t; };
+template <> struct Cx<double> { typedef double2 t; };
+
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { C r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { C r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
+template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
+  C r; r.x = a.x * b.x - a.y * b.y; r.y = a.x * b.y + a.y * b.x; return r;
+}
+// multiply by sign * i
+template <typename C> __device__ __forceinline__ C cmul_si(C a, int sign) {
+  C r; r.x = -sign * a.y; r.y = sign * a.x; return r;
+}
+template <typename C, typename T> __device__ __forceinline__ C cscale(C a, T s) { C r; r.x = a.x * s; r.y = a.y * s; return r; }
+
+template <typename C> __device__ __forceinline__ C load_c(const double2* p) { C r; r.x = p->x; r.y = p->y; return r; }
+template <typename C> __device__ __forceinline__ C load_c(const float2* p) { C r; r.x = p->x; r.y = p->y; return r; }
+
+struct FftArgs {
+  int n;
+  int nstages;
+  int radix[24];
+};
+
+template <typename T, int LINES>
+__device__ void stockham_lds(typename Cx<T>::t* x, typename Cx<T>::t* y, const FftArgs& a,
+                             const typename Cx<T>::t* __restrict__ tw, int sign,
+                             typename Cx<T>::t** result) {
+  typedef typename Cx<T>::t C;
+  const int n = a.n;
+  int len = n, s = 1;
+  for (int st = 0; st < a.nstages; ++st) {
+    const int p = a.radix[st];
+    const int m = len / p;
+    const int nbf = LINES * (n / p);
+    for (int b = threadIdx.x; b < nbf; b += blockDim.x) {
+      const int l = b % LINES;
+      const int qt = b / LINES;
+      const int t = qt % s;
+      const int q = qt / s;
+      const int in0 = t + s * q;
+      const int out0 = t + s * p * q;
+      C v[5];
+      if (p == 4) {
+        C a0 = x[(in0) * LINES + l], a1 = x[(in0 + s * m) * LINES + l];
+        C a2 = x[(in0 + 2 * s * m) * LINES + l], a3 = x[(in0 + 3 * s * m) * LINES + l];
+        C t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = cmul_si(csub(a1, a3), sign);
+        v[0] = cadd(t0, t2); v[1] = cadd(t1, t3); v[2] = csub(t0, t2); v[3] = csub(t1, t3);
+      } else if (p == 2) {
+        C a0 = x[in0 * LINES + l], a1 = x[(in0 + s * m) * LINES + l];
+        v[0] = cadd(a0, a1); v[1] = csub(a0, a1);
+      } else if (p == 3) {
+        const T h = (T)0.86602540378443864676372317075294;
+        C a0 = x[in0 * LINES + l], a1 = x[(in0 + s * m) * LINES + l], a2 = x[(in0 + 2 * s * m) * LINES + l];
+        C sm = cadd(a1, a2), df = cmul_si(cscale(csub(a1, a2), h), sign);
+        C base = csub(a0, cscale(sm, (T)0.5));
+        v[0] = cadd(a0, sm); v[1] = cadd(base, df); v[2] = csub(base, df);
+      } else if (p == 5) {
+        const T c1 = (T)0.30901699437494742410229341718282, c2 = (T)-0.80901699437494742410229341718282;
+        const T s1 = (T)0.95105651629515357211643933337938, s2 = (T)0.58778525229247312916870595463907;
+        C a0 = x[in0 * LINES + l], a1 = x[(in0 + s * m) * LINES + l], a2 = x[(in0 + 2 * s * m) * LINES + l];
+        C a3 = x[(in0 + 3 * s * m) * LINES + l], a4 = x[(in0 + 4 * s * m) * LINES + l];
+        C b1 = cadd(a1, a4), b2 = cadd(a2, a3), d1 = csub(a1, a4), d2 = csub(a2, a3);
+        C r1 = cadd(a0, cadd(cscale(b1, c1), cscale(b2, c2)));
+        C r2 = cadd(a0, cadd(cscale(b1, c2), cscale(b2, c1)));
+        C i1 = cmul_si(cadd(cscale(d1, s1), cscale(d2, s2)), sign);
+        C i2 = cmul_si(csub(cscale(d1, s2), cscale(d2, s1)), sign);
+        v[0] = cadd(a0, cadd(b1, b2)); v[1] = cadd(r1, i1); v[4] = csub(r1, i1); v[2] = cadd(r2, i2); v[3] = csub(r2, i2);
+      }
+      if (p <= 5) {
+        for (int k = 0; k < p; ++k) {
+          C w = tw[(q * k * s) % n];
+          if (sign > 0) w.y = -w.y;
+          y[(out0 + s * k) * LINES + l] = (k == 0) ? v[0] : cmul(v[k], w);
+        }
+      } else {
+        // generic prime radix: direct sum with table twiddles
+        const int np = n / p;
+        for (int k = 0; k < p; ++k) {
+          C acc; acc.x = 0; acc.y = 0;
+          for (int r = 0; r < p; ++r) {
+            C w = tw[(r * k * np) % n];
+            if (sign > 0) w.y = -w.y;
+            acc = cadd(acc, cmul(x[(in0 + r * s * m) * LINES + l], w));
+          }
+          C w = tw[(q * k * s) % n];
+          if (sign > 0) w.y = -w.y;
+          y[(out0 + s * k) * LINES + l] = cmul(acc, w);
+        }
+      }
+    }
+    __syncthreads();
+    C* tmp = x; x = y; y = tmp;
+    len = m;
+    s *= p;
+  }
+  *result = x;
+}
+
+template <typename T, typename TI, typename TO, int EPI, int LINES>
+__global__ __launch_bounds__(256) void k_fft_z(const TI* __restrict__ in, TO* __restrict__ out, int64_t n2,
+                                               int64_t N, FftArgs a, const typename Cx<T>::t* __restrict__ tw,
+                                               int sign, T scale, const double2* __restrict__ w,
+                                               const double2* __restrict__ dL) {
+  typedef typename Cx<T>::t C;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* buf0 = reinterpret_cast<C*>(smem_raw);
+  C* buf1 = buf0 + LINES * a.n;
+  const int n = a.n;
+  const int64_t rhs = blockIdx.y;
+  const int64_t w0 = (int64_t)blockIdx.x * LINES;
+  const TI* src = in + rhs * N;
+  TO* dst = out + rhs * N;
+  for (int idx = threadIdx.x; idx < LINES * n; idx += blockDim.x) {
+    const int l = idx % LINES, z = idx / LINES;
+    const int64_t g = w0 + l;
+    C v;
+    v.x = 0; v.y = 0;
+    if (g < n2) v = load_c<C>(src + g + n2 * z);
+    buf0[z * LINES + l] = v;
+  }
+  __syncthreads();
+  C* res;
+  stockham_lds<T, LINES>(buf0, buf1, a, tw, sign, &res);
+  for (int idx = threadIdx.x; idx < LINES * n; idx += blockDim.x) {
+    const int l = idx % LINES, z = idx / LINES;
+    const int64_t g = w0 + l;
+    if (g >= n2) continue;
+    C v = res[z * LINES + l];
+    const int64_t gi = g + n2 * z;
+    if (EPI == 1) {
+      const double2 ww = w[rhs * N + gi], dd = dL[gi];
+      const double vx = (double)v.x * (double)scale, vy = (double)v.y * (double)scale;
+      double2 o;
+      o.x = ww.x - (dd.x * vx - dd.y * vy);
+      o.y = ww.y - (dd.x * vy + dd.y * vx);
+      TO r; r.x = o.x; r.y = o.y;
+      dst[gi] = r;
+    } else {
+      TO r;
+      r.x = v.x * scale;
+      r.y = v.y * scale;
+      dst[gi] = r;
+    }
+  }
+}
+
+template <typename T, typename TI, typename TO, int EPI>
+void launch_typed(hz_problem* p, int sign, int nrhs, const void* in, void* out, const double2* w,
+                  const double2* dL, hipStream_t s) {
+  constexpr int LINES = 16;
+  FftArgs a;
+  a.n = p->fft.n;
+  a.nstages = p->fft.nstages;
+  for (int i = 0; i < 24; ++i) a.radix[i] = i < a.nstages ? p->fft.radix[i] : 1;
+  const typename Cx<T>::t* tw =
+      sizeof(T) == 8 ? (const typename Cx<T>::t*)p->fft.tw64.ptr : (const typename Cx<T>::t*)p->fft.tw32.ptr;
+  size_t lds = 2 * (size_t)LINES * a.n * sizeof(typename Cx<T>::t);
+  HZ_CHECK(lds <= 160 * 1024, HZ_ERR_ARG, "nz=%d too long for the LDS-resident FFT", a.n);
+  dim3 grid((unsigned)((p->n2 + LINES - 1) / LINES), (unsigned)nrhs);
+  if (lds > 64 * 1024)
+    HZ_HIP(hipFuncSetAttribute((const void*)&k_fft_z<T, TI, TO, EPI, LINES>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  T scale = sign > 0 ? (T)(1.0 / (double)a.n) : (T)1;
+  hipLaunchKernelGGL((k_fft_z<T, TI, TO, EPI, LINES>), grid, dim3(256), lds, s, (const TI*)in, (TO*)out, p->n2,
+                     p->N, a, tw, sign, scale, w, dL);
+  HZ_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void launch_fft_z(hz_problem* p, int prec, int sign, int nrhs, const void* in, int in_prec, void* out,
+                  int out_prec, int epi, const double2* w, const double2* dL, hipStream_t s) {
+  HZ_CHECK(!(epi && out_prec != 0), HZ_ERR_ARG, "fft epilogue needs a complex128 output");
+  if (prec == HZ_PREC_FP64) {
+    HZ_CHECK(in_prec == 0 && out_prec == 0, HZ_ERR_ARG, "FP64 FFT takes complex128 in and out");
+    if (epi) launch_typed<double, double2, double2, 1>(p, sign, nrhs, in, out, w, dL, s);
+    else launch_typed<double, double2, double2, 0>(p, sign, nrhs, in, out, w, dL, s);
+    return;
+  }
+  if (in_prec == 0 && out_prec == 1) launch_typed<float, double2, float2, 0>(p, sign, nrhs, in, out, w, dL, s);
+  else if (in_prec == 1 && out_prec == 0) {
+    if (epi) launch_typed<float, float2, double2, 1>(p, sign, nrhs, in, out, w, dL, s);
+    else launch_typed<float, float2, double2, 0>(p, sign, nrhs, in, out, w, dL, s);
+  } else if (in_prec == 1 && out_prec == 1) launch_typed<float, float2, float2, 0>(p, sign, nrhs, in, out, w, dL, s);
+  else launch_typed<float, double2, double2, 0>(p, sign, nrhs, in, out, w, dL, s);
+}
+
+}  // namespace hz

@@ -1,0 +1,288 @@
+"""GPU parity: every stage of the HIP path against the CPU oracle on the same
+seeded inputs, plus size-independent invariants at the full benchmark sizes.
+
+Tolerances (relative 2-norm errors):
+  FP64 stages (stencil, FFT, plane solves, preconditioner)  1e-10 or tighter
+  complex64 preconditioner (FP32 arithmetic)                 1e-4
+  Krylov solutions vs the oracle's                           1e-8 (FP64), 1e-6 (mixed)
+  iteration counts                                           within +-1 of the oracle
+"""
+import numpy as np
+import pytest
+
+from conftest import load_pkg
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+@pytest.fixture(scope="module")
+def hz():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return load_pkg()
+
+
+def _dev(x):
+    return torch.from_numpy(np.ascontiguousarray(x, dtype=np.complex128)).cuda()
+
+
+def _host(x):
+    return x.cpu().numpy()
+
+
+def ref_case(hz, homo=False, nxp=6, nyp=7, pml=3, spg=3, nzp=6, beta=0.5, length=240.0):
+    pr = hz.problem
+    d = pr.ref_geometry(n_nopml=nxp, pml=pml, spg=spg, nz_nopml=nzp, length=(length,) * 3, beta=beta, homo=homo)
+    ny = nyp + 2 * pml
+    ly = length + 2.0 * pml * length / nyp
+    d = d.replace(n=(d.n[0], ny, d.n[2]), l=(d.l[0], ly, d.l[2]), h=(d.h[0], ly / ny, d.h[2]))
+    return d, pr.ref_sound_speed(d)
+
+
+def s7_case(hz, n=(12, 10, 14), npml=3, seed=3, beta=0.5, smooth=False):
+    pr = hz.problem
+    h = 1.0 / max(n)
+    d = pr.cube_geometry(n, npml, pr.omega_for_ppw(0.8, h, 8), beta=beta, pml_c=30.0 if smooth else 1000.0)
+    if smooth:
+        # mild smooth medium: the solves converge in a few tens of iterations
+        c = pr.random_velocity(d, seed, corr_len=0.25, amp=0.1, cmin=0.8, cmax=1.2)
+    else:
+        c = 0.8 + 0.4 * np.random.default_rng(seed).random(d.N)
+    return d, c
+
+
+def rand_vec(N, seed, nrhs=None):
+    rng = np.random.default_rng(seed)
+    shape = (N,) if nrhs is None else (nrhs, N)
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+# ----------------------------------------------------------------- FFT
+@pytest.mark.parametrize("nz", [1, 2, 7, 12, 16, 20, 25, 30, 49, 64, 96, 150, 200])
+def test_fft_z_vs_numpy(hz, nz):
+    print("case fft nz=%d" % nz, flush=True)
+    d = hz.problem.cube_geometry((5, 3, nz), 0, 10.0)
+    S = hz.Solver(d, np.ones(d.N))
+    x = rand_vec(d.N, nz)
+    X = x.reshape(d.shape_zyx)
+    want_f = np.fft.fft(X, axis=0).reshape(-1)
+    want_b = np.fft.ifft(X, axis=0).reshape(-1)
+    assert _rel(_host(S.fft_z(_dev(x), -1, hz.abi.PREC_FP64)), want_f) < 1e-14
+    assert _rel(_host(S.fft_z(_dev(x), +1, hz.abi.PREC_FP64)), want_b) < 1e-14
+    assert _rel(_host(S.fft_z(_dev(x), -1, hz.abi.PREC_C64)), want_f) < 2e-6
+    assert _rel(_host(S.fft_z(_dev(x), +1, hz.abi.PREC_C64)), want_b) < 2e-6
+
+
+def test_fft_z_multi_rhs_and_ragged_lines(hz):
+    # nx*ny = 37 lines: not a multiple of the 16-line tile
+    d = hz.problem.cube_geometry((37, 1, 24), 0, 10.0)
+    S = hz.Solver(d, np.ones(d.N))
+    x = rand_vec(d.N, 5, nrhs=3)
+    got = _host(S.fft_z(_dev(x), -1))
+    want = np.fft.fft(x.reshape(3, *d.shape_zyx), axis=1).reshape(3, -1)
+    assert _rel(got, want) < 1e-14
+
+
+# --------------------------------------------------------------- stencils
+def test_stencil_s7_vs_oracle(hz, oracle):
+    d, c = s7_case(hz)
+    S, P = hz.Solver(d, c), oracle.Problem(d, c)
+    x = rand_vec(d.N, 1)
+    assert _rel(_host(S.op_apply(_dev(x))), P.op_apply(x)) < 1e-14
+    assert _rel(_host(S.op_apply(_dev(x), adjoint=True)), P.op_apply(x, adjoint=True)) < 1e-14
+    xs = rand_vec(d.N, 2, nrhs=2)
+    got = _host(S.op_apply(_dev(xs)))
+    assert _rel(got[1], P.op_apply(xs[1])) < 1e-14
+
+
+def test_stencil_s7_adjoint_identity_large(hz):
+    """<A x, y> = <x, A^H y> at the 128^3 size (configs 2 and 5)."""
+    pr = hz.problem
+    d = pr.cube_geometry(128, 16, pr.omega_for_ppw(0.7, 1 / 128, 10))
+    c = pr.random_velocity(d, 7)
+    S = hz.Solver(d, c)
+    x, y = _dev(rand_vec(d.N, 3)), _dev(rand_vec(d.N, 4))
+    ax = S.op_apply(x)
+    ahy = S.op_apply(y, adjoint=True)
+    lhs = torch.vdot(y, ax).item()
+    rhs = torch.vdot(ahy, x).item()
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+
+
+@pytest.mark.parametrize("homo", [True, False])
+def test_L0_and_true_operator_ref_vs_oracle(hz, oracle, homo):
+    d, c = ref_case(hz, homo=homo)
+    S, P = hz.Solver(d, c), oracle.Problem(d, c)
+    x = rand_vec(d.N, 11)
+    assert _rel(_host(S.L0_apply(_dev(x))), P.L0_apply(x)) < 1e-13
+    assert _rel(_host(S.op_apply(_dev(x))), P.op_apply(x)) < 1e-13
+    assert _rel(_host(S.op_apply(_dev(x), adjoint=True)), P.op_apply(x, adjoint=True)) < 1e-13
+
+
+# ---------------------------------------------------------- preconditioner
+@pytest.mark.parametrize("case", ["ref_homo", "ref_hetero", "s7", "s7_odd", "s7_prime_nz", "s7_nx1"])
+def test_precond_fp64_vs_oracle(hz, oracle, case):
+    print("case precond fp64 %s" % case, flush=True)
+    if case.startswith("ref"):
+        d, c = ref_case(hz, homo=(case == "ref_homo"))
+    elif case == "s7":
+        d, c = s7_case(hz)
+    elif case == "s7_odd":
+        d, c = s7_case(hz, n=(13, 9, 15), npml=2)
+    elif case == "s7_prime_nz":
+        d, c = s7_case(hz, n=(10, 11, 13), npml=2)
+    else:
+        d, c = s7_case(hz, n=(1, 9, 8), npml=(0, 2, 2))
+    S, P = hz.Solver(d, c), oracle.Problem(d, c)
+    S.factor(hz.abi.PREC_FP64)
+    x = rand_vec(d.N, 21)
+    assert _rel(_host(S.precond_apply(_dev(x))), P.precond_apply(x)) < 1e-10
+    assert _rel(_host(S.precond_apply(_dev(x), adjoint=True)), P.precond_apply(x, adjoint=True)) < 1e-10
+    assert _rel(_host(S.op_pre_apply(_dev(x))), P.op_pre_apply(x)) < 1e-10
+
+
+def test_precond_c64_vs_oracle(hz, oracle):
+    d, c = s7_case(hz, n=(24, 20, 16), npml=4, smooth=True)
+    S, P = hz.Solver(d, c), oracle.Problem(d, c)
+    S.factor(hz.abi.PREC_C64)
+    x = rand_vec(d.N, 22)
+    e = _rel(_host(S.precond_apply(_dev(x), hz.abi.PREC_C64)), P.precond_apply(x))
+    ea = _rel(_host(S.precond_apply(_dev(x), hz.abi.PREC_C64, adjoint=True)), P.precond_apply(x, adjoint=True))
+    print("c64 preconditioner error %.2e / adjoint %.2e" % (e, ea))
+    assert e < 1e-4 and ea < 1e-4
+
+
+def test_plane_pivoting_indefinite(hz, oracle):
+    """beta = 0 and no layers: indefinite planes need the partial pivoting."""
+    pr = hz.problem
+    d = pr.cube_geometry((11, 9, 6), 0, 37.0, beta=0.0)
+    c = np.ones(d.N)
+    S, P = hz.Solver(d, c), oracle.Problem(d, c)
+    S.factor(hz.abi.PREC_FP64)
+    x = rand_vec(d.N, 23)
+    assert _rel(_host(S.precond_apply(_dev(x))), P.precond_apply(x)) < 1e-9
+
+
+@pytest.mark.parametrize("n,npml", [(128, 16), (200, 20)])
+def test_precond_inverse_identity_full_size(hz, n, npml):
+    """L0 (L0^{-1} x) = x at the configs' full sizes (FP64 1e-10, complex64 1e-4),
+    the TEST_L0 check of FGMRES.cpp:606-621."""
+    print("case inverse identity n=%d" % n, flush=True)
+    pr = hz.problem
+    d = pr.cube_geometry(n, npml, pr.omega_for_ppw(0.7, 1.0 / n, 10))
+    c = pr.random_velocity(d, 1234)
+    S = hz.Solver(d, c)
+    x = _dev(rand_vec(d.N, 5))
+    for prec, tol in ((hz.abi.PREC_C64, 1e-4), (hz.abi.PREC_FP64, 1e-10)):
+        S.factor(prec)
+        y = S.L0_apply(S.precond_apply(x, prec))
+        e = (torch.linalg.vector_norm(y - x) / torch.linalg.vector_norm(x)).item()
+        print("n=%d prec=%d ||L0 L0^-1 x - x||/||x|| = %.2e" % (n, prec, e), flush=True)
+        assert e < tol
+        del y
+    S.close()
+    torch.cuda.empty_cache()
+
+
+# ------------------------------------------------------------- drivers
+@pytest.mark.parametrize("homo", [True, False])
+def test_gmres_ref_matches_reference_path(hz, oracle, homo):
+    """Reference GMRES (conj Arnoldi coefficient, zgels LS, true residual on the
+    physical box): same per-iteration residuals, iteration count within +-1 and
+    the same u as the oracle."""
+    d, c = ref_case(hz, homo=homo)
+    S, P = hz.Solver(d, c), oracle.Problem(d, c)
+    S.factor(hz.abi.PREC_FP64)
+    f = hz.problem.point_source(d)
+    u_o, st_o = P.gmres_ref(f, m=10, rest=5, thresh=1e-12, tol=1e-8, mask=1, ortho=0)
+    u, st = S.fgmres(_dev(f), m=10, restarts=5, tol=1e-8, driver=hz.abi.DRIVER_GMRES_REF,
+                     ortho=hz.abi.ORTHO_REF_CONJ, mask=hz.abi.MASK_PHYS, breakdown=1e-12)
+    print("ref gmres iters gpu %d oracle %d" % (st.iters, st_o.iters))
+    assert abs(st.iters - st_o.iters) <= 1
+    k = min(len(st.hist), len(st_o.hist), 8)
+    assert np.allclose(st.hist[:k], st_o.hist[:k], rtol=1e-7)
+    if st.iters == st_o.iters:
+        assert _rel(_host(u), u_o) < 1e-10
+
+
+def test_fgmres_fp64_and_mixed_vs_oracle(hz, oracle):
+    d, c = s7_case(hz, n=(20, 18, 22), npml=4, smooth=True)
+    S, P = hz.Solver(d, c), oracle.Problem(d, c)
+    S.factor(hz.abi.PREC_FP64)
+    S.factor(hz.abi.PREC_C64)
+    f = hz.problem.point_source(d, (9, 8, 11), amplitude=1.0)
+    u_o, st_o = P.fgmres(f, m=40, rest=10, tol=1e-10)
+    u64, st64 = S.fgmres(_dev(f), m=40, restarts=10, tol=1e-10, prec=hz.abi.PREC_FP64)
+    u32, st32 = S.fgmres(_dev(f), m=40, restarts=10, tol=1e-10, prec=hz.abi.PREC_C64)
+    print("fgmres iters oracle %d fp64 %d mixed %d" % (st_o.iters, st64.iters, st32.iters))
+    assert st64.converged and st32.converged and st_o.converged
+    assert abs(st64.iters - st_o.iters) <= 1
+    assert np.allclose(st64.hist[:10], st_o.hist[:10], rtol=1e-8)
+    assert _rel(_host(u64), u_o) < 1e-8
+    assert _rel(_host(u32), u_o) < 1e-8
+    # true residuals with the oracle's operator
+    for u in (u64, u32):
+        r = P.op_apply(_host(u)) - f
+        assert np.linalg.norm(r) / np.linalg.norm(f) < 1.5e-10
+
+
+@pytest.mark.parametrize("mode", ["ref", "s7"])
+def test_bicgstab_vs_oracle(hz, oracle, mode):
+    d, c = ref_case(hz) if mode == "ref" else s7_case(hz, smooth=True)
+    S, P = hz.Solver(d, c), oracle.Problem(d, c)
+    S.factor(hz.abi.PREC_FP64)
+    f = hz.problem.point_source(d, amplitude=1.0)
+    u_o, st_o = P.bicgstab(f, maxit=300, tol=1e-9, tol_abs=0, mask=0)
+    u, st = S.bicgstab(_dev(f), maxit=300, tol=1e-9, tol_abs=False, mask=hz.abi.MASK_FULL)
+    print("bicgstab iters gpu %d oracle %d" % (st.iters, st_o.iters))
+    assert st.converged and st_o.converged
+    assert abs(st.iters - st_o.iters) <= 1
+    assert _rel(_host(u), u_o) < 1e-7
+
+
+def test_misfit_gradient_vs_oracle(hz, oracle):
+    d, c = s7_case(hz, n=(14, 12, 16), npml=3, smooth=True)
+    nsrc = 2
+    f = np.zeros((nsrc, d.N), complex)
+    f[0, hz.problem.flat_index(d, 5, 5, 6)] = 1.0
+    f[1, hz.problem.flat_index(d, 8, 6, 9)] = 1.0
+    rec = np.array([hz.problem.flat_index(d, i, j, 12) for i in range(3, 11, 2) for j in range(3, 9, 2)])
+    rng = np.random.default_rng(9)
+    data = (rng.standard_normal((nsrc, rec.size)) + 1j * rng.standard_normal((nsrc, rec.size))) * 1e-3
+    P = oracle.Problem(d, c)
+    J_o, g_o, _, ok = P.misfit_grad(f, rec, data, m=60, rest=10, tol=1e-12)
+    assert ok
+    S = hz.Solver(d, c)
+    S.factor(hz.abi.PREC_FP64)
+    for store in (True, False):
+        J = S.misfit_forward(_dev(f), rec, data, m=60, restarts=10, tol=1e-12, prec=hz.abi.PREC_FP64,
+                             store_fields=store)
+        g = _host(S.misfit_backward())
+        assert abs(J - J_o) < 1e-9 * abs(J_o)
+        assert _rel(g, g_o) < 1e-7
+    S.factor(hz.abi.PREC_C64)
+    J = S.misfit_forward(_dev(f), rec, data, m=60, restarts=10, tol=1e-10, prec=hz.abi.PREC_C64)
+    g = _host(S.misfit_backward())
+    assert _rel(g, g_o) < 1e-6
+
+
+def test_error_paths(hz):
+    """Apply before factor, bad sizes and a non-convergent solve return errors."""
+    d, c = s7_case(hz)
+    S = hz.Solver(d, c)
+    x = _dev(rand_vec(d.N, 1))
+    with pytest.raises(hz.abi.HzError) as e:
+        S.precond_apply(x)
+    assert e.value.code == 3
+    S.factor(hz.abi.PREC_FP64)
+    u, st = S.fgmres(x, m=2, restarts=1, tol=1e-14)
+    assert not st.converged
+    bad = d.replace(pml=(20, 20, 20))
+    with pytest.raises(hz.abi.HzError):
+        hz.Solver(bad, c)

@@ -179,9 +179,10 @@ int hz_fgmres(hz_problem* p, int nrhs, const hz_c128* f, hz_c128* u, const hz_kr
               hz_krylov_stats* stats);
 /* Right-preconditioned BiCGStab with shadow residual 2 r0; stops when the
  * residual over the mask (absolute when tol_abs, else relative) < tol.
- * Replaces BCGSTAB (templates.h:60; BcGSTAB.cpp:6-666). */
+ * hist (host, optional): relative residual after each iteration of the first
+ * right-hand side.  Replaces BCGSTAB (templates.h:60; BcGSTAB.cpp:6-666). */
 int hz_bicgstab(hz_problem* p, int prec, int nrhs, const hz_c128* f, hz_c128* u, int maxit, double tol,
-                int tol_abs, int mask, hz_krylov_stats* stats);
+                int tol_abs, int mask, double* hist, int hist_cap, hz_krylov_stats* stats);
 
 /* ------------------------------------------------------------- adjoint */
 /* Receiver misfit J = 1/2 sum_s ||P u_s - d_s||^2 (S7 mode) for nsrc sources

@@ -85,7 +85,8 @@ SIGNATURES = {
     "hz_op_apply_preconditioned": (_i, [_vp, _i, _i, _vp, _vp]),
     "hz_residual": (_i, [_vp, _i, _vp, _vp, _vp, _i, C.POINTER(C.c_double)]),
     "hz_fgmres": (_i, [_vp, _i, _vp, _vp, C.POINTER(KrylovOpts), C.POINTER(KrylovStats)]),
-    "hz_bicgstab": (_i, [_vp, _i, _i, _vp, _vp, _i, _d, _i, _i, C.POINTER(KrylovStats)]),
+    "hz_bicgstab": (_i, [_vp, _i, _i, _vp, _vp, _i, _d, _i, _i, C.POINTER(C.c_double), _i,
+                         C.POINTER(KrylovStats)]),
     "hz_misfit_forward": (_i, [_vp, _i, _vp, _i, C.POINTER(C.c_int64), _vp, C.POINTER(KrylovOpts), _i,
                                C.POINTER(C.c_double)]),
     "hz_misfit_backward": (_i, [_vp, _vp]),

@@ -250,13 +250,15 @@ static void fgmres_one(hz_problem* p, const double2* f, double2* x, const hz_kry
       precond_apply(p, o.prec, 1, Vj, Zj, adjoint, false);
       op_apply(p, 1, Zj, w, adjoint);
       total++;
+      // modified Gram-Schmidt: h_ij = V_i^H w, w -= h_ij V_i in sequence; each
+      // update is fused with the next coefficient's dot (the last with ||w||)
+      double2* hn_d = dH + (j + 1) + (size_t)ldh * j;
+      launch_dot(p, N, V, w, HZ_MASK_FULL, dH + (size_t)ldh * j, p->stream);
       for (int i = 0; i <= j; ++i) {
         double2* hij = dH + i + (size_t)ldh * j;
-        launch_dot(p, N, V + (size_t)N * i, w, HZ_MASK_FULL, hij, p->stream);
-        launch_axpy_dev(N, hij, true, false, V + (size_t)N * i, w, p->stream);
+        const double2* vnext = i < j ? V + (size_t)N * (i + 1) : nullptr;
+        launch_axpy_dot(p, N, hij, V + (size_t)N * i, w, vnext, hij + 1, p->stream);
       }
-      double2* hn_d = dH + (j + 1) + (size_t)ldh * j;
-      launch_dot(p, N, w, nullptr, HZ_MASK_FULL, hn_d, p->stream);
       HZ_HIP(hipMemcpyAsync(hcol.data(), dH + (size_t)ldh * j, (j + 2) * sizeof(double2), hipMemcpyDeviceToHost,
                             p->stream));
       sync(p);
@@ -433,7 +435,7 @@ static void gmres_ref_one(hz_problem* p, const double2* f, double2* u, const hz_
 // BiCGStab of BcGSTAB.cpp:397-568 (w-space recurrences on A L0^{-1}, x in
 // u-space).  The residual uses the true operator A x - f directly.
 static void bicgstab_one(hz_problem* p, int prec, const double2* f, double2* x, int maxit, double tol, int tol_abs,
-                         int mask, hz_krylov_stats& st) {
+                         int mask, HistW& hist, hz_krylov_stats& st) {
   const int64_t N = p->N;
   ensure_krylov(p, 8);
   double2* V = p->kV.as<double2>();
@@ -478,6 +480,7 @@ static void bicgstab_one(hz_problem* p, int prec, const double2* f, double2* x, 
     residual(p, 1, f, x, w, false);
     Res = (nrm_host(p, w, mask));
     RelRes = Res / norm_f;
+    hist.push(RelRes);
     if (!std::isfinite(RelRes)) {
       set_error("non-finite residual in BiCGStab");
       throw Error{HZ_ERR_NONFINITE};
@@ -492,6 +495,7 @@ static void bicgstab_one(hz_problem* p, int prec, const double2* f, double2* x, 
   st.converged = conv;
   st.res = Res;
   st.rel_res = RelRes;
+  st.n_hist = hist.n;
   st.t_solve_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -695,13 +699,14 @@ int hz_fgmres(hz_problem* p, int nrhs, const hz_c128* f, hz_c128* u, const hz_kr
 }
 
 int hz_bicgstab(hz_problem* p, int prec, int nrhs, const hz_c128* f, hz_c128* u, int maxit, double tol, int tol_abs,
-                int mask, hz_krylov_stats* st) {
+                int mask, double* hist, int hist_cap, hz_krylov_stats* st) {
   HZ_API_BEGIN
   HZ_CHECK(p && f && u && st && nrhs >= 1 && maxit >= 1, HZ_ERR_ARG, "bad arguments");
   int rc = HZ_OK;
   for (int q = 0; q < nrhs; ++q) {
+    HistW hw{q == 0 ? hist : nullptr, q == 0 ? hist_cap : 0};
     bicgstab_one(p, prec, (const double2*)f + (size_t)q * p->N, (double2*)u + (size_t)q * p->N, maxit, tol, tol_abs,
-                 mask, st[q]);
+                 mask, hw, st[q]);
     if (!st[q].converged && rc == HZ_OK) {
       rc = HZ_ERR_NOT_CONVERGED;
       set_error("BiCGStab %d stopped at residual %.3e after %d iterations", q, st[q].res, st[q].iters);

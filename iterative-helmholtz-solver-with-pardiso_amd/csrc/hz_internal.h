@@ -154,6 +154,9 @@ void launch_dot_multi(hz_problem* p, int64_t n, int nv, const double2* x, int64_
 // y = y + a * x where a = coef[0] * (neg ? -1 : 1) (device scalar)
 void launch_axpy_dev(int64_t n, const double2* coef, bool neg, bool conj_coef, const double2* x, double2* y,
                      hipStream_t s);
+// w -= coef[0] * x; out = y^H w (y == nullptr: out = ||w||), one pass
+void launch_axpy_dot(hz_problem* p, int64_t n, const double2* coef, const double2* x, double2* w,
+                     const double2* y, double2* out, hipStream_t s);
 // y = x / |coef| (coef holds a norm in .x)
 void launch_scale_inv_dev(int64_t n, const double2* coef, const double2* x, double2* y, hipStream_t s);
 // y = sum_i c[i] * X_i  + (accumulate ? y : 0)

@@ -150,7 +150,8 @@ void BCGSTAB(size_m x, size_m y, size_m z, int m, const point source, dtype* x_s
   double2* du = g_cache.buf + N;
   up(f, df, N);
   hz_krylov_stats st;
-  int rc = hz_bicgstab(p, g_cfg.prec, 1, (const hz_c128*)df, (hz_c128*)du, m, g_cfg.res_exit, 1, HZ_MASK_PHYS, &st);
+  int rc = hz_bicgstab(p, g_cfg.prec, 1, (const hz_c128*)df, (hz_c128*)du, m, g_cfg.res_exit, 1, HZ_MASK_PHYS,
+                       nullptr, 0, &st);
   if (rc && rc != HZ_ERR_NOT_CONVERGED) fail("BCGSTAB", rc);
   down(du, x_sol, N);
   diff_sol = 0.0;

@@ -413,6 +413,144 @@ __global__ __launch_bounds__(APPLY_THREADS) void k_apply(const typename Cx<T>::t
   }
 }
 
+// ------------------------------------------------------------ fast solve
+// M_k x = b with the factors streamed in a register pipeline.  One workgroup of
+// 1024 threads per (plane, rhs).  Thread (g, ru) owns load unit ru (16 bytes:
+// RPL = 2 rows of complex64 or 1 row of complex128) and the g-th contiguous
+// column range of every block; it walks its columns in chunks of CH loads and
+// issues the first chunk of the NEXT block before the cross-group reduction
+// and the barrier of the current step, so the serial part of each step runs
+// under that chunk's memory latency.  The block sequence of one solve is
+// G_0 .. G_{ny-2} (forward) then G_{ny-1} .. G_0 (backward): 2 ny - 1 GEMVs.
+template <typename T> struct Unit;
+template <> struct Unit<float> { typedef float4 t; };
+template <> struct Unit<double> { typedef double2 t; };
+
+template <typename T, int RPL>
+__device__ __forceinline__ void unit_fma(const typename Unit<T>::t& a, typename Cx<T>::t vc, T* acc) {
+  if (RPL == 2) {
+    const float4& q = reinterpret_cast<const float4&>(a);
+    acc[0] += q.x * vc.x - q.y * vc.y;
+    acc[1] += q.x * vc.y + q.y * vc.x;
+    acc[2] += q.z * vc.x - q.w * vc.y;
+    acc[3] += q.z * vc.y + q.w * vc.x;
+  } else {
+    acc[0] += (T)a.x * vc.x - (T)a.y * vc.y;
+    acc[1] += (T)a.x * vc.y + (T)a.y * vc.x;
+  }
+}
+
+template <typename T, int RPL, int CH>
+__global__ __launch_bounds__(1024) void k_solve_fast(const typename Cx<T>::t* __restrict__ G,
+                                                     typename Cx<T>::t* __restrict__ X,
+                                                     const double2* __restrict__ cmy,
+                                                     const double2* __restrict__ cpy, int n, int ny, int64_t n2,
+                                                     int64_t N, int nru, int ncg, int cpg) {
+  typedef typename Cx<T>::t C;
+  typedef typename Unit<T>::t U;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* v = reinterpret_cast<C*>(smem_raw);  // n
+  C* part = v + n;                        // ncg * n
+  const int k = blockIdx.x;
+  const int64_t rhs = blockIdx.y;
+  C* x = X + rhs * N + (int64_t)k * n2;
+  const size_t nn = (size_t)n * n;
+  const C* Gk = G + (size_t)k * ny * nn;
+  const int tid = threadIdx.x;
+  const int g = tid / nru, ru = tid - g * nru;
+  const bool worker = g < ncg;
+  const int c0 = worker ? g * cpg : 0;
+  const int c1 = worker ? min(n, c0 + cpg) : 0;
+  const int ncol = max(0, c1 - c0);
+  const int nq = (ncol + CH - 1) / CH;
+  const int nsteps = 2 * ny - 1;
+  // unit base offset (in C elements) of this thread within a block
+  const size_t ubase = (size_t)ru * RPL + (size_t)n * c0;
+
+  auto blk_of = [&](int s) { return s <= ny - 1 ? s : 2 * ny - 2 - s; };
+  auto load_chunk = [&](const C* B, int q, U* buf) {
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      const int cc = q * CH + i;
+      if (cc < ncol) buf[i] = *reinterpret_cast<const U*>(B + ubase + (size_t)n * cc);
+    }
+  };
+
+  // v = t_0 = b_0
+  for (int i = tid; i < n; i += blockDim.x) v[i] = x[i];
+  U bufA[CH], bufB[CH];
+  if (worker && nq > 0) load_chunk(Gk, 0, bufA);
+  __syncthreads();
+
+  for (int s = 0; s < nsteps; ++s) {
+    const C* B = Gk + (size_t)blk_of(s) * nn;
+    const C* Bn = (s + 1 < nsteps) ? Gk + (size_t)blk_of(s + 1) * nn : nullptr;
+    // prefetch of the epilogue operand (b_{s+1} forward, t_{j-1} backward)
+    C pre;
+    pre.x = 0; pre.y = 0;
+    int slot_next = -1;
+    if (s < ny - 1) slot_next = s + 1;                       // forward: read b_{s+1}
+    else if (2 * ny - 2 - s > 0) slot_next = 2 * ny - 3 - s;  // backward: read t_{j-1}
+    if (tid < n && slot_next >= 0) pre = x[(size_t)n * slot_next + tid];
+
+    T acc[2 * RPL];
+#pragma unroll
+    for (int i = 0; i < 2 * RPL; ++i) acc[i] = 0;
+    if (worker) {
+      for (int q = 0; q < nq; ++q) {
+        // the next chunk: the rest of this block, else the head of the next one
+        if (q + 1 < nq) load_chunk(B, q + 1, bufB);
+        else if (Bn) load_chunk(Bn, 0, bufB);
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+          const int cc = q * CH + i;
+          if (cc < ncol) unit_fma<T, RPL>(bufA[i], v[c0 + cc], acc);
+        }
+#pragma unroll
+        for (int i = 0; i < CH; ++i) bufA[i] = bufB[i];
+      }
+#pragma unroll
+      for (int r = 0; r < RPL; ++r) {
+        C o;
+        o.x = acc[2 * r];
+        o.y = acc[2 * r + 1];
+        part[(size_t)g * n + ru * RPL + r] = o;
+      }
+    }
+    __syncthreads();
+    if (tid < n) {
+      C y;
+      y.x = 0; y.y = 0;
+      for (int q = 0; q < ncg; ++q) {
+        const C t = part[(size_t)q * n + tid];
+        y.x += t.x;
+        y.y += t.y;
+      }
+      C nv;
+      if (s < ny - 1) {
+        // t_{s+1} = b_{s+1} - cm_y(s+1) y
+        const double2 cm = cmy[s + 1];
+        nv.x = pre.x - (T)(cm.x * y.x - cm.y * y.y);
+        nv.y = pre.y - (T)(cm.x * y.y + cm.y * y.x);
+        x[(size_t)n * (s + 1) + tid] = nv;
+      } else {
+        const int j = 2 * ny - 2 - s;
+        x[(size_t)n * j + tid] = y;  // x_j
+        if (j > 0) {
+          // v for x_{j-1}: t_{j-1} - cp_y(j-1) x_j
+          const double2 cp = cpy[j - 1];
+          nv.x = pre.x - (T)(cp.x * y.x - cp.y * y.y);
+          nv.y = pre.y - (T)(cp.x * y.y + cp.y * y.x);
+        } else {
+          nv = y;
+        }
+      }
+      v[tid] = nv;
+    }
+    __syncthreads();
+  }
+}
+
 PlaneTabs plane_tabs(hz_problem* p) {
   PlaneTabs t;
   t.cmx = p->tab(0, 0);
@@ -479,6 +617,27 @@ void launch_plane_solve(hz_problem* p, int prec, int nrhs, void* x, bool adjoint
   const double2* cmy = p->tab(1, 0);
   const double2* cpy = p->tab(1, 1);
   dim3 grid((unsigned)p->nz, (unsigned)nrhs);
+  if (!adjoint && ny >= 1) {
+    constexpr int CH = 8;
+    if (prec == HZ_PREC_C64 && n % 2 == 0) {
+      const int nru = n / 2, ncg = std::max(1, 1024 / nru), cpg = (n + ncg - 1) / ncg;
+      const size_t lds = sizeof(float2) * (size_t)n * (1 + ncg);
+      HZ_HIP(hipFuncSetAttribute((const void*)&k_solve_fast<float, 2, CH>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      hipLaunchKernelGGL((k_solve_fast<float, 2, CH>), grid, dim3(1024), lds, s, f.G.as<float2>(), (float2*)x, cmy, cpy,
+                         n, ny, p->n2, p->N, nru, ncg, cpg);
+      HZ_HIP(hipGetLastError());
+      return;
+    }
+    if (prec == HZ_PREC_FP64) {
+      const int nru = n, ncg = std::max(1, 1024 / nru), cpg = (n + ncg - 1) / ncg;
+      const size_t lds = sizeof(double2) * (size_t)n * (1 + ncg);
+      HZ_HIP(hipFuncSetAttribute((const void*)&k_solve_fast<double, 1, CH>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      hipLaunchKernelGGL((k_solve_fast<double, 1, CH>), grid, dim3(1024), lds, s, f.G.as<double2>(), (double2*)x, cmy,
+                         cpy, n, ny, p->n2, p->N, nru, ncg, cpg);
+      HZ_HIP(hipGetLastError());
+      return;
+    }
+  }
   if (prec == HZ_PREC_FP64) {
     const size_t lds = sizeof(double2) * (size_t)n * (2 + ng);
     HZ_CHECK(lds <= 160 * 1024, HZ_ERR_ARG, "nx too large for the apply kernel");

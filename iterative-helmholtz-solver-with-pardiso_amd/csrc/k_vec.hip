@@ -10,7 +10,7 @@
 namespace hz {
 namespace {
 
-constexpr int RB = 512;   // stage-1 blocks per vector
+constexpr int RB = 1024;  // stage-1 blocks per vector
 constexpr int RT = 256;   // threads per block
 
 struct Box {
@@ -78,6 +78,33 @@ __global__ __launch_bounds__(RT) void k_dot_final(const double2* __restrict__ pa
   }
   const double2 s = block_sum(make_double2(sx, sy));
   if (threadIdx.x == 0) out[v] = mode == 1 ? make_double2(sqrt(s.x), 0.0) : s;
+}
+
+// w -= coef * x, then partial[b] = sum conj(y) w  (y == nullptr: sum |w|^2):
+// one modified Gram-Schmidt update fused with the next coefficient's dot.
+__global__ __launch_bounds__(RT) void k_axpy_dot_partial(int64_t n, const double2* __restrict__ coef,
+                                                         const double2* __restrict__ x, double2* __restrict__ w,
+                                                         const double2* __restrict__ y,
+                                                         double2* __restrict__ partial) {
+  const double2 a = coef[0];
+  double sx = 0.0, sy = 0.0;
+  const int64_t stride = (int64_t)RB * RT;
+  for (int64_t l = (int64_t)blockIdx.x * RT + threadIdx.x; l < n; l += stride) {
+    const double2 xv = x[l];
+    double2 wv = w[l];
+    wv.x -= a.x * xv.x - a.y * xv.y;
+    wv.y -= a.x * xv.y + a.y * xv.x;
+    w[l] = wv;
+    if (y) {
+      const double2 b = y[l];
+      sx += b.x * wv.x + b.y * wv.y;
+      sy += b.x * wv.y - b.y * wv.x;
+    } else {
+      sx += wv.x * wv.x + wv.y * wv.y;
+    }
+  }
+  const double2 s = block_sum(make_double2(sx, sy));
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
 __global__ void k_axpy_dev(int64_t n, const double2* __restrict__ coef, double sgn, int cj,
@@ -206,6 +233,14 @@ void launch_dot_multi(hz_problem* p, int64_t n, int nv, const double2* x, int64_
 void launch_dot(hz_problem* p, int64_t n, const double2* x, const double2* y, int mask, double2* out,
                 hipStream_t s) {
   launch_dot_multi(p, n, 1, x, 0, y, 0, mask, out, s);
+}
+
+void launch_axpy_dot(hz_problem* p, int64_t n, const double2* coef, const double2* x, double2* w,
+                     const double2* y, double2* out, hipStream_t s) {
+  ensure_red(p, 1);
+  hipLaunchKernelGGL(k_axpy_dot_partial, dim3(RB), dim3(RT), 0, s, n, coef, x, w, y, p->red.as<double2>());
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RT), 0, s, p->red.as<double2>(), y ? 0 : 1, out);
+  HZ_HIP(hipGetLastError());
 }
 
 void launch_axpy_dev(int64_t n, const double2* coef, bool neg, bool conj_coef, const double2* x, double2* y,

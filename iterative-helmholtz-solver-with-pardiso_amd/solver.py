@@ -166,16 +166,17 @@ class Solver:
         stats = [_stats(st[q], hist if q == 0 else None) for q in range(nr)]
         return u, (stats[0] if nr == 1 else stats)
 
-    def bicgstab(self, f, prec=PREC_FP64, maxit=200, tol=1e-8, tol_abs=False, mask=MASK_FULL):
+    def bicgstab(self, f, prec=PREC_FP64, maxit=200, tol=1e-8, tol_abs=False, mask=MASK_FULL, hist_cap=8192):
         f, nr = self._vec(f)
         u = _torch().zeros_like(f)
         st = (abi.KrylovStats * nr)()
+        hist = np.zeros(hist_cap)
         _torch().cuda.synchronize()
         rc = abi.lib().hz_bicgstab(self.h, int(prec), nr, _ptr(f), _ptr(u), int(maxit), float(tol), int(tol_abs),
-                                   int(mask), st)
+                                   int(mask), hist.ctypes.data_as(C.POINTER(C.c_double)), hist_cap, st)
         if rc not in (abi.HZ_OK, 5):
             abi.check(rc, "hz_bicgstab")
-        stats = [_stats(st[q]) for q in range(nr)]
+        stats = [_stats(st[q], hist if q == 0 else None) for q in range(nr)]
         return u, (stats[0] if nr == 1 else stats)
 
     # ----------------------------------------------------------- adjoint

@@ -97,6 +97,7 @@ struct OrProblem {
   // plane factors (banded LU)
   bool factored = false;
   int64_t kl = 0, ku = 0, ldab = 0;
+  int64_t k_off = 0;                      // first plane held in ab/ipiv
   std::vector<cd> ab;                     // nz * ldab * n2
   std::vector<int64_t> ipiv;              // nz * n2
 };
@@ -311,8 +312,8 @@ static inline cd& AB(std::vector<cd>& ab, int64_t base, int64_t ldab, int64_t kv
 static void plane_factor(OrProblem& p, int64_t k) {
   int64_t n = p.n2, nx = p.nx, ny = p.ny;
   int64_t kl = p.kl, ku = p.ku, kv = kl + ku, ldab = p.ldab;
-  int64_t base = k * ldab * n;
-  int64_t* ipiv = &p.ipiv[k * n];
+  int64_t base = (k - p.k_off) * ldab * n;
+  int64_t* ipiv = &p.ipiv[(k - p.k_off) * n];
   for (int64_t c = 0; c < n; ++c)
     for (int64_t r = 0; r < ldab; ++r) p.ab[base + r + c * ldab] = 0.0;
   // scatter the matrix
@@ -357,8 +358,8 @@ static void plane_factor(OrProblem& p, int64_t k) {
 // functions.cpp:5959.
 static void plane_solve(OrProblem& p, int64_t k, cd* b, bool adjoint) {
   int64_t n = p.n2, kl = p.kl, ku = p.ku, kv = kl + ku, ldab = p.ldab;
-  int64_t base = k * ldab * n;
-  const int64_t* ipiv = &p.ipiv[k * n];
+  int64_t base = (k - p.k_off) * ldab * n;
+  const int64_t* ipiv = &p.ipiv[(k - p.k_off) * n];
   if (!adjoint) {
     for (int64_t j = 0; j < n - 1; ++j) {
       int64_t km = std::min(kl, n - 1 - j);
@@ -392,6 +393,7 @@ static void plane_solve(OrProblem& p, int64_t k, cd* b, bool adjoint) {
 
 static void ensure_factored(OrProblem& p) {
   if (p.factored) return;
+  p.k_off = 0;
   p.kl = p.nx; p.ku = p.nx; p.ldab = 2 * p.kl + p.ku + 1;
   p.ab.assign((size_t)(p.nz * p.ldab * p.n2), cd(0, 0));
   p.ipiv.assign((size_t)(p.nz * p.n2), 0);
@@ -878,14 +880,15 @@ void or_plane_solve(void* h, int64_t k, double* b, int adjoint) {
   plane_solve(*p, k, (cd*)b, adjoint != 0);
 }
 void or_factor(void* h) { ensure_factored(*(OrProblem*)h); }
-// Factor only planes [k0, k1) (CPU baseline sampling); does not mark the problem factored.
+// Factor only planes [k0, k1) (CPU baseline sampling): holds just those planes
+// and leaves the problem unfactored for the full-path functions.
 void or_factor_planes(void* h, int64_t k0, int64_t k1) {
   OrProblem* p = (OrProblem*)h;
+  p->factored = false;
+  p->k_off = k0;
   p->kl = p->nx; p->ku = p->nx; p->ldab = 2 * p->kl + p->ku + 1;
-  if (p->ab.size() != (size_t)(p->nz * p->ldab * p->n2)) {
-    p->ab.assign((size_t)(p->nz * p->ldab * p->n2), cd(0, 0));
-    p->ipiv.assign((size_t)(p->nz * p->n2), 0);
-  }
+  p->ab.assign((size_t)((k1 - k0) * p->ldab * p->n2), cd(0, 0));
+  p->ipiv.assign((size_t)((k1 - k0) * p->n2), 0);
 #pragma omp parallel for schedule(dynamic)
   for (int64_t k = k0; k < k1; ++k) plane_factor(*p, k);
 }

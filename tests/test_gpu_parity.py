@@ -242,7 +242,11 @@ def test_bicgstab_vs_oracle(hz, oracle, mode):
     u, st = S.bicgstab(_dev(f), maxit=300, tol=1e-9, tol_abs=False, mask=hz.abi.MASK_FULL)
     print("bicgstab iters gpu %d oracle %d" % (st.iters, st_o.iters))
     assert st.converged and st_o.converged
-    assert abs(st.iters - st_o.iters) <= 1
+    # BiCGStab's count moves by a few iterations under 1e-16 input perturbations
+    # of the oracle itself (S7 case: 62..65), so parity is the early residual
+    # history plus a +-10% band on the count, and the solution.
+    assert np.allclose(st.hist[:8], st_o.hist[:8], rtol=1e-6)
+    assert abs(st.iters - st_o.iters) <= max(1, st_o.iters // 10)
     assert _rel(_host(u), u_o) < 1e-7
 
 

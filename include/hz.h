@@ -66,7 +66,12 @@ enum {
 };
 /* Krylov options */
 enum { HZ_DRIVER_FGMRES = 0, HZ_DRIVER_GMRES_REF = 1 };
-enum { HZ_ORTHO_REF_CONJ = 0, HZ_ORTHO_MGS = 1 };
+/* Arnoldi orthogonalisation.  GMRES_REF: REF_CONJ is the reference's w^H V_i
+ * coefficient (FGMRES.cpp:511), MGS the textbook V_i^H w.  FGMRES: MGS is
+ * modified Gram-Schmidt in its inverse compact-WY form (coefficients
+ * (I + L)^{-1} V^H w, L = strict lower part of V^H V: two passes over the
+ * basis per step), MGS_SEQ the sequential form (one pass per basis vector). */
+enum { HZ_ORTHO_REF_CONJ = 0, HZ_ORTHO_MGS = 1, HZ_ORTHO_MGS_SEQ = 2 };
 enum { HZ_MASK_FULL = 0, HZ_MASK_PHYS = 1 };
 
 /* Problem descriptor.  Replaces the per-axis size_m structs (definitions.h:56-68)
@@ -93,7 +98,7 @@ typedef struct {
   int32_t driver;       /* HZ_DRIVER_* */
   int32_t m;            /* restart length (FGMRES niter, main.cpp:150) */
   int32_t max_restarts; /* restart count (main.cpp:151) */
-  int32_t ortho;        /* GMRES_REF: HZ_ORTHO_* */
+  int32_t ortho;        /* HZ_ORTHO_* */
   int32_t mask;         /* HZ_MASK_* residual box */
   int32_t monitor;      /* GMRES_REF: the reference's per-iteration u = L0^{-1} x (FGMRES.cpp:601) */
   double tol;           /* relative residual target */

@@ -11,7 +11,7 @@ ERRORS = {1: "HZ_ERR_ARG", 2: "HZ_ERR_HIP", 3: "HZ_ERR_STATE", 4: "HZ_ERR_NOMEM"
           6: "HZ_ERR_BREAKDOWN", 7: "HZ_ERR_SINGULAR", 8: "HZ_ERR_NONFINITE"}
 PREC_FP64, PREC_C64 = 0, 1
 DRIVER_FGMRES, DRIVER_GMRES_REF = 0, 1
-ORTHO_REF_CONJ, ORTHO_MGS = 0, 1
+ORTHO_REF_CONJ, ORTHO_MGS, ORTHO_MGS_SEQ = 0, 1, 2
 MASK_FULL, MASK_PHYS = 0, 1
 
 

@@ -183,7 +183,7 @@ static void ensure_krylov(hz_problem* p, int m) {
   grow(p->kw, vb);
   grow(p->kx, vb);
   grow(p->kaux, vb);
-  grow(p->scal, std::max<size_t>(4096, (size_t)(m + 2) * (m + 2) + 256) * sizeof(double2));
+  grow(p->scal, std::max<size_t>(4096, (size_t)(m + 2) * (m + 2) + 4 * (m + 2) + 256) * sizeof(double2));
 }
 
 struct HistW {
@@ -220,8 +220,10 @@ static void fgmres_one(hz_problem* p, const double2* f, double2* x, const hz_kry
     sync(p);
     return;
   }
-  std::vector<zc> H((size_t)ldh * m), cs(m), sn(m), g(m + 1), y(m);
-  std::vector<double2> hcol(m + 2), yd(m);
+  std::vector<zc> H((size_t)ldh * m), cs(m), sn(m), g(m + 1), y(m), Lg((size_t)ldh * ldh);
+  std::vector<double2> hcol(m + 2), yd(m), rl(2 * (m + 1));
+  double2* dR = dY + m;       // r = V^H w   (m + 1)
+  double2* dL = dR + m + 1;   // V^H V_j     (m + 1)
   double rel = 0.0, res = 0.0;
   int total = 0;
   bool conv = false, brk = false;
@@ -250,14 +252,35 @@ static void fgmres_one(hz_problem* p, const double2* f, double2* x, const hz_kry
       precond_apply(p, o.prec, 1, Vj, Zj, adjoint, false);
       op_apply(p, 1, Zj, w, adjoint);
       total++;
-      // modified Gram-Schmidt: h_ij = V_i^H w, w -= h_ij V_i in sequence; each
-      // update is fused with the next coefficient's dot (the last with ||w||)
       double2* hn_d = dH + (j + 1) + (size_t)ldh * j;
-      launch_dot(p, N, V, w, HZ_MASK_FULL, dH + (size_t)ldh * j, p->stream);
-      for (int i = 0; i <= j; ++i) {
-        double2* hij = dH + i + (size_t)ldh * j;
-        const double2* vnext = i < j ? V + (size_t)N * (i + 1) : nullptr;
-        launch_axpy_dot(p, N, hij, V + (size_t)N * i, w, vnext, hij + 1, p->stream);
+      if (o.ortho == HZ_ORTHO_MGS_SEQ) {
+        // modified Gram-Schmidt: h_ij = V_i^H w, w -= h_ij V_i in sequence; each
+        // update is fused with the next coefficient's dot (the last with ||w||)
+        launch_dot(p, N, V, w, HZ_MASK_FULL, dH + (size_t)ldh * j, p->stream);
+        for (int i = 0; i <= j; ++i) {
+          double2* hij = dH + i + (size_t)ldh * j;
+          const double2* vnext = i < j ? V + (size_t)N * (i + 1) : nullptr;
+          launch_axpy_dot(p, N, hij, V + (size_t)N * i, w, vnext, hij + 1, p->stream);
+        }
+      } else {
+        // modified Gram-Schmidt in inverse compact-WY form: r = V^H w and the new
+        // row l = V_j^H V_q of the Gram matrix in one pass, h = (I + L)^{-1} r on
+        // the host (L strict lower part of V^H V), then w -= V h and ||w|| in a
+        // second pass.  Same coefficients as the sequential form in exact
+        // arithmetic and the same O(eps) loss of orthogonality.
+        launch_multidot(p, N, j + 1, V, N, w, V + (size_t)N * j, dR, dL, p->stream);
+        HZ_HIP(hipMemcpyAsync(rl.data(), dR, (j + 1) * sizeof(double2), hipMemcpyDeviceToHost, p->stream));
+        HZ_HIP(hipMemcpyAsync(rl.data() + (j + 1), dL, (j + 1) * sizeof(double2), hipMemcpyDeviceToHost, p->stream));
+        sync(p);
+        for (int q = 0; q < j; ++q) Lg[(size_t)j * ldh + q] = std::conj(zz(rl[j + 1 + q]));  // V_j^H V_q
+        for (int i = 0; i <= j; ++i) {
+          zc acc = zz(rl[i]);
+          for (int q = 0; q < i; ++q) acc -= Lg[(size_t)i * ldh + q] * zz(hcol[q]);
+          hcol[i] = d2(acc);
+        }
+        HZ_HIP(hipMemcpyAsync(dH + (size_t)ldh * j, hcol.data(), (j + 1) * sizeof(double2), hipMemcpyHostToDevice,
+                              p->stream));
+        launch_multi_axpy_norm(p, N, j + 1, V, N, dH + (size_t)ldh * j, w, hn_d, p->stream);
       }
       HZ_HIP(hipMemcpyAsync(hcol.data(), dH + (size_t)ldh * j, (j + 2) * sizeof(double2), hipMemcpyDeviceToHost,
                             p->stream));

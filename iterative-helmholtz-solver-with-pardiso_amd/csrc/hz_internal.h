@@ -157,6 +157,12 @@ void launch_axpy_dev(int64_t n, const double2* coef, bool neg, bool conj_coef, c
 // w -= coef[0] * x; out = y^H w (y == nullptr: out = ||w||), one pass
 void launch_axpy_dot(hz_problem* p, int64_t n, const double2* coef, const double2* x, double2* w,
                      const double2* y, double2* out, hipStream_t s);
+// out_r[q] = V_q^H w and (vj) out_l[q] = V_q^H vj for q < nv, one pass over V
+void launch_multidot(hz_problem* p, int64_t n, int nv, const double2* V, int64_t vstride, const double2* w,
+                     const double2* vj, double2* out_r, double2* out_l, hipStream_t s);
+// w -= sum_q coef[q] V_q; out_norm = ||w||, one pass over V
+void launch_multi_axpy_norm(hz_problem* p, int64_t n, int nv, const double2* V, int64_t vstride,
+                            const double2* coef, double2* w, double2* out_norm, hipStream_t s);
 // y = x / |coef| (coef holds a norm in .x)
 void launch_scale_inv_dev(int64_t n, const double2* coef, const double2* x, double2* y, hipStream_t s);
 // y = sum_i c[i] * X_i  + (accumulate ? y : 0)

@@ -551,6 +551,120 @@ __global__ __launch_bounds__(1024) void k_solve_fast(const typename Cx<T>::t* __
   }
 }
 
+// Prefetching variant: each worker thread loads up to PF units of a block with
+// unconditional (index-clamped) loads and masks the tail in the FMA, so the
+// compiler keeps the loads in flight; the first PF units of the NEXT block are
+// issued right after the current block's FMAs, before the partial-sum write,
+// the barriers and the serial epilogue of the step.  With PF = 20 and 1024
+// threads one 320 KB complex64 block per CU is in flight across the step
+// boundary.
+template <typename T, int RPL, int PF>
+__global__ __launch_bounds__(1024) void k_solve_pf(const typename Cx<T>::t* __restrict__ G,
+                                                   typename Cx<T>::t* __restrict__ X,
+                                                   const double2* __restrict__ cmy, const double2* __restrict__ cpy,
+                                                   int n, int ny, int64_t n2, int64_t N, int nru, int ncg, int cpg) {
+  typedef typename Cx<T>::t C;
+  typedef typename Unit<T>::t U;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* v = reinterpret_cast<C*>(smem_raw);  // n
+  C* part = v + n;                        // ncg * n
+  const int k = blockIdx.x;
+  const int64_t rhs = blockIdx.y;
+  C* x = X + rhs * N + (int64_t)k * n2;
+  const size_t nn = (size_t)n * n;
+  const C* Gk = G + (size_t)k * ny * nn;
+  const int tid = threadIdx.x;
+  const int g = tid / nru, ru = tid - g * nru;
+  const int c0 = g * cpg;
+  const bool worker = (g < ncg) && (c0 < n);
+  const int ncol = worker ? min(n, c0 + cpg) - c0 : 0;
+  const int nchunk = (ncol + PF - 1) / PF;
+  const int nsteps = 2 * ny - 1;
+  const size_t ubase = (size_t)ru * RPL + (size_t)n * c0;
+
+  U buf[PF];
+  auto issue = [&](const C* B, int chunk) {
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      const int cc = min(chunk * PF + i, ncol - 1);
+      buf[i] = *reinterpret_cast<const U*>(B + ubase + (size_t)n * cc);
+    }
+  };
+  T acc[2 * RPL];
+  auto consume = [&](int chunk) {
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      const int cc = chunk * PF + i;
+      C vc = v[c0 + min(cc, ncol - 1)];
+      const bool ok = cc < ncol;
+      vc.x = ok ? vc.x : (T)0;
+      vc.y = ok ? vc.y : (T)0;
+      unit_fma<T, RPL>(buf[i], vc, acc);
+    }
+  };
+  auto blk_of = [&](int s) { return s <= ny - 1 ? s : 2 * ny - 2 - s; };
+
+  for (int i = tid; i < n; i += blockDim.x) v[i] = x[i];
+  if (worker) issue(Gk, 0);
+  __syncthreads();
+
+  for (int s = 0; s < nsteps; ++s) {
+    const C* B = Gk + (size_t)blk_of(s) * nn;
+    C pre;
+    pre.x = 0; pre.y = 0;
+    int slot_next = -1;
+    if (s < ny - 1) slot_next = s + 1;
+    else if (2 * ny - 2 - s > 0) slot_next = 2 * ny - 3 - s;
+    if (tid < n && slot_next >= 0) pre = x[(size_t)n * slot_next + tid];
+#pragma unroll
+    for (int i = 0; i < 2 * RPL; ++i) acc[i] = 0;
+    if (worker) {
+      consume(0);
+      for (int ch = 1; ch < nchunk; ++ch) {
+        issue(B, ch);
+        consume(ch);
+      }
+      if (s + 1 < nsteps) issue(Gk + (size_t)blk_of(s + 1) * nn, 0);
+#pragma unroll
+      for (int r = 0; r < RPL; ++r) {
+        C o;
+        o.x = acc[2 * r];
+        o.y = acc[2 * r + 1];
+        part[(size_t)g * n + ru * RPL + r] = o;
+      }
+    }
+    __syncthreads();
+    if (tid < n) {
+      C y;
+      y.x = 0; y.y = 0;
+      for (int q = 0; q < ncg; ++q) {
+        const C t = part[(size_t)q * n + tid];
+        y.x += t.x;
+        y.y += t.y;
+      }
+      C nv;
+      if (s < ny - 1) {
+        const double2 cm = cmy[s + 1];
+        nv.x = pre.x - (T)(cm.x * y.x - cm.y * y.y);
+        nv.y = pre.y - (T)(cm.x * y.y + cm.y * y.x);
+        x[(size_t)n * (s + 1) + tid] = nv;
+      } else {
+        const int j = 2 * ny - 2 - s;
+        x[(size_t)n * j + tid] = y;
+        if (j > 0) {
+          const double2 cp = cpy[j - 1];
+          nv.x = pre.x - (T)(cp.x * y.x - cp.y * y.y);
+          nv.y = pre.y - (T)(cp.x * y.y + cp.y * y.x);
+        } else {
+          nv = y;
+        }
+      }
+      v[tid] = nv;
+    }
+    __syncthreads();
+  }
+}
+
 PlaneTabs plane_tabs(hz_problem* p) {
   PlaneTabs t;
   t.cmx = p->tab(0, 0);
@@ -618,12 +732,12 @@ void launch_plane_solve(hz_problem* p, int prec, int nrhs, void* x, bool adjoint
   const double2* cpy = p->tab(1, 1);
   dim3 grid((unsigned)p->nz, (unsigned)nrhs);
   if (!adjoint && ny >= 1) {
-    constexpr int CH = 8;
+    constexpr int PF = 20;
     if (prec == HZ_PREC_C64 && n % 2 == 0) {
       const int nru = n / 2, ncg = std::max(1, 1024 / nru), cpg = (n + ncg - 1) / ncg;
       const size_t lds = sizeof(float2) * (size_t)n * (1 + ncg);
-      HZ_HIP(hipFuncSetAttribute((const void*)&k_solve_fast<float, 2, CH>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      hipLaunchKernelGGL((k_solve_fast<float, 2, CH>), grid, dim3(1024), lds, s, f.G.as<float2>(), (float2*)x, cmy, cpy,
+      HZ_HIP(hipFuncSetAttribute((const void*)&k_solve_pf<float, 2, PF>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      hipLaunchKernelGGL((k_solve_pf<float, 2, PF>), grid, dim3(1024), lds, s, f.G.as<float2>(), (float2*)x, cmy, cpy,
                          n, ny, p->n2, p->N, nru, ncg, cpg);
       HZ_HIP(hipGetLastError());
       return;
@@ -631,8 +745,8 @@ void launch_plane_solve(hz_problem* p, int prec, int nrhs, void* x, bool adjoint
     if (prec == HZ_PREC_FP64) {
       const int nru = n, ncg = std::max(1, 1024 / nru), cpg = (n + ncg - 1) / ncg;
       const size_t lds = sizeof(double2) * (size_t)n * (1 + ncg);
-      HZ_HIP(hipFuncSetAttribute((const void*)&k_solve_fast<double, 1, CH>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      hipLaunchKernelGGL((k_solve_fast<double, 1, CH>), grid, dim3(1024), lds, s, f.G.as<double2>(), (double2*)x, cmy,
+      HZ_HIP(hipFuncSetAttribute((const void*)&k_solve_pf<double, 1, PF>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      hipLaunchKernelGGL((k_solve_pf<double, 1, PF>), grid, dim3(1024), lds, s, f.G.as<double2>(), (double2*)x, cmy,
                          cpy, n, ny, p->n2, p->N, nru, ncg, cpg);
       HZ_HIP(hipGetLastError());
       return;

@@ -107,6 +107,116 @@ __global__ __launch_bounds__(RT) void k_axpy_dot_partial(int64_t n, const double
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+
+// ------------------------------------------------ one-pass Gram-Schmidt blocks
+// Multi-dot over a group of MD vectors: r_q = V_q^H w and (vj != nullptr)
+// l_q = V_q^H vj for q in the group; partial[(group * RB + block) * 2 MD + ...].
+constexpr int MD = 8;
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+__global__ __launch_bounds__(RT) void k_multidot_partial(int64_t n, int nv, const double2* __restrict__ V,
+                                                         int64_t vs, const double2* __restrict__ w,
+                                                         const double2* __restrict__ vj,
+                                                         double2* __restrict__ partial) {
+  const int group = blockIdx.y;
+  const int q0 = group * MD;
+  const int nq = min(MD, nv - q0);
+  double ar[MD][2], al[MD][2];
+#pragma unroll
+  for (int g = 0; g < MD; ++g) { ar[g][0] = ar[g][1] = al[g][0] = al[g][1] = 0.0; }
+  const int64_t stride = (int64_t)RB * RT;
+  for (int64_t l = (int64_t)blockIdx.x * RT + threadIdx.x; l < n; l += stride) {
+    const double2 wv = w[l];
+    const double2 jv = vj ? vj[l] : make_double2(0, 0);
+#pragma unroll
+    for (int g = 0; g < MD; ++g) {
+      if (g < nq) {
+        const double2 a = V[(int64_t)(q0 + g) * vs + l];
+        ar[g][0] += a.x * wv.x + a.y * wv.y;
+        ar[g][1] += a.x * wv.y - a.y * wv.x;
+        al[g][0] += a.x * jv.x + a.y * jv.y;
+        al[g][1] += a.x * jv.y - a.y * jv.x;
+      }
+    }
+  }
+  __shared__ double sm[RT / 64][MD * 4];
+  const int lane = threadIdx.x & 63, wv_ = threadIdx.x >> 6;
+#pragma unroll
+  for (int g = 0; g < MD; ++g) {
+    const double s0 = wave_sum(ar[g][0]), s1 = wave_sum(ar[g][1]);
+    const double s2 = wave_sum(al[g][0]), s3 = wave_sum(al[g][1]);
+    if (lane == 0) { sm[wv_][4 * g] = s0; sm[wv_][4 * g + 1] = s1; sm[wv_][4 * g + 2] = s2; sm[wv_][4 * g + 3] = s3; }
+  }
+  __syncthreads();
+  if (threadIdx.x < MD * 4) {
+    double t = 0.0;
+    for (int i = 0; i < RT / 64; ++i) t += sm[i][threadIdx.x];
+    // layout: [r_0..r_{MD-1}, l_0..l_{MD-1}] as double2
+    const int g = threadIdx.x / 4, c = threadIdx.x % 4;
+    double* out = reinterpret_cast<double*>(partial + ((int64_t)group * RB + blockIdx.x) * 2 * MD);
+    const int slot = (c < 2) ? g : MD + g;
+    out[2 * slot + (c & 1)] = t;
+  }
+}
+
+// out_r[q], out_l[q] = sums over the RB partials of vector q
+__global__ __launch_bounds__(RT) void k_multidot_final(const double2* __restrict__ partial, int nv,
+                                                       double2* __restrict__ out_r, double2* __restrict__ out_l) {
+  const int q = blockIdx.x;
+  const int group = q / MD, g = q % MD;
+  double sx = 0, sy = 0, tx = 0, ty = 0;
+  for (int b = threadIdx.x; b < RB; b += RT) {
+    const double2* pb = partial + ((int64_t)group * RB + b) * 2 * MD;
+    sx += pb[g].x; sy += pb[g].y;
+    tx += pb[MD + g].x; ty += pb[MD + g].y;
+  }
+  const double2 s = block_sum(make_double2(sx, sy));
+  __syncthreads();
+  const double2 t = block_sum(make_double2(tx, ty));
+  if (threadIdx.x == 0) {
+    out_r[q] = s;
+    if (out_l) out_l[q] = t;
+  }
+}
+
+// w -= sum_q coef_q V_q, partial[b] = sum |w|^2
+__global__ __launch_bounds__(RT) void k_multi_axpy_norm_partial(int64_t n, int nv, const double2* __restrict__ V,
+                                                                int64_t vs, const double2* __restrict__ coef,
+                                                                double2* __restrict__ w,
+                                                                double2* __restrict__ partial) {
+  extern __shared__ double2 cf[];
+  for (int q = threadIdx.x; q < nv; q += RT) cf[q] = coef[q];
+  __syncthreads();
+  double acc = 0.0;
+  const int64_t stride = (int64_t)RB * RT;
+  for (int64_t l = (int64_t)blockIdx.x * RT + threadIdx.x; l < n; l += stride) {
+    double2 s = w[l];
+    int q = 0;
+    for (; q + 3 < nv; q += 4) {
+      const double2 a0 = V[(int64_t)q * vs + l], a1 = V[(int64_t)(q + 1) * vs + l];
+      const double2 a2 = V[(int64_t)(q + 2) * vs + l], a3 = V[(int64_t)(q + 3) * vs + l];
+      const double2 c0 = cf[q], c1 = cf[q + 1], c2 = cf[q + 2], c3 = cf[q + 3];
+      s.x -= c0.x * a0.x - c0.y * a0.y; s.y -= c0.x * a0.y + c0.y * a0.x;
+      s.x -= c1.x * a1.x - c1.y * a1.y; s.y -= c1.x * a1.y + c1.y * a1.x;
+      s.x -= c2.x * a2.x - c2.y * a2.y; s.y -= c2.x * a2.y + c2.y * a2.x;
+      s.x -= c3.x * a3.x - c3.y * a3.y; s.y -= c3.x * a3.y + c3.y * a3.x;
+    }
+    for (; q < nv; ++q) {
+      const double2 a = V[(int64_t)q * vs + l], c = cf[q];
+      s.x -= c.x * a.x - c.y * a.y;
+      s.y -= c.x * a.y + c.y * a.x;
+    }
+    w[l] = s;
+    acc += s.x * s.x + s.y * s.y;
+  }
+  const double2 t = block_sum(make_double2(acc, 0.0));
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
 __global__ void k_axpy_dev(int64_t n, const double2* __restrict__ coef, double sgn, int cj,
                            const double2* __restrict__ x, double2* __restrict__ y) {
   double2 a = coef[0];
@@ -240,6 +350,29 @@ void launch_axpy_dot(hz_problem* p, int64_t n, const double2* coef, const double
   ensure_red(p, 1);
   hipLaunchKernelGGL(k_axpy_dot_partial, dim3(RB), dim3(RT), 0, s, n, coef, x, w, y, p->red.as<double2>());
   hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RT), 0, s, p->red.as<double2>(), y ? 0 : 1, out);
+  HZ_HIP(hipGetLastError());
+}
+
+void launch_multidot(hz_problem* p, int64_t n, int nv, const double2* V, int64_t vstride, const double2* w,
+                     const double2* vj, double2* out_r, double2* out_l, hipStream_t s) {
+  const int ng = (nv + MD - 1) / MD;
+  const size_t need = (size_t)ng * RB * 2 * MD * sizeof(double2);
+  if (p->red.bytes < need) {
+    p->red.release();
+    p->red.alloc(need);
+  }
+  hipLaunchKernelGGL(k_multidot_partial, dim3(RB, ng), dim3(RT), 0, s, n, nv, V, vstride, w, vj,
+                     p->red.as<double2>());
+  hipLaunchKernelGGL(k_multidot_final, dim3(nv), dim3(RT), 0, s, p->red.as<double2>(), nv, out_r, out_l);
+  HZ_HIP(hipGetLastError());
+}
+
+void launch_multi_axpy_norm(hz_problem* p, int64_t n, int nv, const double2* V, int64_t vstride,
+                            const double2* coef, double2* w, double2* out_norm, hipStream_t s) {
+  ensure_red(p, 1);
+  hipLaunchKernelGGL(k_multi_axpy_norm_partial, dim3(RB), dim3(RT), nv * sizeof(double2), s, n, nv, V, vstride, coef,
+                     w, p->red.as<double2>());
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RT), 0, s, p->red.as<double2>(), 1, out_norm);
   HZ_HIP(hipGetLastError());
 }
 

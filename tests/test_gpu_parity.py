@@ -220,7 +220,14 @@ def test_fgmres_fp64_and_mixed_vs_oracle(hz, oracle):
     u_o, st_o = P.fgmres(f, m=40, rest=10, tol=1e-10)
     u64, st64 = S.fgmres(_dev(f), m=40, restarts=10, tol=1e-10, prec=hz.abi.PREC_FP64)
     u32, st32 = S.fgmres(_dev(f), m=40, restarts=10, tol=1e-10, prec=hz.abi.PREC_C64)
-    print("fgmres iters oracle %d fp64 %d mixed %d" % (st_o.iters, st64.iters, st32.iters))
+    useq, stseq = S.fgmres(_dev(f), m=40, restarts=10, tol=1e-10, prec=hz.abi.PREC_FP64,
+                           ortho=hz.abi.ORTHO_MGS_SEQ)
+    print("fgmres iters oracle %d fp64 %d (sequential MGS %d) mixed %d" % (st_o.iters, st64.iters, stseq.iters,
+                                                                           st32.iters))
+    # the one-pass (compact-WY) and sequential MGS give the same Arnoldi process
+    k = min(len(st64.hist), len(stseq.hist))
+    assert abs(st64.iters - stseq.iters) <= 1
+    assert np.allclose(st64.hist[:k - 2], stseq.hist[:k - 2], rtol=1e-6)
     assert st64.converged and st32.converged and st_o.converged
     assert abs(st64.iters - st_o.iters) <= 1
     assert np.allclose(st64.hist[:10], st_o.hist[:10], rtol=1e-8)

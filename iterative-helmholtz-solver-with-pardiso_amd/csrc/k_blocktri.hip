@@ -23,6 +23,9 @@
 //   backward x_{ny-1} = y_{ny-1},  x_j = y_j - G_j^H conj(cm_y(j+1)) x_{j+1}.
 // One workgroup per (plane, right-hand side); each step is a GEMV that streams
 // the nx x nx block G_j once from HBM.
+#include <cstdlib>
+#include <cstring>
+
 #include "hz_internal.h"
 
 namespace hz {
@@ -551,20 +554,27 @@ __global__ __launch_bounds__(1024) void k_solve_fast(const typename Cx<T>::t* __
   }
 }
 
-// Prefetching variant: each worker thread loads up to PF units of a block with
-// unconditional (index-clamped) loads and masks the tail in the FMA, so the
-// compiler keeps the loads in flight; the first PF units of the NEXT block are
-// issued right after the current block's FMAs, before the partial-sum write,
-// the barriers and the serial epilogue of the step.  With PF = 20 and 1024
-// threads one 320 KB complex64 block per CU is in flight across the step
-// boundary.
-template <typename T, int RPL, int PF>
-__global__ __launch_bounds__(1024) void k_solve_pf(const typename Cx<T>::t* __restrict__ G,
-                                                   typename Cx<T>::t* __restrict__ X,
-                                                   const double2* __restrict__ cmy, const double2* __restrict__ cpy,
-                                                   int n, int ny, int64_t n2, int64_t N, int nru, int ncg, int cpg) {
+// Prefetching variant with a compile-time column count.  The column groups
+// divide nx exactly, so every worker thread owns CPG whole columns of each block
+// and needs no bounds checks: its loads are (uniform column base) + (one lane
+// byte offset).  The columns are walked in chunks of PF loads through a ring of
+// two register buffers: chunk c+1 is issued before chunk c is consumed, and the
+// first chunk of the NEXT block is issued right after the last FMA of the
+// current one, before the partial-sum write, the barriers and the serial
+// epilogue of the step.  512 threads per workgroup leave 256 VGPRs per thread
+// for the two buffers.
+constexpr int PF_THREADS = 512;
+
+template <typename T, int RPL, int CPG>
+__global__ __launch_bounds__(PF_THREADS) void k_solve_pf(const typename Cx<T>::t* __restrict__ G,
+                                                         typename Cx<T>::t* __restrict__ X,
+                                                         const double2* __restrict__ cmy,
+                                                         const double2* __restrict__ cpy, int n, int ny, int64_t n2,
+                                                         int64_t N, int nru, int ncg) {
   typedef typename Cx<T>::t C;
   typedef typename Unit<T>::t U;
+  constexpr int PF = CPG < 20 ? CPG : 20;
+  constexpr int NCH = (CPG + PF - 1) / PF;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C* v = reinterpret_cast<C*>(smem_raw);  // n
   C* part = v + n;                        // ncg * n
@@ -575,37 +585,31 @@ __global__ __launch_bounds__(1024) void k_solve_pf(const typename Cx<T>::t* __re
   const C* Gk = G + (size_t)k * ny * nn;
   const int tid = threadIdx.x;
   const int g = tid / nru, ru = tid - g * nru;
-  const int c0 = g * cpg;
-  const bool worker = (g < ncg) && (c0 < n);
-  const int ncol = worker ? min(n, c0 + cpg) - c0 : 0;
-  const int nchunk = (ncol + PF - 1) / PF;
+  const int c0 = g * CPG;
+  const bool worker = g < ncg;
+  // lane offset inside a column slice, in bytes (32-bit, unsigned): the loads
+  // are (uniform column base) + zext(offset), the scalar-base addressing form
+  const unsigned toff = (unsigned)((ru * RPL + n * c0) * (int)sizeof(C));
+  const size_t colbytes = (size_t)n * sizeof(C);
   const int nsteps = 2 * ny - 1;
-  const size_t ubase = (size_t)ru * RPL + (size_t)n * c0;
 
-  U buf[PF];
-  auto issue = [&](const C* B, int chunk) {
-#pragma unroll
-    for (int i = 0; i < PF; ++i) {
-      const int cc = min(chunk * PF + i, ncol - 1);
-      buf[i] = *reinterpret_cast<const U*>(B + ubase + (size_t)n * cc);
-    }
-  };
+  U buf[2][PF];
   T acc[2 * RPL];
-  auto consume = [&](int chunk) {
+  auto issue = [&](const C* B, int ch, U* b) {
+    const unsigned char* base = reinterpret_cast<const unsigned char*>(B) + (size_t)(ch * PF) * colbytes;
 #pragma unroll
-    for (int i = 0; i < PF; ++i) {
-      const int cc = chunk * PF + i;
-      C vc = v[c0 + min(cc, ncol - 1)];
-      const bool ok = cc < ncol;
-      vc.x = ok ? vc.x : (T)0;
-      vc.y = ok ? vc.y : (T)0;
-      unit_fma<T, RPL>(buf[i], vc, acc);
-    }
+    for (int i = 0; i < PF; ++i)
+      if (ch * PF + i < CPG) b[i] = *reinterpret_cast<const U*>(base + (size_t)i * colbytes + toff);
+  };
+  auto consume = [&](int ch, const U* b) {
+#pragma unroll
+    for (int i = 0; i < PF; ++i)
+      if (ch * PF + i < CPG) unit_fma<T, RPL>(b[i], v[c0 + ch * PF + i], acc);
   };
   auto blk_of = [&](int s) { return s <= ny - 1 ? s : 2 * ny - 2 - s; };
 
   for (int i = tid; i < n; i += blockDim.x) v[i] = x[i];
-  if (worker) issue(Gk, 0);
+  if (worker) issue(Gk, 0, buf[0]);
   __syncthreads();
 
   for (int s = 0; s < nsteps; ++s) {
@@ -619,12 +623,12 @@ __global__ __launch_bounds__(1024) void k_solve_pf(const typename Cx<T>::t* __re
 #pragma unroll
     for (int i = 0; i < 2 * RPL; ++i) acc[i] = 0;
     if (worker) {
-      consume(0);
-      for (int ch = 1; ch < nchunk; ++ch) {
-        issue(B, ch);
-        consume(ch);
+#pragma unroll
+      for (int ch = 0; ch < NCH; ++ch) {
+        if (ch + 1 < NCH) issue(B, ch + 1, buf[(ch + 1) & 1]);
+        consume(ch, buf[ch & 1]);
       }
-      if (s + 1 < nsteps) issue(Gk + (size_t)blk_of(s + 1) * nn, 0);
+      if (s + 1 < nsteps) issue(Gk + (size_t)blk_of(s + 1) * nn, 0, buf[0]);
 #pragma unroll
       for (int r = 0; r < RPL; ++r) {
         C o;
@@ -663,6 +667,242 @@ __global__ __launch_bounds__(1024) void k_solve_pf(const typename Cx<T>::t* __re
     }
     __syncthreads();
   }
+}
+
+// ------------------------------------------------------------ LDS-DMA solve
+// The same forward/backward sweep with the factor blocks streamed through LDS
+// by the gfx950 direct-to-LDS loads (global_load_lds_dwordx4).  A block is one
+// contiguous column-major array; it is cut into tiles of TCOL whole columns
+// (R rounds of 512 x 16 bytes) that flow through a ring of four LDS slots with
+// two tiles in flight at all times -- also across the reduction, barriers and
+// serial epilogue that end each step.  The ring slots are distinct LDS objects
+// and the ring is unrolled four ways, so each DMA and each read names its slot
+// statically; waits are explicit (s_waitcnt vmcnt(k R)) and barriers are raw
+// s_barrier, which keeps in-flight DMA out of every wait except the one that
+// needs it.  512 threads: thread (cg, ru) owns load unit ru (16 bytes = RPL
+// rows) of TCOL/NCG columns of each tile.
+namespace dma {
+#define HZ_LDS __attribute__((address_space(3)))
+constexpr int NT = 512;
+constexpr int NMAX = 256;
+
+__device__ __forceinline__ constexpr int wait_vm(int vm) {
+  // s_waitcnt: vmcnt = vm, expcnt and lgkmcnt at their maxima (no wait)
+  return (vm & 0xF) | (0x7 << 4) | (0xF << 8) | (((vm >> 4) & 3) << 14);
+}
+constexpr int WAIT_LGKM0 = 0xF | (0x7 << 4) | (0x0 << 8) | (0x3 << 14);  // lgkmcnt(0) only
+
+__device__ __forceinline__ void raw_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+}  // namespace dma
+
+template <typename T, int RPL, int NCG, int TCOL, int NX>
+__global__ __launch_bounds__(dma::NT) void k_solve_dma(const typename Cx<T>::t* __restrict__ G,
+                                                       typename Cx<T>::t* __restrict__ X,
+                                                       const double2* __restrict__ cmy,
+                                                       const double2* __restrict__ cpy, int ny, int64_t n2,
+                                                       int64_t N) {
+  typedef typename Cx<T>::t C;
+  typedef typename Unit<T>::t U;
+  constexpr int n = NX;
+  constexpr int nru = n / RPL;
+  constexpr int units = TCOL * nru;                   // 16-byte units per tile
+  constexpr int R = (units + dma::NT - 1) / dma::NT;  // DMA rounds per tile
+  constexpr int PU = n * (int)sizeof(C) / 16;         // 16-byte units of one y-line
+  static_assert(n % RPL == 0 && n % TCOL == 0 && TCOL % NCG == 0 && NCG * nru <= dma::NT, "tile split");
+  static_assert(PU <= dma::NT && (n * (int)sizeof(C)) % 16 == 0, "line DMA");
+  static_assert(2 * R <= 63, "vmcnt range");
+  __shared__ U s0[R * dma::NT], s1[R * dma::NT], s2[R * dma::NT], s3[R * dma::NT];
+  __shared__ U pbuf[dma::NT];   // epilogue operand of the current step (first PU units used)
+  const C* pline = reinterpret_cast<const C*>(pbuf);
+  __shared__ C v[n];
+  __shared__ C part[NCG * n];
+
+  const int k = blockIdx.x;
+  const int64_t rhs = blockIdx.y;
+  C* x = X + rhs * N + (int64_t)k * n2;
+  const size_t nn = (size_t)n * n;
+  const C* Gk = G + (size_t)k * ny * nn;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6;
+  const int cg = tid / nru, ru = tid - cg * nru;
+  const bool worker = cg < NCG;
+  constexpr int CPT = TCOL / NCG;                     // columns of a tile per worker
+  constexpr int tps = n / TCOL;                       // tiles per block
+  const int nsteps = 2 * ny - 1;
+  const int TT = nsteps * tps;                        // tiles of the whole solve
+  auto blk_of = [&](int s) { return s <= ny - 1 ? s : 2 * ny - 2 - s; };
+
+  // DMA of tile Tc into slot SLOT: a flat copy of TCOL*n complex (contiguous).
+  // Tiles past the end re-read tile 0 so that every wave always has exactly
+  // two tiles of DMA behind the one it waits for.  (Macros, so that every DMA
+  // and every read names its LDS slot object directly.)
+#define HZ_DMA_TILE(TC_, SLOT)                                                                          \
+  {                                                                                                     \
+    int tc_ = (TC_) >= TT ? 0 : (TC_);  /* past the end: re-read tile 0 */                             \
+    const int s_ = tc_ / tps, t_ = tc_ - s_ * tps;                                                      \
+    const unsigned char* base_ = reinterpret_cast<const unsigned char*>(Gk + (size_t)blk_of(s_) * nn) + \
+                                 (size_t)t_ * TCOL * n * sizeof(C);                                     \
+    _Pragma("unroll") for (int r_ = 0; r_ < R; ++r_) {                                                  \
+      const int pos_ = r_ * dma::NT + wv * 64 + lane;                                                   \
+      const int u_ = pos_ < units ? pos_ : units - 1;                                                   \
+      __builtin_amdgcn_global_load_lds((const void*)(base_ + (size_t)u_ * 16),                          \
+                                       (HZ_LDS void*)(&SLOT[r_ * dma::NT + wv * 64]), 16, 0, 0);        \
+    }                                                                                                   \
+  }
+  // DMA of one y-line of the plane vector (slot J_) into pbuf
+#define HZ_DMA_LINE(J_)                                                                                   \
+  {                                                                                                       \
+    const unsigned char* base_ = reinterpret_cast<const unsigned char*>(x + (size_t)n * (J_));            \
+    const int u_ = min(wv * 64 + lane, PU - 1);                                                           \
+    __builtin_amdgcn_global_load_lds((const void*)(base_ + (size_t)u_ * 16), (HZ_LDS void*)(&pbuf[wv * 64]), \
+                                     16, 0, 0);                                                           \
+  }
+  T acc[2 * RPL];
+#define HZ_PROCESS(TC_, SLOT, NEXT)                                                   \
+  {                                                                                   \
+    const int Tc = (TC_);                                                             \
+    const int s = Tc / tps, t = Tc - s * tps;                                         \
+    __builtin_amdgcn_s_waitcnt(dma::wait_vm(2 * R));                                  \
+    /* reads of the slot that the DMA below refills are done in every wave */       \
+    __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);                                      \
+    dma::raw_barrier();                                                               \
+    if (Tc < TT && t == 0) {                                                          \
+      _Pragma("unroll") for (int i = 0; i < 2 * RPL; ++i) acc[i] = 0;                 \
+      int slot_line = 0;                                                              \
+      if (s < ny - 1) slot_line = s + 1;                                              \
+      else if (2 * ny - 2 - s > 0) slot_line = 2 * ny - 3 - s;                        \
+      HZ_DMA_LINE(slot_line)                                                          \
+    }                                                                                 \
+    if (Tc < TT && worker) {                                                          \
+      _Pragma("unroll") for (int j = 0; j < CPT; ++j) {                               \
+        const int cl = cg * CPT + j;                                                  \
+        unit_fma<T, RPL>(SLOT[cl * nru + ru], v[t * TCOL + cl], acc);                  \
+      }                                                                               \
+    }                                                                                 \
+    HZ_DMA_TILE(Tc + 3, NEXT)                                                         \
+    if (Tc < TT && t == tps - 1) HZ_EPILOGUE(s)                                      \
+  }
+
+  // end of a step: cross-group reduction, t_{s+1} / x_j, next vector
+#define HZ_EPILOGUE(S_)                                                         \
+  {                                                                             \
+    const int se_ = (S_);                                                       \
+    if (worker) {                                                               \
+      _Pragma("unroll") for (int r = 0; r < RPL; ++r) {                         \
+        C o;                                                                    \
+        o.x = acc[2 * r];                                                       \
+        o.y = acc[2 * r + 1];                                                   \
+        part[cg * n + ru * RPL + r] = o;                                        \
+      }                                                                         \
+    }                                                                           \
+    /* the line DMA of this step is older than the last 3 tiles of DMA */     \
+    __builtin_amdgcn_s_waitcnt(dma::wait_vm(3 * R));                            \
+    __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);                                \
+    dma::raw_barrier();                                                         \
+    if (tid < n) {                                                              \
+      C y;                                                                      \
+      y.x = 0; y.y = 0;                                                         \
+      _Pragma("unroll") for (int q = 0; q < NCG; ++q) {                         \
+        const C u = part[q * n + tid];                                          \
+        y.x += u.x;                                                             \
+        y.y += u.y;                                                             \
+      }                                                                         \
+      const C pre = pline[tid];                                                 \
+      C nv;                                                                     \
+      if (se_ < ny - 1) {                                                       \
+        const double2 cm = cmy[se_ + 1];                                        \
+        nv.x = pre.x - (T)(cm.x * y.x - cm.y * y.y);                            \
+        nv.y = pre.y - (T)(cm.x * y.y + cm.y * y.x);                            \
+        x[(size_t)n * (se_ + 1) + tid] = nv;                                    \
+      } else {                                                                  \
+        const int j = 2 * ny - 2 - se_;                                         \
+        x[(size_t)n * j + tid] = y;                                             \
+        if (j > 0) {                                                            \
+          const double2 cp = cpy[j - 1];                                        \
+          nv.x = pre.x - (T)(cp.x * y.x - cp.y * y.y);                          \
+          nv.y = pre.y - (T)(cp.x * y.y + cp.y * y.x);                          \
+        } else {                                                                \
+          nv = y;                                                               \
+        }                                                                       \
+      }                                                                         \
+      v[tid] = nv;                                                              \
+    }                                                                           \
+    __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);                                \
+  }
+
+  for (int i = tid; i < n; i += dma::NT) v[i] = x[i];
+  __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
+  HZ_DMA_TILE(0, s0)
+  HZ_DMA_TILE(1, s1)
+  HZ_DMA_TILE(2, s2)
+  // the tile stream is padded to a multiple of four stages; stages past the
+  // end only keep the DMA accounting uniform
+  for (int T0 = 0; T0 < TT; T0 += 4) {
+    HZ_PROCESS(T0, s0, s3)
+    HZ_PROCESS(T0 + 1, s1, s0)
+    HZ_PROCESS(T0 + 2, s2, s1)
+    HZ_PROCESS(T0 + 3, s3, s2)
+  }
+#undef HZ_EPILOGUE
+#undef HZ_PROCESS
+#undef HZ_DMA_LINE
+#undef HZ_DMA_TILE
+  // drain the trailing DMA before the workgroup retires
+  __builtin_amdgcn_s_waitcnt(dma::wait_vm(0));
+}
+
+template <typename T, int RPL, int NCG, int TCOL, int NX>
+bool try_launch_dma(int n, int ny, dim3 grid, hipStream_t s, const void* G, void* x, const double2* cmy,
+                    const double2* cpy, int64_t n2, int64_t N) {
+  if (n != NX) return false;
+  typedef typename Cx<T>::t C;
+  hipLaunchKernelGGL((k_solve_dma<T, RPL, NCG, TCOL, NX>), grid, dim3(dma::NT), 0, s, (const C*)G, (C*)x, cmy, cpy,
+                     ny, n2, N);
+  HZ_HIP(hipGetLastError());
+  return true;
+}
+
+// Column groups of the prefetching kernel: the largest divisor of n whose
+// groups fit 1024 threads; returns the columns per group (0: no divisor > 0).
+int pf_columns(int n, int nru, int* ncg_out) {
+  const int maxg = PF_THREADS / nru;
+  for (int g = std::min(maxg, n); g >= 1; --g)
+    if (n % g == 0) {
+      *ncg_out = g;
+      return n / g;
+    }
+  return 0;
+}
+
+template <typename T, int RPL, int CPG>
+bool try_launch_pf(int cpg, int ncg, int nru, dim3 grid, hipStream_t s, const void* G, void* x, const double2* cmy,
+                   const double2* cpy, int n, int ny, int64_t n2, int64_t N) {
+  if (cpg != CPG) return false;
+  typedef typename Cx<T>::t C;
+  const size_t lds = sizeof(C) * (size_t)n * (1 + ncg);
+  HZ_HIP(hipFuncSetAttribute((const void*)&k_solve_pf<T, RPL, CPG>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)lds));
+  hipLaunchKernelGGL((k_solve_pf<T, RPL, CPG>), grid, dim3(PF_THREADS), lds, s, (const C*)G, (C*)x, cmy, cpy, n, ny,
+                     n2, N, nru, ncg);
+  HZ_HIP(hipGetLastError());
+  return true;
+}
+
+// Plane-solve kernel family: HZ_SOLVE_KERNEL=dma|pf|fast picks one for
+// timing; the default is the LDS-DMA kernel where it is instantiated and the
+// chunked kernel elsewhere.
+enum SolveKernel { SK_DEFAULT, SK_DMA, SK_PF, SK_FAST };
+SolveKernel solve_kernel_choice() {
+  const char* v = getenv("HZ_SOLVE_KERNEL");
+  if (!v || !v[0]) return SK_DEFAULT;
+  if (!strcmp(v, "dma")) return SK_DMA;
+  if (!strcmp(v, "pf")) return SK_PF;
+  if (!strcmp(v, "fast")) return SK_FAST;
+  return SK_DEFAULT;
 }
 
 PlaneTabs plane_tabs(hz_problem* p) {
@@ -731,13 +971,52 @@ void launch_plane_solve(hz_problem* p, int prec, int nrhs, void* x, bool adjoint
   const double2* cmy = p->tab(1, 0);
   const double2* cpy = p->tab(1, 1);
   dim3 grid((unsigned)p->nz, (unsigned)nrhs);
-  if (!adjoint && ny >= 1) {
-    constexpr int PF = 20;
+  const SolveKernel sk = solve_kernel_choice();
+  if (!adjoint && ny >= 1 && (sk == SK_DEFAULT || sk == SK_DMA)) {
+    // LDS-DMA streaming kernel for the configurations it is instantiated for
+    const void* Gp = f.G.ptr;
+    if (prec == HZ_PREC_C64 &&
+        (try_launch_dma<float, 2, 5, 20, 200>(n, ny, grid, s, Gp, x, cmy, cpy, p->n2, p->N)))
+      return;
+    if (prec == HZ_PREC_FP64 &&
+        (try_launch_dma<double, 1, 2, 10, 200>(n, ny, grid, s, Gp, x, cmy, cpy, p->n2, p->N)))
+      return;
+  }
+  if (!adjoint && ny >= 1 && sk == SK_PF) {
+    // prefetching kernel for the column splits it is instantiated for
+    if (prec == HZ_PREC_C64 && n % 2 == 0) {
+      const int nru = n / 2;
+      int ncg = 0;
+      const int cpg = pf_columns(n, nru, &ncg);
+      const void* G = f.G.ptr;
+      if (try_launch_pf<float, 2, 40>(cpg, ncg, nru, grid, s, G, x, cmy, cpy, n, ny, p->n2, p->N) ||
+          try_launch_pf<float, 2, 48>(cpg, ncg, nru, grid, s, G, x, cmy, cpy, n, ny, p->n2, p->N) ||
+          try_launch_pf<float, 2, 32>(cpg, ncg, nru, grid, s, G, x, cmy, cpy, n, ny, p->n2, p->N) ||
+          try_launch_pf<float, 2, 16>(cpg, ncg, nru, grid, s, G, x, cmy, cpy, n, ny, p->n2, p->N) ||
+          try_launch_pf<float, 2, 12>(cpg, ncg, nru, grid, s, G, x, cmy, cpy, n, ny, p->n2, p->N) ||
+          try_launch_pf<float, 2, 4>(cpg, ncg, nru, grid, s, G, x, cmy, cpy, n, ny, p->n2, p->N))
+        return;
+    }
+    if (prec == HZ_PREC_FP64) {
+      const int nru = n;
+      int ncg = 0;
+      const int cpg = pf_columns(n, nru, &ncg);
+      const void* G = f.G.ptr;
+      if (try_launch_pf<double, 1, 100>(cpg, ncg, nru, grid, s, G, x, cmy, cpy, n, ny, p->n2, p->N) ||
+          try_launch_pf<double, 1, 96>(cpg, ncg, nru, grid, s, G, x, cmy, cpy, n, ny, p->n2, p->N) ||
+          try_launch_pf<double, 1, 80>(cpg, ncg, nru, grid, s, G, x, cmy, cpy, n, ny, p->n2, p->N) ||
+          try_launch_pf<double, 1, 32>(cpg, ncg, nru, grid, s, G, x, cmy, cpy, n, ny, p->n2, p->N) ||
+          try_launch_pf<double, 1, 24>(cpg, ncg, nru, grid, s, G, x, cmy, cpy, n, ny, p->n2, p->N) ||
+          try_launch_pf<double, 1, 8>(cpg, ncg, nru, grid, s, G, x, cmy, cpy, n, ny, p->n2, p->N))
+        return;
+    }
+    // generic chunked kernel for every other split
+    constexpr int CH = 8;
     if (prec == HZ_PREC_C64 && n % 2 == 0) {
       const int nru = n / 2, ncg = std::max(1, 1024 / nru), cpg = (n + ncg - 1) / ncg;
       const size_t lds = sizeof(float2) * (size_t)n * (1 + ncg);
-      HZ_HIP(hipFuncSetAttribute((const void*)&k_solve_pf<float, 2, PF>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      hipLaunchKernelGGL((k_solve_pf<float, 2, PF>), grid, dim3(1024), lds, s, f.G.as<float2>(), (float2*)x, cmy, cpy,
+      HZ_HIP(hipFuncSetAttribute((const void*)&k_solve_fast<float, 2, CH>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      hipLaunchKernelGGL((k_solve_fast<float, 2, CH>), grid, dim3(1024), lds, s, f.G.as<float2>(), (float2*)x, cmy, cpy,
                          n, ny, p->n2, p->N, nru, ncg, cpg);
       HZ_HIP(hipGetLastError());
       return;
@@ -745,8 +1024,8 @@ void launch_plane_solve(hz_problem* p, int prec, int nrhs, void* x, bool adjoint
     if (prec == HZ_PREC_FP64) {
       const int nru = n, ncg = std::max(1, 1024 / nru), cpg = (n + ncg - 1) / ncg;
       const size_t lds = sizeof(double2) * (size_t)n * (1 + ncg);
-      HZ_HIP(hipFuncSetAttribute((const void*)&k_solve_pf<double, 1, PF>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      hipLaunchKernelGGL((k_solve_pf<double, 1, PF>), grid, dim3(1024), lds, s, f.G.as<double2>(), (double2*)x, cmy,
+      HZ_HIP(hipFuncSetAttribute((const void*)&k_solve_fast<double, 1, CH>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      hipLaunchKernelGGL((k_solve_fast<double, 1, CH>), grid, dim3(1024), lds, s, f.G.as<double2>(), (double2*)x, cmy,
                          cpy, n, ny, p->n2, p->N, nru, ncg, cpg);
       HZ_HIP(hipGetLastError());
       return;

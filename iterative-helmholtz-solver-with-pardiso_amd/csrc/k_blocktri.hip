@@ -24,6 +24,7 @@
 // One workgroup per (plane, right-hand side); each step is a GEMV that streams
 // the nx x nx block G_j once from HBM.
 #include <cstdlib>
+#include <cstdint>
 #include <cstring>
 
 #include "hz_internal.h"
@@ -866,6 +867,240 @@ bool try_launch_dma(int n, int ny, dim3 grid, hipStream_t s, const void* G, void
   return true;
 }
 
+// ------------------------------------------------ multi-RHS LDS-DMA solve
+// NR right-hand sides of one plane share every factor tile: the tile is
+// streamed once through the LDS ring and each thread applies its 16-byte unit
+// (RPL rows of one column) to all NR vectors.  Thread tid = ru * NCG + cg owns
+// row unit ru and column group cg; the NCG partial sums of a row unit sit in
+// adjacent lanes and are combined with xor shuffles, so no LDS reduction
+// buffer is needed.  The tile DMA scatters units so that the LDS slot is read
+// as SLOT[j * NW + tid] (contiguous per wave): LDS position p = j * NW + tid
+// holds unit (cg * CPT + j) * nru + ru of the column-major tile.  The vectors
+// sit in LDS as v[col][q] (right-hand sides fastest).  In the epilogue lane cg
+// finishes right-hand sides cg, cg + NCG, ...; the operand it needs (one
+// 16-byte unit of the next y-line per right-hand side) was fetched by the
+// lane's own DMA at the start of the step, so only the wave's own wait count
+// guards it.
+template <typename T, int RPL, int NCG, int TCOL, int NX, int NR>
+struct MrShape {
+  typedef typename Cx<T>::t C;
+  static constexpr int n = NX;
+  static constexpr int nru = n / RPL;
+  static constexpr int NW = NCG * nru;                  // worker threads
+  static constexpr int CPT = TCOL / NCG;                // columns of a tile per thread
+  static constexpr int tps = n / TCOL;                  // tiles per block
+  static constexpr int units = TCOL * nru;              // 16-byte units per tile
+  static constexpr int R = (units + dma::NT - 1) / dma::NT;
+  static constexpr int KQ = (NR + NCG - 1) / NCG;       // right-hand sides per lane in the epilogue
+  static constexpr int WE = (tps < 3 ? tps : 3) * R;    // epilogue wait: tiles issued after the line DMA
+  static constexpr size_t lds = 4 * (size_t)R * dma::NT * 16 + (size_t)NR * n * sizeof(C) +
+                                (size_t)(dma::NT / 64) * KQ * 64 * 16;
+  static_assert(n % RPL == 0 && n % TCOL == 0 && TCOL % NCG == 0, "tile split");
+  static_assert(NW <= dma::NT && (NCG & (NCG - 1)) == 0 && NCG <= 64, "thread split");
+  static_assert(2 * R <= 63 && WE <= 63, "vmcnt range");
+  static_assert(lds <= 160 * 1024, "LDS budget");
+};
+
+template <typename T, int RPL, int NCG, int TCOL, int NX, int NR>
+__global__ __launch_bounds__(dma::NT) void k_solve_mr(const typename Cx<T>::t* __restrict__ G,
+                                                      typename Cx<T>::t* __restrict__ X,
+                                                      const double2* __restrict__ cmy,
+                                                      const double2* __restrict__ cpy, int ny, int64_t n2, int64_t N) {
+  typedef MrShape<T, RPL, NCG, TCOL, NX, NR> S;
+  typedef typename Cx<T>::t C;
+  typedef typename Unit<T>::t U;
+  constexpr int n = S::n, nru = S::nru, NW = S::NW, CPT = S::CPT, tps = S::tps, units = S::units, R = S::R;
+  constexpr int KQ = S::KQ;
+  __shared__ U s0[R * dma::NT], s1[R * dma::NT], s2[R * dma::NT], s3[R * dma::NT];
+  __shared__ U pbuf[(dma::NT / 64) * KQ * 64];
+  __shared__ C v[n * NR];
+
+  const int k = blockIdx.x;
+  const int64_t rhs0 = (int64_t)blockIdx.y * NR;
+  C* xk = X + rhs0 * N + (int64_t)k * n2;   // right-hand side q of this plane: xk + q * N
+  const size_t nn = (size_t)n * n;
+  const C* Gk = G + (size_t)k * ny * nn;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6;
+  const int cg = tid & (NCG - 1), ru = tid / NCG;
+  const bool worker = tid < NW;
+  const int nsteps = 2 * ny - 1;
+  const int TT = nsteps * tps;
+  auto blk_of = [&](int s) { return s <= ny - 1 ? s : 2 * ny - 2 - s; };
+
+  // tile-relative source unit of each DMA round of this lane (same for every tile)
+  int uoff[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int p = r * dma::NT + tid;
+    int u = units - 1;
+    if (p < units) {
+      const int j = p / NW, rem = p - j * NW;
+      const int rr = rem / NCG, cc = rem - rr * NCG;
+      u = (cc * CPT + j) * nru + rr;
+    }
+    uoff[r] = u;
+  }
+  // the epilogue right-hand sides of this lane and its 16-byte unit of a line
+  const int ru_c = worker ? ru : 0;
+
+#define HZ_MR_TILE(TC_, SLOT)                                                                           \
+  {                                                                                                     \
+    const int tc_ = (TC_) >= TT ? 0 : (TC_); /* past the end: re-read tile 0 */                        \
+    const int s_ = tc_ / tps, t_ = tc_ - s_ * tps;                                                      \
+    const unsigned char* base_ = reinterpret_cast<const unsigned char*>(Gk + (size_t)blk_of(s_) * nn) + \
+                                 (size_t)t_ * TCOL * n * sizeof(C);                                     \
+    _Pragma("unroll") for (int r_ = 0; r_ < R; ++r_) {                                                  \
+      __builtin_amdgcn_global_load_lds((const void*)(base_ + (size_t)uoff[r_] * 16),                    \
+                                       (HZ_LDS void*)(&SLOT[r_ * dma::NT + wv * 64]), 16, 0, 0);        \
+    }                                                                                                   \
+  }
+#define HZ_MR_LINE(J_)                                                                                  \
+  {                                                                                                     \
+    _Pragma("unroll") for (int kq_ = 0; kq_ < KQ; ++kq_) {                                              \
+      const int q_ = cg + NCG * kq_;                                                                    \
+      const C* src_ = xk + (int64_t)(q_ < NR ? q_ : 0) * N + (size_t)n * (J_) + (size_t)ru_c * RPL;     \
+      __builtin_amdgcn_global_load_lds((const void*)src_, (HZ_LDS void*)(&pbuf[(wv * KQ + kq_) * 64]),  \
+                                       16, 0, 0);                                                       \
+    }                                                                                                   \
+  }
+  T acc[NR * 2 * RPL];
+#define HZ_MR_EPILOGUE(S_)                                                                      \
+  {                                                                                             \
+    const int se_ = (S_);                                                                       \
+    _Pragma("unroll") for (int off_ = 1; off_ < NCG; off_ <<= 1) {                              \
+      _Pragma("unroll") for (int i_ = 0; i_ < NR * 2 * RPL; ++i_) acc[i_] += __shfl_xor(acc[i_], off_); \
+    }                                                                                           \
+    __builtin_amdgcn_s_waitcnt(dma::wait_vm(S::WE));                                            \
+    __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);                                                \
+    dma::raw_barrier(); /* every wave is done reading v of this step */                        \
+    if (worker) {                                                                               \
+      _Pragma("unroll") for (int kq_ = 0; kq_ < KQ; ++kq_) {                                    \
+        const int q_ = cg + NCG * kq_;                                                          \
+        if (q_ < NR) {                                                                          \
+          const U pu_ = pbuf[(wv * KQ + kq_) * 64 + lane];                                      \
+          const C* pre_ = reinterpret_cast<const C*>(&pu_);                                     \
+          C* xq_ = xk + (int64_t)q_ * N;                                                        \
+          U out_;                                                                               \
+          C* o_ = reinterpret_cast<C*>(&out_);                                                  \
+          _Pragma("unroll") for (int r_ = 0; r_ < RPL; ++r_) {                                  \
+            C y_;                                                                               \
+            y_.x = acc[(q_ * RPL + r_) * 2];                                                    \
+            y_.y = acc[(q_ * RPL + r_) * 2 + 1];                                                \
+            C nv_;                                                                              \
+            if (se_ < ny - 1) {                                                                 \
+              const double2 cm = cmy[se_ + 1];                                                  \
+              nv_.x = pre_[r_].x - (T)(cm.x * y_.x - cm.y * y_.y);                              \
+              nv_.y = pre_[r_].y - (T)(cm.x * y_.y + cm.y * y_.x);                              \
+              o_[r_] = nv_;                                                                     \
+            } else {                                                                            \
+              const int j = 2 * ny - 2 - se_;                                                   \
+              o_[r_] = y_;                                                                      \
+              if (j > 0) {                                                                      \
+                const double2 cp = cpy[j - 1];                                                  \
+                nv_.x = pre_[r_].x - (T)(cp.x * y_.x - cp.y * y_.y);                            \
+                nv_.y = pre_[r_].y - (T)(cp.x * y_.y + cp.y * y_.x);                            \
+              } else {                                                                          \
+                nv_ = y_;                                                                       \
+              }                                                                                 \
+            }                                                                                   \
+            v[(ru * RPL + r_) * NR + q_] = nv_;                                                 \
+          }                                                                                     \
+          const int line_ = se_ < ny - 1 ? se_ + 1 : 2 * ny - 2 - se_;                          \
+          *reinterpret_cast<U*>(xq_ + (size_t)n * line_ + (size_t)ru * RPL) = out_;             \
+        }                                                                                       \
+      }                                                                                         \
+    }                                                                                           \
+    __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);                                                \
+  }
+#define HZ_MR_STAGE(TC_, SLOT, NEXT)                                                    \
+  {                                                                                     \
+    const int Tc = (TC_);                                                               \
+    const int s = Tc / tps, t = Tc - s * tps;                                           \
+    __builtin_amdgcn_s_waitcnt(dma::wait_vm(2 * R));                                    \
+    __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);                                        \
+    dma::raw_barrier();                                                                 \
+    if (Tc < TT && t == 0) {                                                            \
+      _Pragma("unroll") for (int i = 0; i < NR * 2 * RPL; ++i) acc[i] = 0;              \
+      int slot_line = 0;                                                                \
+      if (s < ny - 1) slot_line = s + 1;                                                \
+      else if (2 * ny - 2 - s > 0) slot_line = 2 * ny - 3 - s;                          \
+      HZ_MR_LINE(slot_line)                                                             \
+    }                                                                                   \
+    if (Tc < TT && worker) {                                                            \
+      _Pragma("unroll") for (int j = 0; j < CPT; ++j) {                                 \
+        const U a = SLOT[j * NW + tid];                                                 \
+        const C* vc = &v[(t * TCOL + cg * CPT + j) * NR];                               \
+        _Pragma("unroll") for (int q = 0; q < NR; ++q)                                  \
+          unit_fma<T, RPL>(a, vc[q], &acc[q * 2 * RPL]);                                \
+      }                                                                                 \
+    }                                                                                   \
+    HZ_MR_TILE(Tc + 3, NEXT)                                                            \
+    if (Tc < TT && t == tps - 1) HZ_MR_EPILOGUE(s)                                      \
+  }
+
+  for (int i = tid; i < n * NR; i += dma::NT) {
+    const int q = i % NR, row = i / NR;
+    v[i] = xk[(int64_t)q * N + row];
+  }
+  __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
+  HZ_MR_TILE(0, s0)
+  HZ_MR_TILE(1, s1)
+  HZ_MR_TILE(2, s2)
+  for (int T0 = 0; T0 < TT; T0 += 4) {
+    HZ_MR_STAGE(T0, s0, s3)
+    HZ_MR_STAGE(T0 + 1, s1, s0)
+    HZ_MR_STAGE(T0 + 2, s2, s1)
+    HZ_MR_STAGE(T0 + 3, s3, s2)
+  }
+#undef HZ_MR_STAGE
+#undef HZ_MR_EPILOGUE
+#undef HZ_MR_LINE
+#undef HZ_MR_TILE
+  __builtin_amdgcn_s_waitcnt(dma::wait_vm(0));
+}
+
+// Launch of one right-hand-side group size NR over nrhs_g * NR vectors.
+template <typename T, int RPL, int NCG, int TCOL, int NX, int NR>
+void launch_mr(int ny, int nz, int groups, hipStream_t s, const void* G, void* x, const double2* cmy,
+               const double2* cpy, int64_t n2, int64_t N) {
+  typedef typename Cx<T>::t C;
+  hipLaunchKernelGGL((k_solve_mr<T, RPL, NCG, TCOL, NX, NR>), dim3((unsigned)nz, (unsigned)groups), dim3(dma::NT), 0,
+                     s, (const C*)G, (C*)x, cmy, cpy, ny, n2, N);
+  HZ_HIP(hipGetLastError());
+}
+
+// All nrhs vectors: as many full groups of NRMAX right-hand sides as fit, then
+// the remainder in groups of 4, 2 and 1 (each smaller than NRMAX); returns
+// false when NX has no instantiation.
+template <typename T, int RPL, int NCG, int TCOL, int NX, int NRMAX>
+bool try_launch_mr(int n, int ny, int nz, int nrhs, hipStream_t s, const void* G, void* x, const double2* cmy,
+                   const double2* cpy, int64_t n2, int64_t N) {
+  if (n != NX) return false;
+  typedef typename Cx<T>::t C;
+  C* xp = (C*)x;
+  int done = 0;
+  const int full = nrhs / NRMAX;
+  if (full > 0) {
+    launch_mr<T, RPL, NCG, TCOL, NX, NRMAX>(ny, nz, full, s, G, xp, cmy, cpy, n2, N);
+    done = full * NRMAX;
+  }
+  if (NRMAX > 4 && nrhs - done >= 4) {
+    launch_mr<T, RPL, NCG, TCOL, NX, (NRMAX > 4 ? 4 : 1)>(ny, nz, 1, s, G, xp + done * N, cmy, cpy, n2, N);
+    done += 4;
+  }
+  if (NRMAX > 2 && nrhs - done >= 2) {
+    launch_mr<T, RPL, NCG, TCOL, NX, (NRMAX > 2 ? 2 : 1)>(ny, nz, 1, s, G, xp + done * N, cmy, cpy, n2, N);
+    done += 2;
+  }
+  if (NRMAX > 1 && nrhs - done >= 1) {
+    launch_mr<T, RPL, NCG, TCOL, NX, 1>(ny, nz, 1, s, G, xp + done * N, cmy, cpy, n2, N);
+    done += 1;
+  }
+  HZ_CHECK(done == nrhs, HZ_ERR_STATE, "plane solve covered %d of %d right-hand sides", done, nrhs);
+  return true;
+}
+
 // Column groups of the prefetching kernel: the largest divisor of n whose
 // groups fit 1024 threads; returns the columns per group (0: no divisor > 0).
 int pf_columns(int n, int nru, int* ncg_out) {
@@ -892,17 +1127,37 @@ bool try_launch_pf(int cpg, int ncg, int nru, dim3 grid, hipStream_t s, const vo
   return true;
 }
 
-// Plane-solve kernel family: HZ_SOLVE_KERNEL=dma|pf|fast picks one for
-// timing; the default is the LDS-DMA kernel where it is instantiated and the
-// chunked kernel elsewhere.
-enum SolveKernel { SK_DEFAULT, SK_DMA, SK_PF, SK_FAST };
+// Plane-solve kernel family: HZ_SOLVE_KERNEL=mr|dma|pf|fast picks one for
+// timing; the default is the multi-RHS LDS-DMA kernel where it is
+// instantiated and the chunked kernel elsewhere.
+enum SolveKernel { SK_DEFAULT, SK_MR, SK_DMA, SK_PF, SK_FAST };
 SolveKernel solve_kernel_choice() {
   const char* v = getenv("HZ_SOLVE_KERNEL");
   if (!v || !v[0]) return SK_DEFAULT;
+  if (!strcmp(v, "mr")) return SK_MR;
   if (!strcmp(v, "dma")) return SK_DMA;
   if (!strcmp(v, "pf")) return SK_PF;
   if (!strcmp(v, "fast")) return SK_FAST;
   return SK_DEFAULT;
+}
+
+// The multi-RHS kernel for the grid widths it is instantiated for.
+bool launch_mr_any(int prec, int n, int ny, int nz, int nrhs, hipStream_t s, const void* G, void* x,
+                   const double2* cmy, const double2* cpy, int64_t n2, int64_t N) {
+  if (reinterpret_cast<uintptr_t>(x) % 16 != 0) return false;
+  if (prec == HZ_PREC_C64)
+    return try_launch_mr<float, 2, 4, 20, 200, 8>(n, ny, nz, nrhs, s, G, x, cmy, cpy, n2, N) ||
+           try_launch_mr<float, 2, 4, 16, 192, 8>(n, ny, nz, nrhs, s, G, x, cmy, cpy, n2, N) ||
+           try_launch_mr<float, 2, 4, 20, 160, 8>(n, ny, nz, nrhs, s, G, x, cmy, cpy, n2, N) ||
+           try_launch_mr<float, 2, 8, 32, 128, 8>(n, ny, nz, nrhs, s, G, x, cmy, cpy, n2, N) ||
+           try_launch_mr<float, 2, 8, 32, 96, 8>(n, ny, nz, nrhs, s, G, x, cmy, cpy, n2, N) ||
+           try_launch_mr<float, 2, 16, 32, 64, 8>(n, ny, nz, nrhs, s, G, x, cmy, cpy, n2, N);
+  return try_launch_mr<double, 1, 2, 10, 200, 4>(n, ny, nz, nrhs, s, G, x, cmy, cpy, n2, N) ||
+         try_launch_mr<double, 1, 2, 8, 192, 4>(n, ny, nz, nrhs, s, G, x, cmy, cpy, n2, N) ||
+         try_launch_mr<double, 1, 2, 10, 160, 4>(n, ny, nz, nrhs, s, G, x, cmy, cpy, n2, N) ||
+         try_launch_mr<double, 1, 4, 16, 128, 4>(n, ny, nz, nrhs, s, G, x, cmy, cpy, n2, N) ||
+         try_launch_mr<double, 1, 4, 16, 96, 4>(n, ny, nz, nrhs, s, G, x, cmy, cpy, n2, N) ||
+         try_launch_mr<double, 1, 8, 32, 64, 4>(n, ny, nz, nrhs, s, G, x, cmy, cpy, n2, N);
 }
 
 PlaneTabs plane_tabs(hz_problem* p) {
@@ -972,7 +1227,10 @@ void launch_plane_solve(hz_problem* p, int prec, int nrhs, void* x, bool adjoint
   const double2* cpy = p->tab(1, 1);
   dim3 grid((unsigned)p->nz, (unsigned)nrhs);
   const SolveKernel sk = solve_kernel_choice();
-  if (!adjoint && ny >= 1 && (sk == SK_DEFAULT || sk == SK_DMA)) {
+  if (!adjoint && ny >= 1 && (sk == SK_DEFAULT || sk == SK_MR) &&
+      launch_mr_any(prec, n, ny, (int)p->nz, nrhs, s, f.G.ptr, x, cmy, cpy, p->n2, p->N))
+    return;
+  if (!adjoint && ny >= 1 && sk == SK_DMA) {
     // LDS-DMA streaming kernel for the configurations it is instantiated for
     const void* Gp = f.G.ptr;
     if (prec == HZ_PREC_C64 &&

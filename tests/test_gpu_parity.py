@@ -169,6 +169,41 @@ def test_plane_pivoting_indefinite(hz, oracle):
     assert _rel(_host(S.precond_apply(_dev(x))), P.precond_apply(x)) < 1e-9
 
 
+@pytest.mark.parametrize("nx", [64, 96, 128, 160, 192, 200])
+def test_precond_wide_planes_multi_rhs_vs_oracle(hz, oracle, nx):
+    """Plane widths of the streaming multi-RHS solve: FP64 (1e-10) and complex64
+    (1e-4) preconditioners vs the oracle for right-hand-side counts that use every
+    group size (8, 4, 2, 1)."""
+    print("case wide planes nx=%d" % nx, flush=True)
+    pr = hz.problem
+    d = pr.cube_geometry((nx, 10, 6), (8, 2, 2), pr.omega_for_ppw(0.8, 1.0 / nx, 8), beta=0.5, pml_c=30.0)
+    c = 0.8 + 0.4 * np.random.default_rng(nx).random(d.N)
+    S, P = hz.Solver(d, c), oracle.Problem(d, c)
+    for prec, nrhs, tol in ((hz.abi.PREC_FP64, 7, 1e-10), (hz.abi.PREC_C64, 15, 1e-4)):
+        S.factor(prec)
+        x = rand_vec(d.N, 40 + nrhs, nrhs)
+        y = _host(S.precond_apply(_dev(x), prec))
+        for q in range(nrhs):
+            e = _rel(y[q], P.precond_apply(x[q]))
+            assert e < tol, (prec, q, e)
+
+
+def test_plane_solve_groups_match_single_rhs(hz):
+    """A batch of 11 right-hand sides (groups of 8, 2, 1) gives each vector's
+    single-RHS solve."""
+    pr = hz.problem
+    d = pr.cube_geometry((200, 12, 5), (8, 2, 2), 40.0, beta=0.5, pml_c=30.0)
+    S = hz.Solver(d, 0.8 + 0.4 * np.random.default_rng(2).random(d.N))
+    for prec, dt, tol in ((hz.abi.PREC_C64, torch.complex64, 1e-6), (hz.abi.PREC_FP64, torch.complex128, 1e-13)):
+        S.factor(prec)
+        x = torch.from_numpy(rand_vec(d.N, 9, 11)).to(dt).cuda()
+        batch = S.plane_solve(x.clone(), prec)
+        for q in range(11):
+            one = S.plane_solve(x[q].clone(), prec)
+            e = (torch.linalg.vector_norm(one - batch[q]) / torch.linalg.vector_norm(one)).item()
+            assert e < tol, (prec, q, e)
+
+
 @pytest.mark.parametrize("n,npml", [(128, 16), (200, 20)])
 def test_precond_inverse_identity_full_size(hz, n, npml):
     """L0 (L0^{-1} x) = x at the configs' full sizes (FP64 1e-10, complex64 1e-4),

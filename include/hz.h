@@ -106,6 +106,10 @@ typedef struct {
   double* hist;         /* optional host array of per-iteration residuals (may be NULL) */
   int32_t hist_cap;
   int32_t prec;         /* HZ_PREC_* used by the driver's preconditioner applies */
+  int32_t batch;        /* FGMRES: right-hand sides advanced in lockstep, sharing each batched
+                           preconditioner and operator apply (0: all of them, at most 16) */
+  int32_t graph;        /* FGMRES: 1 = replay each Arnoldi step's preconditioner + operator
+                           launches from a captured HIP graph */
 } hz_krylov_opts;
 
 typedef struct {
@@ -175,7 +179,8 @@ int hz_residual(hz_problem* p, int nrhs, const hz_c128* f, const hz_c128* u, hz_
 /* ------------------------------------------------------ Krylov drivers */
 /* Solve A u = f for nrhs right-hand sides (stats: host array of nrhs).
  *  HZ_DRIVER_FGMRES:    flexible GMRES, MGS, complex Givens, FP64 true-operator
- *                       matvec and residual, preconditioner in opts->prec;
+ *                       matvec and residual, preconditioner in opts->prec; the
+ *                       right-hand sides run in lockstep batches of opts->batch;
  *  HZ_DRIVER_GMRES_REF: the reference's restarted GMRES on (I - deltaL L0^{-1}),
  *                       zgels least squares, per-iteration true residual over
  *                       the mask, u = L0^{-1} x (FGMRES.cpp:6-725).

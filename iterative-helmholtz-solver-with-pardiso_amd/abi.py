@@ -46,6 +46,8 @@ class KrylovOpts(C.Structure):
         ("hist", C.POINTER(C.c_double)),
         ("hist_cap", C.c_int32),
         ("prec", C.c_int32),
+        ("batch", C.c_int32),
+        ("graph", C.c_int32),
     ]
 
 

@@ -175,15 +175,23 @@ static void lsq(int m, int n, std::vector<zc>& H, int ld, std::vector<zc>& b) {
   }
 }
 
-static void ensure_krylov(hz_problem* p, int m) {
-  const size_t vb = (size_t)p->N * sizeof(double2);
+static void ensure_krylov(hz_problem* p, int m, int R = 1) {
+  const size_t vb = (size_t)p->N * sizeof(double2) * R;
+  const bool fresh = p->kV.bytes < vb * (m + 1) || p->kZ.bytes < vb * m;
   grow(p->kV, vb * (m + 1));
   grow(p->kZ, vb * m);
   grow(p->kr, vb);
   grow(p->kw, vb);
   grow(p->kx, vb);
   grow(p->kaux, vb);
-  grow(p->scal, std::max<size_t>(4096, (size_t)(m + 2) * (m + 2) + 4 * (m + 2) + 256) * sizeof(double2));
+  const size_t sb = (size_t)(m + 2) * (m + 2) + 4 * (m + 2) + 64;
+  grow(p->scal, std::max<size_t>(4096, 64 + sb * R + 4 * R + 256) * sizeof(double2));
+  if (fresh) {
+    // basis slots a batch member never reaches still enter the batched
+    // preconditioner: keep them finite
+    launch_fill((int64_t)(p->kV.bytes / sizeof(double2)), make_double2(0, 0), p->kV.as<double2>(), p->stream);
+    launch_fill((int64_t)(p->kZ.bytes / sizeof(double2)), make_double2(0, 0), p->kZ.as<double2>(), p->stream);
+  }
 }
 
 struct HistW {
@@ -196,71 +204,164 @@ struct HistW {
   }
 };
 
-// Flexible GMRES (MGS, complex Givens) on A u = f, right preconditioner
-// L0^{-1} in `prec`, FP64 Krylov vectors; x (device) holds the solution.
-static void fgmres_one(hz_problem* p, const double2* f, double2* x, const hz_krylov_opts& o, hz_krylov_stats& st,
-                       bool adjoint) {
+// Flexible GMRES (modified Gram-Schmidt, complex Givens) on A u = f with the
+// right preconditioner L0^{-1} in `prec` and FP64 Krylov vectors, for R
+// right-hand sides advanced in lockstep: each Arnoldi step applies the
+// preconditioner and the operator to the R current basis vectors in one
+// batched call (the plane solve streams each factor block once for the batch),
+// then orthogonalises every member against its own basis.  Members stop on
+// their own convergence or breakdown; the others go on.  Basis layout: V_j of
+// member q at V + (j R + q) N (Z likewise), so step j's batch is contiguous.
+// With R = 1 this is the plain FGMRES of FGMRES.cpp:458-661 with a flexible
+// basis Z and the MGS coefficient h_ij = V_i^H w.
+namespace {
+struct Member {
+  std::vector<zc> H, cs, sn, g, y, Lg;
+  std::vector<double2> hcol, yd, rl;
+  double norm_f = 0, rel = 0, res = 0;
+  int jj = 0;
+  bool done = false, in_cycle = false, conv = false, brk = false;
+  HistW hist{nullptr, 0};
+};
+}  // namespace
+
+static void fgmres_lockstep(hz_problem* p, int R, const double2* f, double2* x, const hz_krylov_opts& o,
+                            hz_krylov_stats* st, bool adjoint) {
   const int m = o.m;
   const int64_t N = p->N;
-  ensure_krylov(p, m);
+  const int64_t RN = (int64_t)R * N;
+  ensure_krylov(p, m, R);
   double2* V = p->kV.as<double2>();
   double2* Z = p->kZ.as<double2>();
-  double2* w = p->kw.as<double2>();
-  double2* r = p->kr.as<double2>();
-  double2* dH = dscal(p, 64);  // (m+1) x m column-major
+  double2* W = p->kw.as<double2>();
+  double2* Rv = p->kr.as<double2>();
   const int ldh = m + 1;
-  double2* dY = dH + (size_t)ldh * m;
-  HistW hist{o.hist, o.hist_cap};
-  memset(&st, 0, sizeof(st));
-  auto t0 = std::chrono::steady_clock::now();
-  const double norm_f = (nrm_host(p, f, HZ_MASK_FULL));
-  launch_fill(N, make_double2(0, 0), x, p->stream);
-  if (norm_f == 0.0) {
-    st.converged = 1;
-    sync(p);
-    return;
+  // per-member device scalars: H (ldh x m), y (m), r = V^H w (m+1), l = V^H V_j (m+1)
+  const size_t SB = (size_t)(m + 2) * (m + 2) + 4 * (m + 2) + 64;
+  double2* dTop = dscal(p, 64);
+  auto dH = [&](int q) { return dTop + SB * q; };
+  auto dY = [&](int q) { return dH(q) + (size_t)ldh * m; };
+  auto dR = [&](int q) { return dY(q) + m; };
+  auto dL = [&](int q) { return dR(q) + m + 1; };
+  double2* dNrm = dTop + SB * R;  // 2 R scalars
+  auto Vb = [&](int j, int q) { return V + ((int64_t)j * R + q) * N; };
+  auto Zb = [&](int j, int q) { return Z + ((int64_t)j * R + q) * N; };
+  const int64_t vstride = RN;
+
+  std::vector<Member> M(R);
+  for (int q = 0; q < R; ++q) {
+    Member& a = M[q];
+    a.H.assign((size_t)ldh * m, zc(0, 0));
+    a.cs.resize(m); a.sn.resize(m); a.g.resize(m + 1); a.y.resize(m);
+    a.Lg.assign((size_t)ldh * ldh, zc(0, 0));
+    a.hcol.resize(m + 2); a.yd.resize(m); a.rl.resize(2 * (m + 1));
+    if (q == 0) a.hist = HistW{o.hist, o.hist_cap};
+    memset(&st[q], 0, sizeof(st[q]));
   }
-  std::vector<zc> H((size_t)ldh * m), cs(m), sn(m), g(m + 1), y(m), Lg((size_t)ldh * ldh);
-  std::vector<double2> hcol(m + 2), yd(m), rl(2 * (m + 1));
-  double2* dR = dY + m;       // r = V^H w   (m + 1)
-  double2* dL = dR + m + 1;   // V^H V_j     (m + 1)
-  double rel = 0.0, res = 0.0;
-  int total = 0;
-  bool conv = false, brk = false;
-  for (int cyc = 0; cyc < o.max_restarts; ++cyc) {
-    residual(p, 1, f, x, r, adjoint);
-    res = (nrm_host(p, r, o.mask));
-    rel = res / norm_f;
-    if (!std::isfinite(rel)) {
-      set_error("non-finite residual in FGMRES");
-      throw Error{HZ_ERR_NONFINITE};
+  auto t0 = std::chrono::steady_clock::now();
+  std::vector<double2> hn2(2 * R);
+  for (int q = 0; q < R; ++q) launch_dot(p, N, f + q * N, nullptr, HZ_MASK_FULL, dNrm + q, p->stream);
+  HZ_HIP(hipMemcpyAsync(hn2.data(), dNrm, R * sizeof(double2), hipMemcpyDeviceToHost, p->stream));
+  launch_fill(RN, make_double2(0, 0), x, p->stream);
+  sync(p);
+  for (int q = 0; q < R; ++q) {
+    M[q].norm_f = hn2[q].x;
+    if (M[q].norm_f == 0.0) { M[q].done = true; M[q].conv = true; }
+  }
+  auto any = [&](bool Member::*flag, bool want) {
+    for (auto& a : M) if ((a.*flag) == want) return true;
+    return false;
+  };
+  // true residuals of the members still running (all R are computed: one batched call)
+  auto true_residuals = [&](bool only_running) {
+    residual(p, R, f, x, Rv, adjoint);
+    for (int q = 0; q < R; ++q) {
+      if (only_running && M[q].done) continue;
+      launch_dot(p, N, Rv + q * N, nullptr, o.mask, dNrm + q, p->stream);
+      launch_dot(p, N, Rv + q * N, nullptr, HZ_MASK_FULL, dNrm + R + q, p->stream);
     }
-    if (rel < o.tol) { conv = true; break; }
-    st.restarts++;
-    launch_dot(p, N, r, nullptr, HZ_MASK_FULL, dscal(p, 1), p->stream);  // beta^2 (sqrt on final)
-    double2 hb;
-    HZ_HIP(hipMemcpyAsync(&hb, dscal(p, 1), sizeof(double2), hipMemcpyDeviceToHost, p->stream));
+    HZ_HIP(hipMemcpyAsync(hn2.data(), dNrm, 2 * R * sizeof(double2), hipMemcpyDeviceToHost, p->stream));
     sync(p);
-    const double beta = hb.x;  // k_dot_final returns the norm in mode 1
-    launch_scale_inv_dev(N, dscal(p, 1), r, V, p->stream);
-    std::fill(g.begin(), g.end(), zc(0, 0));
-    g[0] = beta;
-    int jj = 0;
+  };
+  // Optional HIP graphs: step j's preconditioner + operator launches (fixed
+  // buffers for the whole solve) are captured on their second use -- the first
+  // runs eagerly and sets every kernel attribute -- and replayed from then on.
+  // The chain holds kernel launches only (no memset nodes).
+  struct StepGraphs {
+    std::vector<hipGraphExec_t> exec;
+    std::vector<int> uses;
+    ~StepGraphs() {
+      for (auto e : exec)
+        if (e) (void)hipGraphExecDestroy(e);
+    }
+  } graphs;
+  if (o.graph) {
+    graphs.exec.assign(m, nullptr);
+    graphs.uses.assign(m, 0);
+    ensure_work(p, R);
+  }
+  auto step_apply = [&](int j) {
+    if (!o.graph || graphs.uses[j]++ == 0) {
+      precond_apply(p, o.prec, R, Vb(j, 0), Zb(j, 0), adjoint, false);
+      op_apply(p, R, Zb(j, 0), W, adjoint);
+      return;
+    }
+    if (!graphs.exec[j]) {
+      hipGraph_t g = nullptr;
+      HZ_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+      try {
+        precond_apply(p, o.prec, R, Vb(j, 0), Zb(j, 0), adjoint, false);
+        op_apply(p, R, Zb(j, 0), W, adjoint);
+      } catch (...) {
+        (void)hipStreamEndCapture(p->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      HZ_HIP(hipStreamEndCapture(p->stream, &g));
+      const hipError_t e = hipGraphInstantiate(&graphs.exec[j], g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HZ_HIP(e);
+    }
+    HZ_HIP(hipGraphLaunch(graphs.exec[j], p->stream));
+  };
+  int total = 0;
+  for (int cyc = 0; cyc < o.max_restarts && any(&Member::done, false); ++cyc) {
+    true_residuals(true);
+    for (int q = 0; q < R; ++q) {
+      Member& a = M[q];
+      if (a.done) continue;
+      a.res = hn2[q].x;
+      a.rel = a.res / a.norm_f;
+      if (!std::isfinite(a.rel)) {
+        set_error("non-finite residual in FGMRES (right-hand side %d)", q);
+        throw Error{HZ_ERR_NONFINITE};
+      }
+      if (a.rel < o.tol) { a.conv = true; a.done = true; continue; }
+      st[q].restarts++;
+      launch_scale_inv_dev(N, dNrm + R + q, Rv + q * N, Vb(0, q), p->stream);
+      std::fill(a.g.begin(), a.g.end(), zc(0, 0));
+      a.g[0] = hn2[R + q].x;
+      a.jj = 0;
+      a.in_cycle = true;
+    }
+    if (!any(&Member::in_cycle, true)) break;
     for (int j = 0; j < m; ++j) {
-      double2* Vj = V + (size_t)N * j;
-      double2* Zj = Z + (size_t)N * j;
-      precond_apply(p, o.prec, 1, Vj, Zj, adjoint, false);
-      op_apply(p, 1, Zj, w, adjoint);
+      step_apply(j);
       total++;
-      double2* hn_d = dH + (j + 1) + (size_t)ldh * j;
+      for (int q = 0; q < R; ++q)
+        if (M[q].in_cycle) st[q].iters++;
       if (o.ortho == HZ_ORTHO_MGS_SEQ) {
         // modified Gram-Schmidt: h_ij = V_i^H w, w -= h_ij V_i in sequence; each
         // update is fused with the next coefficient's dot (the last with ||w||)
-        launch_dot(p, N, V, w, HZ_MASK_FULL, dH + (size_t)ldh * j, p->stream);
-        for (int i = 0; i <= j; ++i) {
-          double2* hij = dH + i + (size_t)ldh * j;
-          const double2* vnext = i < j ? V + (size_t)N * (i + 1) : nullptr;
-          launch_axpy_dot(p, N, hij, V + (size_t)N * i, w, vnext, hij + 1, p->stream);
+        for (int q = 0; q < R; ++q) {
+          if (!M[q].in_cycle) continue;
+          double2* wq = W + q * N;
+          double2* hc = dH(q) + (size_t)ldh * j;
+          launch_dot(p, N, Vb(0, q), wq, HZ_MASK_FULL, hc, p->stream);
+          for (int i = 0; i <= j; ++i) {
+            const double2* vnext = i < j ? Vb(i + 1, q) : nullptr;
+            launch_axpy_dot(p, N, hc + i, Vb(i, q), wq, vnext, hc + i + 1, p->stream);
+          }
         }
       } else {
         // modified Gram-Schmidt in inverse compact-WY form: r = V^H w and the new
@@ -268,84 +369,110 @@ static void fgmres_one(hz_problem* p, const double2* f, double2* x, const hz_kry
         // the host (L strict lower part of V^H V), then w -= V h and ||w|| in a
         // second pass.  Same coefficients as the sequential form in exact
         // arithmetic and the same O(eps) loss of orthogonality.
-        launch_multidot(p, N, j + 1, V, N, w, V + (size_t)N * j, dR, dL, p->stream);
-        HZ_HIP(hipMemcpyAsync(rl.data(), dR, (j + 1) * sizeof(double2), hipMemcpyDeviceToHost, p->stream));
-        HZ_HIP(hipMemcpyAsync(rl.data() + (j + 1), dL, (j + 1) * sizeof(double2), hipMemcpyDeviceToHost, p->stream));
-        sync(p);
-        for (int q = 0; q < j; ++q) Lg[(size_t)j * ldh + q] = std::conj(zz(rl[j + 1 + q]));  // V_j^H V_q
-        for (int i = 0; i <= j; ++i) {
-          zc acc = zz(rl[i]);
-          for (int q = 0; q < i; ++q) acc -= Lg[(size_t)i * ldh + q] * zz(hcol[q]);
-          hcol[i] = d2(acc);
+        for (int q = 0; q < R; ++q) {
+          if (!M[q].in_cycle) continue;
+          launch_multidot(p, N, j + 1, Vb(0, q), vstride, W + q * N, Vb(j, q), dR(q), dL(q), p->stream);
+          HZ_HIP(hipMemcpyAsync(M[q].rl.data(), dR(q), (size_t)2 * (m + 1) * sizeof(double2), hipMemcpyDeviceToHost,
+                                p->stream));
         }
-        HZ_HIP(hipMemcpyAsync(dH + (size_t)ldh * j, hcol.data(), (j + 1) * sizeof(double2), hipMemcpyHostToDevice,
-                              p->stream));
-        launch_multi_axpy_norm(p, N, j + 1, V, N, dH + (size_t)ldh * j, w, hn_d, p->stream);
+        sync(p);
+        for (int q = 0; q < R; ++q) {
+          Member& a = M[q];
+          if (!a.in_cycle) continue;
+          const double2* rr = a.rl.data();
+          const double2* ll = a.rl.data() + (m + 1);
+          for (int c = 0; c < j; ++c) a.Lg[(size_t)j * ldh + c] = std::conj(zz(ll[c]));  // V_j^H V_c
+          for (int i = 0; i <= j; ++i) {
+            zc acc = zz(rr[i]);
+            for (int c = 0; c < i; ++c) acc -= a.Lg[(size_t)i * ldh + c] * zz(a.hcol[c]);
+            a.hcol[i] = d2(acc);
+          }
+          HZ_HIP(hipMemcpyAsync(dH(q) + (size_t)ldh * j, a.hcol.data(), (j + 1) * sizeof(double2),
+                                hipMemcpyHostToDevice, p->stream));
+          launch_multi_axpy_norm(p, N, j + 1, Vb(0, q), vstride, dH(q) + (size_t)ldh * j, W + q * N,
+                                 dH(q) + (j + 1) + (size_t)ldh * j, p->stream);
+        }
       }
-      HZ_HIP(hipMemcpyAsync(hcol.data(), dH + (size_t)ldh * j, (j + 2) * sizeof(double2), hipMemcpyDeviceToHost,
-                            p->stream));
+      for (int q = 0; q < R; ++q)
+        if (M[q].in_cycle)
+          HZ_HIP(hipMemcpyAsync(M[q].hcol.data(), dH(q) + (size_t)ldh * j, (j + 2) * sizeof(double2),
+                                hipMemcpyDeviceToHost, p->stream));
       sync(p);
-      for (int i = 0; i <= j + 1; ++i) H[i + (size_t)ldh * j] = zz(hcol[i]);
-      const double hn = hcol[j + 1].x;
-      for (int i = 0; i < j; ++i) {
-        zc a = H[i + (size_t)ldh * j], b = H[i + 1 + (size_t)ldh * j];
-        H[i + (size_t)ldh * j] = cs[i] * a + sn[i] * b;
-        H[i + 1 + (size_t)ldh * j] = -std::conj(sn[i]) * a + cs[i] * b;
+      for (int q = 0; q < R; ++q) {
+        Member& a = M[q];
+        if (!a.in_cycle) continue;
+        std::vector<zc>& H = a.H;
+        for (int i = 0; i <= j + 1; ++i) H[i + (size_t)ldh * j] = zz(a.hcol[i]);
+        const double hn = a.hcol[j + 1].x;
+        for (int i = 0; i < j; ++i) {
+          zc u = H[i + (size_t)ldh * j], w2 = H[i + 1 + (size_t)ldh * j];
+          H[i + (size_t)ldh * j] = a.cs[i] * u + a.sn[i] * w2;
+          H[i + 1 + (size_t)ldh * j] = -std::conj(a.sn[i]) * u + a.cs[i] * w2;
+        }
+        zc u = H[j + (size_t)ldh * j], w2 = H[j + 1 + (size_t)ldh * j];
+        const double aa = std::abs(u), bb = std::abs(w2);
+        if (bb == 0.0) { a.cs[j] = 1.0; a.sn[j] = 0.0; }
+        else if (aa == 0.0) { a.cs[j] = 0.0; a.sn[j] = std::conj(w2) / bb; }
+        else {
+          const double t = std::sqrt(aa * aa + bb * bb);
+          a.cs[j] = aa / t;
+          a.sn[j] = (u / aa) * std::conj(w2) / t;
+        }
+        H[j + (size_t)ldh * j] = a.cs[j] * u + a.sn[j] * w2;
+        H[j + 1 + (size_t)ldh * j] = 0.0;
+        a.g[j + 1] = -std::conj(a.sn[j]) * a.g[j];
+        a.g[j] = a.cs[j] * a.g[j];
+        a.jj = j + 1;
+        const double est = std::abs(a.g[j + 1]) / a.norm_f;
+        a.hist.push(est);
+        if (!std::isfinite(est)) {
+          set_error("non-finite Arnoldi coefficient in FGMRES (right-hand side %d)", q);
+          throw Error{HZ_ERR_NONFINITE};
+        }
+        if (est < o.tol) { a.in_cycle = false; continue; }
+        if (hn <= o.breakdown) { a.brk = true; a.in_cycle = false; continue; }
+        launch_scale_inv_dev(N, dH(q) + (j + 1) + (size_t)ldh * j, W + q * N, Vb(j + 1, q), p->stream);
       }
-      zc a = H[j + (size_t)ldh * j], b = H[j + 1 + (size_t)ldh * j];
-      const double aa = std::abs(a), bb = std::abs(b);
-      if (bb == 0.0) { cs[j] = 1.0; sn[j] = 0.0; }
-      else if (aa == 0.0) { cs[j] = 0.0; sn[j] = std::conj(b) / bb; }
-      else {
-        const double t = std::sqrt(aa * aa + bb * bb);
-        cs[j] = aa / t;
-        sn[j] = (a / aa) * std::conj(b) / t;
-      }
-      H[j + (size_t)ldh * j] = cs[j] * a + sn[j] * b;
-      H[j + 1 + (size_t)ldh * j] = 0.0;
-      g[j + 1] = -std::conj(sn[j]) * g[j];
-      g[j] = cs[j] * g[j];
-      jj = j + 1;
-      const double est = std::abs(g[j + 1]) / norm_f;
-      hist.push(est);
-      if (!std::isfinite(est)) {
-        set_error("non-finite Arnoldi coefficient in FGMRES");
-        throw Error{HZ_ERR_NONFINITE};
-      }
-      if (est < o.tol) break;
-      if (hn <= o.breakdown) { brk = true; break; }
-      launch_scale_inv_dev(N, hn_d, w, V + (size_t)N * (j + 1), p->stream);
+      if (!any(&Member::in_cycle, true)) break;
     }
-    for (int i = jj - 1; i >= 0; --i) {
-      zc s = g[i];
-      for (int c = i + 1; c < jj; ++c) s -= H[i + (size_t)ldh * c] * y[c];
-      y[i] = s / H[i + (size_t)ldh * i];
+    // x_q += Z_q y_q for every member that ran this cycle
+    for (int q = 0; q < R; ++q) {
+      Member& a = M[q];
+      if (a.done || a.jj == 0) continue;
+      a.in_cycle = false;
+      for (int i = a.jj - 1; i >= 0; --i) {
+        zc sacc = a.g[i];
+        for (int c = i + 1; c < a.jj; ++c) sacc -= a.H[i + (size_t)ldh * c] * a.y[c];
+        a.y[i] = sacc / a.H[i + (size_t)ldh * i];
+      }
+      for (int i = 0; i < a.jj; ++i) a.yd[i] = d2(a.y[i]);
+      HZ_HIP(hipMemcpyAsync(dY(q), a.yd.data(), a.jj * sizeof(double2), hipMemcpyHostToDevice, p->stream));
+      launch_lincomb(N, a.jj, dY(q), Zb(0, q), vstride, x + q * N, true, p->stream);
+      a.jj = 0;
+      if (a.brk) a.done = true;
     }
-    for (int i = 0; i < jj; ++i) yd[i] = d2(y[i]);
-    HZ_HIP(hipMemcpyAsync(dY, yd.data(), jj * sizeof(double2), hipMemcpyHostToDevice, p->stream));
-    launch_lincomb(N, jj, dY, Z, N, x, true, p->stream);
     sync(p);
-    if (brk) {
-      st.breakdown = 1;
-      residual(p, 1, f, x, r, adjoint);
-      res = (nrm_host(p, r, o.mask));
-      rel = res / norm_f;
-      conv = rel < o.tol;
-      break;
+  }
+  // final true residuals (also for members that stopped on breakdown)
+  true_residuals(false);
+  for (int q = 0; q < R; ++q) {
+    Member& a = M[q];
+    if (a.norm_f == 0.0) {
+      st[q].converged = 1;
+      continue;
     }
+    a.res = hn2[q].x;
+    a.rel = a.res / a.norm_f;
+    a.conv = a.rel < o.tol;
+    st[q].converged = a.conv;
+    st[q].breakdown = a.brk && !a.conv;
+    st[q].rel_res = a.rel;
+    st[q].res = a.res;
+    st[q].n_hist = a.hist.n;
   }
-  if (!conv && !brk) {
-    residual(p, 1, f, x, r, adjoint);
-    res = (nrm_host(p, r, o.mask));
-    rel = res / norm_f;
-    conv = rel < o.tol;
-  }
-  st.iters = total;
-  st.converged = conv;
-  st.rel_res = rel;
-  st.res = res;
-  st.n_hist = hist.n;
-  st.t_solve_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  (void)total;
+  const double ts = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  for (int q = 0; q < R; ++q) st[q].t_solve_s = ts;
 }
 
 // The reference's restarted GMRES on (I - deltaL L0^{-1}) w = f
@@ -704,13 +831,21 @@ int hz_fgmres(hz_problem* p, int nrhs, const hz_c128* f, hz_c128* u, const hz_kr
   HZ_CHECK(p && f && u && o && st && nrhs >= 1, HZ_ERR_ARG, "bad arguments");
   HZ_CHECK(o->m >= 1 && o->m <= 2000 && o->max_restarts >= 1, HZ_ERR_ARG, "bad restart parameters");
   int rc = HZ_OK;
+  if (o->driver != HZ_DRIVER_GMRES_REF) {
+    const int B = std::max(1, std::min(o->batch > 0 ? o->batch : 16, nrhs));
+    for (int q0 = 0; q0 < nrhs; q0 += B) {
+      hz_krylov_opts oq = *o;
+      if (q0 > 0) { oq.hist = nullptr; oq.hist_cap = 0; }
+      fgmres_lockstep(p, std::min(B, nrhs - q0), (const double2*)f + (size_t)q0 * p->N, (double2*)u + (size_t)q0 * p->N,
+                      oq, st + q0, false);
+    }
+  }
   for (int q = 0; q < nrhs; ++q) {
-    const double2* fq = (const double2*)f + (size_t)q * p->N;
-    double2* uq = (double2*)u + (size_t)q * p->N;
-    hz_krylov_opts oq = *o;
-    if (q > 0) { oq.hist = nullptr; oq.hist_cap = 0; }
-    if (o->driver == HZ_DRIVER_GMRES_REF) gmres_ref_one(p, fq, uq, oq, st[q]);
-    else fgmres_one(p, fq, uq, oq, st[q], false);
+    if (o->driver == HZ_DRIVER_GMRES_REF) {
+      hz_krylov_opts oq = *o;
+      if (q > 0) { oq.hist = nullptr; oq.hist_cap = 0; }
+      gmres_ref_one(p, (const double2*)f + (size_t)q * p->N, (double2*)u + (size_t)q * p->N, oq, st[q]);
+    }
     if (!st[q].converged && rc == HZ_OK) {
       rc = st[q].breakdown ? HZ_ERR_BREAKDOWN : HZ_ERR_NOT_CONVERGED;
       set_error("Krylov solve %d stopped at relative residual %.3e after %d iterations", q, st[q].rel_res,
@@ -748,19 +883,20 @@ int hz_fft_z(hz_problem* p, int prec, int sign, int nrhs, const hz_c128* in, hz_
 }
 
 // ------------------------------------------------------------- misfit
-static void scatter_rec(hz_problem* p, const std::vector<double2>& vals, double2* dst) {
-  // dst = 0; dst[rec[q]] += vals[q]  (receivers are distinct after the check in forward)
-  launch_fill(p->N, make_double2(0, 0), dst, p->stream);
-  for (size_t q = 0; q < p->rec.size(); ++q)
-    HZ_HIP(hipMemcpyAsync(dst + p->rec[q], &vals[q], sizeof(double2), hipMemcpyHostToDevice, p->stream));
-  sync(p);
+// Sources run in lockstep batches (opts.batch, default min(nsrc, 16)); the
+// receiver restriction P and its transpose are device gathers/scatters.
+static int misfit_batch(const hz_problem* p) {
+  const int b = p->mopts.batch > 0 ? p->mopts.batch : 16;
+  return std::max(1, std::min(b, p->nsrc));
 }
 
-static void gather_rec(hz_problem* p, const double2* src, std::vector<double2>& vals) {
-  vals.resize(p->rec.size());
-  for (size_t q = 0; q < p->rec.size(); ++q)
-    HZ_HIP(hipMemcpyAsync(&vals[q], src + p->rec[q], sizeof(double2), hipMemcpyDeviceToHost, p->stream));
-  sync(p);
+// forward fields of sources [s0, s0 + nb) into us (stride N)
+static void misfit_solve_forward(hz_problem* p, int s0, int nb, double2* us) {
+  std::vector<hz_krylov_stats> st(nb);
+  fgmres_lockstep(p, nb, p->src_f.as<double2>() + (size_t)p->N * s0, us, p->mopts, st.data(), false);
+  for (int q = 0; q < nb; ++q)
+    HZ_CHECK(st[q].converged, HZ_ERR_NOT_CONVERGED, "forward solve %d: relative residual %.3e", s0 + q,
+             st[q].rel_res);
 }
 
 int hz_misfit_forward(hz_problem* p, int nsrc, const hz_c128* f, int nrec, const int64_t* rec, const hz_c128* data,
@@ -773,32 +909,38 @@ int hz_misfit_forward(hz_problem* p, int nsrc, const hz_c128* f, int nrec, const
   std::vector<int64_t> sorted = r;
   std::sort(sorted.begin(), sorted.end());
   HZ_CHECK(std::adjacent_find(sorted.begin(), sorted.end()) == sorted.end(), HZ_ERR_ARG, "duplicate receivers");
+  p->misfit_ready = false;
   p->rec = r;
   p->nsrc = nsrc;
-  p->data.assign((const double2*)data, (const double2*)data + (size_t)nsrc * nrec);
   p->mopts = *o;
   p->mopts.driver = HZ_DRIVER_FGMRES;
   p->mopts.hist = nullptr;
   p->mopts.hist_cap = 0;
+  const int B = misfit_batch(p);
   const size_t vb = (size_t)p->N * sizeof(double2);
+  p->d_rec.alloc(sizeof(int64_t) * nrec);
+  p->d_data.alloc(sizeof(double2) * (size_t)nsrc * nrec);
+  p->d_res.alloc(sizeof(double2) * (size_t)B * nrec);
+  HZ_HIP(hipMemcpyAsync(p->d_rec.ptr, r.data(), sizeof(int64_t) * nrec, hipMemcpyHostToDevice, p->stream));
+  HZ_HIP(hipMemcpyAsync(p->d_data.ptr, data, sizeof(double2) * (size_t)nsrc * nrec, hipMemcpyHostToDevice,
+                        p->stream));
   p->src_f.alloc(vb * nsrc);
   HZ_HIP(hipMemcpyAsync(p->src_f.ptr, f, vb * nsrc, hipMemcpyDeviceToDevice, p->stream));
   p->fields_stored = store_fields != 0;
   if (p->fields_stored) p->fields_u.alloc(vb * nsrc);
-  else grow(p->fields_u, vb);
+  else grow(p->fields_u, vb * B);
   double Jt = 0.0;
-  std::vector<double2> pu;
-  for (int s = 0; s < nsrc; ++s) {
-    double2* us = p->fields_u.as<double2>() + (p->fields_stored ? (size_t)p->N * s : 0);
-    hz_krylov_stats st;
-    fgmres_one(p, p->src_f.as<double2>() + (size_t)p->N * s, us, p->mopts, st, false);
-    HZ_CHECK(st.converged, HZ_ERR_NOT_CONVERGED, "forward solve %d: relative residual %.3e", s, st.rel_res);
-    gather_rec(p, us, pu);
-    for (int q = 0; q < nrec; ++q) {
-      const double2 d = p->data[(size_t)s * nrec + q];
-      const double rx = pu[q].x - d.x, ry = pu[q].y - d.y;
-      Jt += 0.5 * (rx * rx + ry * ry);
-    }
+  std::vector<double2> res((size_t)B * nrec);
+  for (int s0 = 0; s0 < nsrc; s0 += B) {
+    const int nb = std::min(B, nsrc - s0);
+    double2* us = p->fields_u.as<double2>() + (p->fields_stored ? (size_t)p->N * s0 : 0);
+    misfit_solve_forward(p, s0, nb, us);
+    launch_rec_residual(nrec, nb, p->d_rec.as<int64_t>(), us, p->N, p->d_data.as<double2>() + (size_t)s0 * nrec,
+                        p->d_res.as<double2>(), p->stream);
+    HZ_HIP(hipMemcpyAsync(res.data(), p->d_res.ptr, sizeof(double2) * (size_t)nb * nrec, hipMemcpyDeviceToHost,
+                          p->stream));
+    sync(p);
+    for (size_t i = 0; i < (size_t)nb * nrec; ++i) Jt += 0.5 * (res[i].x * res[i].x + res[i].y * res[i].y);
   }
   p->misfit_ready = true;
   *J = Jt;
@@ -812,27 +954,27 @@ int hz_misfit_backward(hz_problem* p, double* grad) {
   const int64_t N = p->N;
   const int nrec = (int)p->rec.size();
   const double w2 = p->d.omega * p->d.omega;
-  grow(p->m_adj, (size_t)N * sizeof(double2) * 2);
+  const int B = misfit_batch(p);
+  grow(p->m_adj, (size_t)N * sizeof(double2) * 2 * B);
   double2* rhs = p->m_adj.as<double2>();
-  double2* lam = rhs + N;
+  double2* lam = rhs + (size_t)B * N;
   launch_fill_real(N, 0.0, grad, p->stream);
-  std::vector<double2> pu, res(nrec);
-  for (int s = 0; s < p->nsrc; ++s) {
-    double2* us = p->fields_u.as<double2>() + (p->fields_stored ? (size_t)N * s : 0);
-    hz_krylov_stats st;
-    if (!p->fields_stored) {
-      fgmres_one(p, p->src_f.as<double2>() + (size_t)N * s, us, p->mopts, st, false);
-      HZ_CHECK(st.converged, HZ_ERR_NOT_CONVERGED, "recomputed forward solve %d did not converge", s);
-    }
-    gather_rec(p, us, pu);
-    for (int q = 0; q < nrec; ++q) {
-      const double2 d = p->data[(size_t)s * nrec + q];
-      res[q] = make_double2(pu[q].x - d.x, pu[q].y - d.y);
-    }
-    scatter_rec(p, res, rhs);
-    fgmres_one(p, rhs, lam, p->mopts, st, true);
-    HZ_CHECK(st.converged, HZ_ERR_NOT_CONVERGED, "adjoint solve %d: relative residual %.3e", s, st.rel_res);
-    launch_grad_acc(N, -w2, us, lam, grad, p->stream);
+  std::vector<hz_krylov_stats> st(B);
+  for (int s0 = 0; s0 < p->nsrc; s0 += B) {
+    const int nb = std::min(B, p->nsrc - s0);
+    double2* us = p->fields_u.as<double2>() + (p->fields_stored ? (size_t)N * s0 : 0);
+    if (!p->fields_stored) misfit_solve_forward(p, s0, nb, us);
+    // adjoint sources P^T (P u_s - d_s)
+    launch_rec_residual(nrec, nb, p->d_rec.as<int64_t>(), us, N, p->d_data.as<double2>() + (size_t)s0 * nrec,
+                        p->d_res.as<double2>(), p->stream);
+    launch_fill((int64_t)nb * N, make_double2(0, 0), rhs, p->stream);
+    launch_rec_scatter(nrec, nb, p->d_rec.as<int64_t>(), p->d_res.as<double2>(), rhs, N, p->stream);
+    fgmres_lockstep(p, nb, rhs, lam, p->mopts, st.data(), true);
+    for (int q = 0; q < nb; ++q)
+      HZ_CHECK(st[q].converged, HZ_ERR_NOT_CONVERGED, "adjoint solve %d: relative residual %.3e", s0 + q,
+               st[q].rel_res);
+    // grad += -w^2 sum_s Re(conj(u_s) lambda_s)
+    launch_grad_acc_multi(N, nb, -w2, us, N, lam, N, grad, p->stream);
   }
   sync(p);
   HZ_API_END

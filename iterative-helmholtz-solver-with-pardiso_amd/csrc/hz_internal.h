@@ -100,7 +100,9 @@ struct hz_problem {
 
   // misfit state
   std::vector<int64_t> rec;
-  std::vector<double2> data;
+  hz::DevBuf d_rec;     // receiver indices (int64)
+  hz::DevBuf d_data;    // nsrc * nrec observed data
+  hz::DevBuf d_res;     // batch * nrec receiver residuals
   int nsrc = 0;
   hz::DevBuf fields_u;  // nsrc * N (stored forward fields)
   hz::DevBuf src_f;     // nsrc * N copy of the sources (recompute policy)
@@ -138,6 +140,10 @@ void launch_stencil(hz_problem* p, int nrhs, const double2* u, double2* y, bool 
 // plane 5-point matvec on z-transformed data: y_k = M_k x_k (adjoint: M_k^H x_k).
 void launch_plane_matvec(hz_problem* p, int nrhs, const double2* x, double2* y, bool adjoint, hipStream_t s);
 
+// Raise a kernel's dynamic-LDS limit to at least `bytes` (hipFuncSetAttribute
+// only when the limit grows, so that launches inside a stream capture make no
+// attribute calls once the kernel has run eagerly).
+void set_max_lds(const void* fn, size_t bytes);
 // block-tridiagonal factorisation of every plane, precision prec.
 void launch_factor(hz_problem* p, int prec, hipStream_t s);
 // in-place plane solves on z-transformed data (prec 0: double2, 1: float2).
@@ -179,6 +185,12 @@ void launch_sub(int64_t n, const double2* a, const double2* b, double2* out, hip
 void launch_w_minus_dl(int64_t n, const double2* w, const double2* dL, const double2* x, double2* out,
                        hipStream_t s);
 // grad[l] += scale * Re(conj(u[l]) * lam[l])
+void launch_grad_acc_multi(int64_t n, int ns, double scale, const double2* u, int64_t us, const double2* lam,
+                           int64_t ls, double* grad, hipStream_t s);
+void launch_rec_residual(int64_t nrec, int ns, const int64_t* idx, const double2* U, int64_t ustride,
+                         const double2* data, double2* res, hipStream_t s);
+void launch_rec_scatter(int64_t nrec, int ns, const int64_t* idx, const double2* vals, double2* dst, int64_t dstride,
+                        hipStream_t s);
 void launch_grad_acc(int64_t n, double scale, const double2* u, const double2* lam, double* grad, hipStream_t s);
 void launch_fill(int64_t n, double2 v, double2* y, hipStream_t s);
 void launch_fill_real(int64_t n, double v, double* y, hipStream_t s);

@@ -4,6 +4,9 @@
 #include <cmath>
 #include <complex>
 
+#include <mutex>
+#include <unordered_map>
+
 #include "hz_internal.h"
 
 namespace hz {
@@ -231,6 +234,16 @@ void setup_tables(hz_problem* p, const double* c) {
     HZ_HIP(hipMemcpy(p->d_dL.ptr, dL.data(), p->N * sizeof(double2), hipMemcpyHostToDevice));
   }
   build_fft_plan(p);
+}
+
+void set_max_lds(const void* fn, size_t bytes) {
+  static std::mutex mu;
+  static std::unordered_map<const void*, size_t> done;
+  std::lock_guard<std::mutex> lk(mu);
+  size_t& have = done[fn];
+  if (bytes <= have) return;
+  HZ_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  have = bytes;
 }
 
 }  // namespace hz

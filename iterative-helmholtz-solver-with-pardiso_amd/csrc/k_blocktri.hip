@@ -1119,8 +1119,7 @@ bool try_launch_pf(int cpg, int ncg, int nru, dim3 grid, hipStream_t s, const vo
   if (cpg != CPG) return false;
   typedef typename Cx<T>::t C;
   const size_t lds = sizeof(C) * (size_t)n * (1 + ncg);
-  HZ_HIP(hipFuncSetAttribute((const void*)&k_solve_pf<T, RPL, CPG>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)lds));
+  set_max_lds((const void*)&k_solve_pf<T, RPL, CPG>, lds);
   hipLaunchKernelGGL((k_solve_pf<T, RPL, CPG>), grid, dim3(PF_THREADS), lds, s, (const C*)G, (C*)x, cmy, cpy, n, ny,
                      n2, N, nru, ncg);
   HZ_HIP(hipGetLastError());
@@ -1198,14 +1197,14 @@ void launch_factor(hz_problem* p, int prec, hipStream_t s) {
     while (nb > 4 && factor_lds<double>(n, nb) > 160 * 1024) nb /= 2;
     const size_t lds = factor_lds<double>(n, nb);
     HZ_CHECK(lds <= 160 * 1024, HZ_ERR_ARG, "nx=%d too large for the LDS panel", n);
-    HZ_HIP(hipFuncSetAttribute((const void*)&k_factor<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    set_max_lds((const void*)&k_factor<double>, lds);
     hipLaunchKernelGGL(k_factor<double>, dim3((unsigned)p->nz), dim3(FACT_THREADS), lds, s, f.G.as<double2>(), tb,
                        n, ny, nb, st.as<int>());
   } else {
     while (nb > 4 && factor_lds<float>(n, nb) > 160 * 1024) nb /= 2;
     const size_t lds = factor_lds<float>(n, nb);
     HZ_CHECK(lds <= 160 * 1024, HZ_ERR_ARG, "nx=%d too large for the LDS panel", n);
-    HZ_HIP(hipFuncSetAttribute((const void*)&k_factor<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    set_max_lds((const void*)&k_factor<float>, lds);
     hipLaunchKernelGGL(k_factor<float>, dim3((unsigned)p->nz), dim3(FACT_THREADS), lds, s, f.G.as<float2>(), tb, n,
                        ny, nb, st.as<int>());
   }
@@ -1273,7 +1272,7 @@ void launch_plane_solve(hz_problem* p, int prec, int nrhs, void* x, bool adjoint
     if (prec == HZ_PREC_C64 && n % 2 == 0) {
       const int nru = n / 2, ncg = std::max(1, 1024 / nru), cpg = (n + ncg - 1) / ncg;
       const size_t lds = sizeof(float2) * (size_t)n * (1 + ncg);
-      HZ_HIP(hipFuncSetAttribute((const void*)&k_solve_fast<float, 2, CH>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      set_max_lds((const void*)&k_solve_fast<float, 2, CH>, lds);
       hipLaunchKernelGGL((k_solve_fast<float, 2, CH>), grid, dim3(1024), lds, s, f.G.as<float2>(), (float2*)x, cmy, cpy,
                          n, ny, p->n2, p->N, nru, ncg, cpg);
       HZ_HIP(hipGetLastError());
@@ -1282,7 +1281,7 @@ void launch_plane_solve(hz_problem* p, int prec, int nrhs, void* x, bool adjoint
     if (prec == HZ_PREC_FP64) {
       const int nru = n, ncg = std::max(1, 1024 / nru), cpg = (n + ncg - 1) / ncg;
       const size_t lds = sizeof(double2) * (size_t)n * (1 + ncg);
-      HZ_HIP(hipFuncSetAttribute((const void*)&k_solve_fast<double, 1, CH>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      set_max_lds((const void*)&k_solve_fast<double, 1, CH>, lds);
       hipLaunchKernelGGL((k_solve_fast<double, 1, CH>), grid, dim3(1024), lds, s, f.G.as<double2>(), (double2*)x, cmy,
                          cpy, n, ny, p->n2, p->N, nru, ncg, cpg);
       HZ_HIP(hipGetLastError());
@@ -1293,22 +1292,22 @@ void launch_plane_solve(hz_problem* p, int prec, int nrhs, void* x, bool adjoint
     const size_t lds = sizeof(double2) * (size_t)n * (2 + ng);
     HZ_CHECK(lds <= 160 * 1024, HZ_ERR_ARG, "nx too large for the apply kernel");
     if (adjoint) {
-      HZ_HIP(hipFuncSetAttribute((const void*)&k_apply<double, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      set_max_lds((const void*)&k_apply<double, true>, lds);
       hipLaunchKernelGGL((k_apply<double, true>), grid, dim3(APPLY_THREADS), lds, s, f.G.as<double2>(), (double2*)x, tb,
                          cmy, cpy, n, ny, p->n2, p->N);
     } else {
-      HZ_HIP(hipFuncSetAttribute((const void*)&k_apply<double, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      set_max_lds((const void*)&k_apply<double, false>, lds);
       hipLaunchKernelGGL((k_apply<double, false>), grid, dim3(APPLY_THREADS), lds, s, f.G.as<double2>(), (double2*)x,
                          tb, cmy, cpy, n, ny, p->n2, p->N);
     }
   } else {
     const size_t lds = sizeof(float2) * (size_t)n * (2 + ng);
     if (adjoint) {
-      HZ_HIP(hipFuncSetAttribute((const void*)&k_apply<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      set_max_lds((const void*)&k_apply<float, true>, lds);
       hipLaunchKernelGGL((k_apply<float, true>), grid, dim3(APPLY_THREADS), lds, s, f.G.as<float2>(), (float2*)x, tb,
                          cmy, cpy, n, ny, p->n2, p->N);
     } else {
-      HZ_HIP(hipFuncSetAttribute((const void*)&k_apply<float, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      set_max_lds((const void*)&k_apply<float, false>, lds);
       hipLaunchKernelGGL((k_apply<float, false>), grid, dim3(APPLY_THREADS), lds, s, f.G.as<float2>(), (float2*)x, tb,
                          cmy, cpy, n, ny, p->n2, p->N);
     }

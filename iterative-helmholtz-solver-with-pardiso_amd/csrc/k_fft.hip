@@ -174,8 +174,7 @@ void launch_typed(hz_problem* p, int sign, int nrhs, const void* in, void* out, 
   HZ_CHECK(lds <= 160 * 1024, HZ_ERR_ARG, "nz=%d too long for the LDS-resident FFT", a.n);
   dim3 grid((unsigned)((p->n2 + LINES - 1) / LINES), (unsigned)nrhs);
   if (lds > 64 * 1024)
-    HZ_HIP(hipFuncSetAttribute((const void*)&k_fft_z<T, TI, TO, EPI, LINES>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    set_max_lds((const void*)&k_fft_z<T, TI, TO, EPI, LINES>, lds);
   T scale = sign > 0 ? (T)(1.0 / (double)a.n) : (T)1;
   hipLaunchKernelGGL((k_fft_z<T, TI, TO, EPI, LINES>), grid, dim3(256), lds, s, (const TI*)in, (TO*)out, p->n2,
                      p->N, a, tw, sign, scale, w, dL);

@@ -293,6 +293,40 @@ __global__ void k_grad_acc(int64_t n, double scale, const double2* __restrict__ 
   }
 }
 
+// grad[l] += scale * sum_s Re(conj(u_s[l]) lam_s[l]) over ns fields (one pass)
+__global__ void k_grad_acc_multi(int64_t n, int ns, double scale, const double2* __restrict__ u, int64_t us,
+                                 const double2* __restrict__ lam, int64_t ls, double* __restrict__ g) {
+  for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += (int64_t)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int s = 0; s < ns; ++s) {
+      const double2 a = u[l + us * s], b = lam[l + ls * s];
+      acc += a.x * b.x + a.y * b.y;
+    }
+    g[l] += scale * acc;
+  }
+}
+
+// res[s][i] = U_s[idx[i]] - data[s][i]  (receiver residuals of ns fields)
+__global__ void k_rec_residual(int64_t nrec, int ns, const int64_t* __restrict__ idx, const double2* __restrict__ U,
+                               int64_t ustride, const double2* __restrict__ data, double2* __restrict__ res) {
+  const int64_t tot = nrec * ns;
+  for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < tot; l += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = l / nrec, i = l - s * nrec;
+    const double2 u = U[ustride * s + idx[i]], d = data[l];
+    res[l] = make_double2(u.x - d.x, u.y - d.y);
+  }
+}
+
+// dst_s[idx[i]] = vals[s][i]  (receivers distinct; dst cleared beforehand)
+__global__ void k_rec_scatter(int64_t nrec, int ns, const int64_t* __restrict__ idx, const double2* __restrict__ vals,
+                              double2* __restrict__ dst, int64_t dstride) {
+  const int64_t tot = nrec * ns;
+  for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < tot; l += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = l / nrec, i = l - s * nrec;
+    dst[dstride * s + idx[i]] = vals[l];
+  }
+}
+
 __global__ void k_fill(int64_t n, double2 v, double2* __restrict__ y) {
   for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += (int64_t)gridDim.x * blockDim.x) y[l] = v;
 }
@@ -419,6 +453,24 @@ void launch_w_minus_dl(int64_t n, const double2* w, const double2* dL, const dou
 
 void launch_grad_acc(int64_t n, double scale, const double2* u, const double2* lam, double* grad, hipStream_t s) {
   hipLaunchKernelGGL(k_grad_acc, dim3(grid_for(n)), dim3(256), 0, s, n, scale, u, lam, grad);
+  HZ_HIP(hipGetLastError());
+}
+
+void launch_grad_acc_multi(int64_t n, int ns, double scale, const double2* u, int64_t us, const double2* lam,
+                           int64_t ls, double* grad, hipStream_t s) {
+  hipLaunchKernelGGL(k_grad_acc_multi, dim3(grid_for(n)), dim3(256), 0, s, n, ns, scale, u, us, lam, ls, grad);
+  HZ_HIP(hipGetLastError());
+}
+
+void launch_rec_residual(int64_t nrec, int ns, const int64_t* idx, const double2* U, int64_t ustride,
+                         const double2* data, double2* res, hipStream_t s) {
+  hipLaunchKernelGGL(k_rec_residual, dim3(grid_for(nrec * ns)), dim3(256), 0, s, nrec, ns, idx, U, ustride, data, res);
+  HZ_HIP(hipGetLastError());
+}
+
+void launch_rec_scatter(int64_t nrec, int ns, const int64_t* idx, const double2* vals, double2* dst, int64_t dstride,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_rec_scatter, dim3(grid_for(nrec * ns)), dim3(256), 0, s, nrec, ns, idx, vals, dst, dstride);
   HZ_HIP(hipGetLastError());
 }
 

@@ -151,13 +151,17 @@ class Solver:
 
     # ----------------------------------------------------------- drivers
     def fgmres(self, f, m=30, restarts=20, tol=1e-8, driver=DRIVER_FGMRES, prec=PREC_FP64, ortho=ORTHO_MGS,
-               mask=MASK_FULL, monitor=0, breakdown=0.0, hist_cap=8192, check=False):
+               mask=MASK_FULL, monitor=0, breakdown=0.0, hist_cap=8192, check=False, batch=0, graph=False):
+        """Solve for every row of f (shape (N,) or (nrhs, N)); FGMRES advances
+        `batch` right-hand sides in lockstep (0: up to 16), `graph` replays each
+        Arnoldi step's launches from a captured HIP graph."""
         f, nr = self._vec(f)
         u = _torch().zeros_like(f)
         hist = np.zeros(hist_cap)
         o = abi.KrylovOpts(driver=int(driver), m=int(m), max_restarts=int(restarts), ortho=int(ortho),
                            mask=int(mask), monitor=int(monitor), tol=float(tol), breakdown=float(breakdown),
-                           hist=hist.ctypes.data_as(C.POINTER(C.c_double)), hist_cap=hist_cap, prec=int(prec))
+                           hist=hist.ctypes.data_as(C.POINTER(C.c_double)), hist_cap=hist_cap, prec=int(prec),
+                           batch=int(batch), graph=int(bool(graph)))
         st = (abi.KrylovStats * nr)()
         _torch().cuda.synchronize()
         rc = abi.lib().hz_fgmres(self.h, nr, _ptr(f), _ptr(u), C.byref(o), st)
@@ -180,13 +184,14 @@ class Solver:
         return u, (stats[0] if nr == 1 else stats)
 
     # ----------------------------------------------------------- adjoint
-    def misfit_forward(self, f, rec, data, m=40, restarts=20, tol=1e-8, prec=PREC_C64, store_fields=True):
+    def misfit_forward(self, f, rec, data, m=40, restarts=20, tol=1e-8, prec=PREC_C64, store_fields=True, batch=0,
+                       graph=False):
         f, nsrc = self._vec(f)
         rec = np.ascontiguousarray(rec, dtype=np.int64)
         data = np.ascontiguousarray(data, dtype=np.complex128).reshape(nsrc, rec.size)
         o = abi.KrylovOpts(driver=DRIVER_FGMRES, m=int(m), max_restarts=int(restarts), ortho=ORTHO_MGS,
                            mask=MASK_FULL, monitor=0, tol=float(tol), breakdown=0.0, hist=None, hist_cap=0,
-                           prec=int(prec))
+                           prec=int(prec), batch=int(batch), graph=int(bool(graph)))
         J = C.c_double()
         self._run("hz_misfit_forward", nsrc, _ptr(f), rec.size, rec.ctypes.data_as(C.POINTER(C.c_int64)),
                   C.c_void_p(data.ctypes.data), C.byref(o), int(bool(store_fields)), C.byref(J))

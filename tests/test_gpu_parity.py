@@ -274,6 +274,51 @@ def test_fgmres_fp64_and_mixed_vs_oracle(hz, oracle):
         assert np.linalg.norm(r) / np.linalg.norm(f) < 1.5e-10
 
 
+@pytest.mark.parametrize("shape", [(20, 18, 22), (64, 14, 12)])
+def test_fgmres_lockstep_batch_matches_single(hz, oracle, shape):
+    """Five point sources advanced in lockstep (one batched preconditioner and
+    operator apply per Arnoldi step) give each source's single solve: same
+    iteration count and solution; the oracle's solve pins the first one.  The
+    64-wide case runs the multi-RHS streaming plane solve inside FGMRES."""
+    print("case lockstep %s" % (shape,), flush=True)
+    d, c = s7_case(hz, n=shape, npml=(8, 3, 3) if shape[0] == 64 else 4, smooth=True)
+    S = hz.Solver(d, c)
+    S.factor(hz.abi.PREC_FP64)
+    S.factor(hz.abi.PREC_C64)
+    nx, ny, nz = shape
+    pts = [(nx // 2, ny // 2, nz // 2), (nx // 3, ny // 2 - 1, nz // 2 + 2), (nx // 2 + 3, ny // 3, nz // 3 + 1),
+           (2 * nx // 3, 2 * ny // 3, nz // 2 - 2), (nx // 2 - 2, ny // 2 + 2, 2 * nz // 3)]
+    f = np.stack([hz.problem.point_source(d, p, amplitude=1.0) for p in pts])
+    for prec in (hz.abi.PREC_FP64, hz.abi.PREC_C64):
+        ub, stb = S.fgmres(_dev(f), m=25, restarts=20, tol=1e-9, prec=prec)
+        for q in range(len(pts)):
+            u1, st1 = S.fgmres(_dev(f[q]), m=25, restarts=20, tol=1e-9, prec=prec)
+            assert st1.converged and stb[q].converged
+            assert abs(st1.iters - stb[q].iters) <= 1, (prec, q, st1.iters, stb[q].iters)
+            assert (torch.linalg.vector_norm(u1 - ub[q]) / torch.linalg.vector_norm(u1)).item() < 1e-9
+    P = oracle.Problem(d, c)
+    u_o, st_o = P.fgmres(f[0], m=25, rest=20, tol=1e-9)
+    assert abs(st_o.iters - stb[0].iters) <= 1
+    assert _rel(_host(ub[0]), u_o) < 1e-7
+
+
+def test_fgmres_graph_replay_matches_eager(hz):
+    """Arnoldi steps replayed from captured HIP graphs over several restart
+    cycles (eager orthogonalisation and host syncs between the replays) give
+    the eager run's residual history and solution."""
+    d, c = s7_case(hz, n=(20, 18, 22), npml=4, smooth=True)
+    S = hz.Solver(d, c)
+    S.factor(hz.abi.PREC_C64)
+    f = np.stack([hz.problem.point_source(d, p, amplitude=1.0) for p in [(9, 8, 11), (5, 12, 7), (14, 6, 15)]])
+    ue, ste = S.fgmres(_dev(f), m=6, restarts=60, tol=1e-9, prec=hz.abi.PREC_C64)
+    ug, stg = S.fgmres(_dev(f), m=6, restarts=60, tol=1e-9, prec=hz.abi.PREC_C64, graph=True)
+    assert ste[0].restarts >= 3  # each step's graph is replayed at least twice
+    for q in range(3):
+        assert stg[q].converged and stg[q].iters == ste[q].iters
+    assert np.allclose(stg[0].hist, ste[0].hist, rtol=1e-12, atol=0)
+    assert (torch.linalg.vector_norm(ug - ue) / torch.linalg.vector_norm(ue)).item() < 1e-12
+
+
 @pytest.mark.parametrize("mode", ["ref", "s7"])
 def test_bicgstab_vs_oracle(hz, oracle, mode):
     d, c = ref_case(hz) if mode == "ref" else s7_case(hz, smooth=True)
@@ -312,6 +357,10 @@ def test_misfit_gradient_vs_oracle(hz, oracle):
         g = _host(S.misfit_backward())
         assert abs(J - J_o) < 1e-9 * abs(J_o)
         assert _rel(g, g_o) < 1e-7
+    # sources one at a time (batch 1) and in one lockstep batch agree
+    J1 = S.misfit_forward(_dev(f), rec, data, m=60, restarts=10, tol=1e-12, prec=hz.abi.PREC_FP64, batch=1)
+    g1 = _host(S.misfit_backward())
+    assert abs(J1 - J_o) < 1e-9 * abs(J_o) and _rel(g1, g_o) < 1e-7
     S.factor(hz.abi.PREC_C64)
     J = S.misfit_forward(_dev(f), rec, data, m=60, restarts=10, tol=1e-10, prec=hz.abi.PREC_C64)
     g = _host(S.misfit_backward())

@@ -194,6 +194,19 @@ int hz_fgmres(hz_problem* p, int nrhs, const hz_c128* f, hz_c128* u, const hz_kr
 int hz_bicgstab(hz_problem* p, int prec, int nrhs, const hz_c128* f, hz_c128* u, int maxit, double tol,
                 int tol_abs, int mask, double* hist, int hist_cap, hz_krylov_stats* stats);
 
+/* ------------------------------------------------------ several problems */
+/* Factor nprob problems (e.g. several frequencies on grids of different sizes)
+ * concurrently, each on its own stream.  No reference counterpart (the
+ * reference factors one frequency per run, FGMRES.cpp:176-298). */
+int hz_precond_factor_multi(int nprob, hz_problem* const* probs, int prec);
+/* hz_fgmres on nprob distinct problems at once: problem i solves nrhs[i]
+ * right-hand sides f[i] -> u[i] (device, nrhs[i]*N_i each) with stats[i] (host,
+ * nrhs[i]); every problem runs its lockstep batches on its own stream from its
+ * own host thread.  opts->hist, when given, records problem 0's first
+ * right-hand side.  No reference counterpart. */
+int hz_fgmres_multi(int nprob, hz_problem* const* probs, const int* nrhs, const hz_c128* const* f, hz_c128* const* u,
+                    const hz_krylov_opts* opts, hz_krylov_stats* const* stats);
+
 /* ------------------------------------------------------------- adjoint */
 /* Receiver misfit J = 1/2 sum_s ||P u_s - d_s||^2 (S7 mode) for nsrc sources
  * f (device, nsrc*N), receivers rec (host, nrec flat indices) and data d

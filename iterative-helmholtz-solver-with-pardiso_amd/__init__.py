@@ -8,3 +8,13 @@ from . import abi  # noqa: F401
 def Solver(*args, **kw):
     from .solver import Solver as _S
     return _S(*args, **kw)
+
+
+def factor_many(*args, **kw):
+    from .solver import factor_many as _f
+    return _f(*args, **kw)
+
+
+def fgmres_many(*args, **kw):
+    from .solver import fgmres_many as _f
+    return _f(*args, **kw)

@@ -79,6 +79,9 @@ SIGNATURES = {
     "hz_problem_stream": (_vp, [_vp]),
     "hz_problem_info": (_i, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "hz_precond_factor": (_i, [_vp, _i]),
+    "hz_precond_factor_multi": (_i, [_i, C.POINTER(C.c_void_p), _i]),
+    "hz_fgmres_multi": (_i, [_i, C.POINTER(C.c_void_p), C.POINTER(C.c_int), C.POINTER(C.c_void_p),
+                             C.POINTER(C.c_void_p), C.POINTER(KrylovOpts), C.POINTER(C.POINTER(KrylovStats))]),
     "hz_precond_apply": (_i, [_vp, _i, _i, _vp, _vp]),
     "hz_precond_apply_adjoint": (_i, [_vp, _i, _i, _vp, _vp]),
     "hz_L0_apply": (_i, [_vp, _i, _vp, _vp]),

@@ -6,7 +6,9 @@
 #include <complex>
 #include <cstdarg>
 #include <cstring>
+#include <functional>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "hz_internal.h"
@@ -882,6 +884,30 @@ int hz_fft_z(hz_problem* p, int prec, int sign, int nrhs, const hz_c128* in, hz_
   HZ_API_END
 }
 
+// ------------------------------------------------------- several problems
+// Each problem (one frequency on its own grid) is driven by its own host
+// thread on its own stream, so the GPU runs the problems' kernels side by side
+// (their plane counts add up to fill the CUs).  The first failure's code and
+// message are returned on the calling thread.
+static int run_concurrently(int nprob, const std::function<int(int)>& body) {
+  std::vector<int> rc(nprob, HZ_OK);
+  std::vector<std::string> msg(nprob);
+  std::vector<std::thread> th;
+  th.reserve(nprob);
+  for (int i = 0; i < nprob; ++i)
+    th.emplace_back([&, i] {
+      rc[i] = body(i);
+      if (rc[i] != HZ_OK) msg[i] = g_err;
+    });
+  for (auto& t : th) t.join();
+  for (int i = 0; i < nprob; ++i)
+    if (rc[i] != HZ_OK) {
+      set_error("problem %d: %s", i, msg[i].c_str());
+      return rc[i];
+    }
+  return HZ_OK;
+}
+
 // ------------------------------------------------------------- misfit
 // Sources run in lockstep batches (opts.batch, default min(nsrc, 16)); the
 // receiver restriction P and its transpose are device gathers/scatters.
@@ -897,6 +923,30 @@ static void misfit_solve_forward(hz_problem* p, int s0, int nb, double2* us) {
   for (int q = 0; q < nb; ++q)
     HZ_CHECK(st[q].converged, HZ_ERR_NOT_CONVERGED, "forward solve %d: relative residual %.3e", s0 + q,
              st[q].rel_res);
+}
+
+int hz_precond_factor_multi(int nprob, hz_problem* const* probs, int prec) {
+  HZ_API_BEGIN
+  HZ_CHECK(nprob >= 1 && probs, HZ_ERR_ARG, "bad arguments");
+  for (int i = 0; i < nprob; ++i) HZ_CHECK(probs[i], HZ_ERR_ARG, "null problem %d", i);
+  return run_concurrently(nprob, [&](int i) { return hz_precond_factor(probs[i], prec); });
+  HZ_API_END
+}
+
+int hz_fgmres_multi(int nprob, hz_problem* const* probs, const int* nrhs, const hz_c128* const* f, hz_c128* const* u,
+                    const hz_krylov_opts* opts, hz_krylov_stats* const* stats) {
+  HZ_API_BEGIN
+  HZ_CHECK(nprob >= 1 && probs && nrhs && f && u && opts && stats, HZ_ERR_ARG, "bad arguments");
+  for (int i = 0; i < nprob; ++i) HZ_CHECK(probs[i] && nrhs[i] >= 1 && f[i] && u[i] && stats[i], HZ_ERR_ARG,
+                                           "bad arguments for problem %d", i);
+  for (int i = 0; i < nprob; ++i)
+    for (int k = 0; k < i; ++k) HZ_CHECK(probs[i] != probs[k], HZ_ERR_ARG, "problem %d listed twice", i);
+  return run_concurrently(nprob, [&](int i) {
+    hz_krylov_opts o = *opts;
+    if (i > 0) { o.hist = nullptr; o.hist_cap = 0; }
+    return hz_fgmres(probs[i], nrhs[i], f[i], u[i], &o, stats[i]);
+  });
+  HZ_API_END
 }
 
 int hz_misfit_forward(hz_problem* p, int nsrc, const hz_c128* f, int nrec, const int64_t* rec, const hz_c128* data,

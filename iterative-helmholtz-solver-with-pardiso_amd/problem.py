@@ -228,7 +228,7 @@ def gaussian_random_field(shape_zyx, seed, corr_len, h):
     ky = 2.0 * math.pi * np.fft.fftfreq(ny, d=h[1])[None, :, None]
     kx = 2.0 * math.pi * np.fft.rfftfreq(nx, d=h[0])[None, None, :]
     W *= np.exp(-(kx * kx + ky * ky + kz * kz) * corr_len * corr_len / 4.0)
-    g = np.fft.irfftn(W, s=(nz, ny, nx))
+    g = np.fft.irfftn(W, s=(nz, ny, nx), axes=(0, 1, 2))
     g -= g.mean()
     g /= g.std()
     return g.reshape(-1)
@@ -261,4 +261,4 @@ def gaussian_smooth(field, desc, sigma):
     ky = 2.0 * math.pi * np.fft.fftfreq(ny, d=desc.h[1])[None, :, None]
     kx = 2.0 * math.pi * np.fft.rfftfreq(nx, d=desc.h[0])[None, None, :]
     F *= np.exp(-(kx * kx + ky * ky + kz * kz) * sigma * sigma / 2.0)
-    return np.fft.irfftn(F, s=(nz, ny, nx)).reshape(-1)
+    return np.fft.irfftn(F, s=(nz, ny, nx), axes=(0, 1, 2)).reshape(-1)

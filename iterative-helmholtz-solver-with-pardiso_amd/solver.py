@@ -202,3 +202,45 @@ class Solver:
         g = torch.empty(self.N, dtype=torch.float64, device="cuda")
         self._run("hz_misfit_backward", _ptr(g))
         return g
+
+
+# ------------------------------------------------------- several problems
+def factor_many(solvers, prec=PREC_C64):
+    """Factor several problems (frequencies / grids) concurrently."""
+    torch = _torch()
+    hs = (C.c_void_p * len(solvers))(*[S.h.value for S in solvers])
+    torch.cuda.synchronize()
+    abi.check(abi.lib().hz_precond_factor_multi(len(solvers), hs, int(prec)), "hz_precond_factor_multi")
+
+
+def fgmres_many(solvers, fs, m=30, restarts=20, tol=1e-8, prec=PREC_C64, ortho=ORTHO_MGS, mask=MASK_FULL,
+                batch=0, graph=False, hist_cap=8192):
+    """FGMRES on several problems at once: fs[i] (complex128 CUDA tensor of
+    shape (N_i,) or (nrhs_i, N_i)) is solved on solvers[i]; every problem runs
+    its lockstep batches on its own stream.  Returns [(u_i, [stats...])]."""
+    torch = _torch()
+    n = len(solvers)
+    fv, nr, us, sts = [], [], [], []
+    for S, f in zip(solvers, fs):
+        f, k = S._vec(f)
+        fv.append(f)
+        nr.append(k)
+        us.append(torch.zeros_like(f))
+        sts.append((abi.KrylovStats * k)())
+    hist = np.zeros(hist_cap)
+    o = abi.KrylovOpts(driver=DRIVER_FGMRES, m=int(m), max_restarts=int(restarts), ortho=int(ortho), mask=int(mask),
+                       monitor=0, tol=float(tol), breakdown=0.0, hist=hist.ctypes.data_as(C.POINTER(C.c_double)),
+                       hist_cap=hist_cap, prec=int(prec), batch=int(batch), graph=int(bool(graph)))
+    hs = (C.c_void_p * n)(*[S.h.value for S in solvers])
+    nra = (C.c_int * n)(*nr)
+    fa = (C.c_void_p * n)(*[f.data_ptr() for f in fv])
+    ua = (C.c_void_p * n)(*[u.data_ptr() for u in us])
+    sa = (C.POINTER(abi.KrylovStats) * n)(*[C.cast(st, C.POINTER(abi.KrylovStats)) for st in sts])
+    torch.cuda.synchronize()
+    rc = abi.lib().hz_fgmres_multi(n, hs, nra, fa, ua, C.byref(o), sa)
+    if rc not in (abi.HZ_OK, 5, 6):
+        abi.check(rc, "hz_fgmres_multi")
+    out = []
+    for i in range(n):
+        out.append((us[i], [_stats(sts[i][q], hist if (i == 0 and q == 0) else None) for q in range(nr[i])]))
+    return out

@@ -319,6 +319,28 @@ def test_fgmres_graph_replay_matches_eager(hz):
     assert (torch.linalg.vector_norm(ug - ue) / torch.linalg.vector_norm(ue)).item() < 1e-12
 
 
+def test_fgmres_many_problems_match_separate_solves(hz):
+    """Three frequencies on grids of different sizes, factored and solved
+    concurrently (one stream and host thread each, several right-hand sides per
+    grid), give the separate solves' iteration counts and solutions."""
+    pr = hz.problem
+    shapes = [((20, 18, 22), 2), ((64, 14, 12), 3), ((24, 20, 16), 2)]
+    solvers, fs = [], []
+    for i, (shape, k) in enumerate(shapes):
+        d, c = s7_case(hz, n=shape, npml=(8, 3, 3) if shape[0] == 64 else 4, seed=10 + i, smooth=True)
+        solvers.append(hz.Solver(d, c))
+        rng = np.random.default_rng(i)
+        pts = [tuple(int(rng.integers(4, n - 4)) for n in shape) for _ in range(k)]
+        fs.append(np.stack([pr.point_source(d, p, amplitude=1.0) for p in pts]))
+    hz.factor_many(solvers, hz.abi.PREC_C64)
+    res = hz.fgmres_many(solvers, [_dev(f) for f in fs], m=25, restarts=20, tol=1e-9, prec=hz.abi.PREC_C64)
+    for S, f, (u, st) in zip(solvers, fs, res):
+        u1, st1 = S.fgmres(_dev(f), m=25, restarts=20, tol=1e-9, prec=hz.abi.PREC_C64)
+        for q in range(f.shape[0]):
+            assert st[q].converged and st[q].iters == st1[q].iters
+        assert (torch.linalg.vector_norm(u - u1) / torch.linalg.vector_norm(u1)).item() < 1e-12
+
+
 @pytest.mark.parametrize("mode", ["ref", "s7"])
 def test_bicgstab_vs_oracle(hz, oracle, mode):
     d, c = ref_case(hz) if mode == "ref" else s7_case(hz, smooth=True)

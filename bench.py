@@ -95,10 +95,14 @@ def main():
 
     def solve(prec):
         S.factor(prec)
-        u, st = S.fgmres(f, m=a.m, restarts=max(1, a.max_iters // a.m), tol=a.tol, prec=prec)
+        u, st = S.fgmres(f, m=a.m, restarts=max(1, a.max_iters // a.m), tol=a.tol, prec=prec, autotune=True)
         return u, st
 
     note("grid %d^3, omega %.2f, pml_c %.1f" % (n, omega, pml_c))
+    # launch choices for one right-hand side (not timed; the solves reuse them)
+    S.factor(A.PREC_C64)
+    tuning = S.autotune(A.PREC_C64, 1)
+    note("tuned: %s" % tuning)
     for w in range(a.warmup):
         _, stw = solve(A.PREC_C64)
         note("warmup %d: %d iterations" % (w, stw.iters))
@@ -112,7 +116,8 @@ def main():
         S.factor(A.PREC_C64)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        u32, st32 = S.fgmres(f, m=a.m, restarts=max(1, a.max_iters // a.m), tol=a.tol, prec=A.PREC_C64)
+        u32, st32 = S.fgmres(f, m=a.m, restarts=max(1, a.max_iters // a.m), tol=a.tol, prec=A.PREC_C64,
+                             autotune=True)
         torch.cuda.synchronize()
         t2 = time.perf_counter()
         times.append(t2 - t0)
@@ -161,6 +166,7 @@ def main():
         "iters_mixed": st32.iters,
         "rel_res_mixed": st32.rel_res,
         "factor_c64_s": round(float(np.mean(t_fac)), 4),
+        "tuning": tuning,
         "roofline": {
             "kernel": "%s (block-tridiagonal plane solve, complex64)" % plane_solve_kernel(nx),
             "bound": "hbm",
@@ -229,6 +235,11 @@ def bench_batched(a, hz, np, torch):
         fs.append(torch.from_numpy(f).cuda())
         info.append({"grid": n, "npml": npml, "omega": omega})
         note("grid %d^3 omega %.2f" % (n, omega))
+    # launch choices per grid, each tuned alone on the idle GPU (not timed)
+    hz.factor_many(solvers, A.PREC_C64)
+    for S, inf in zip(solvers, info):
+        inf["tuning"] = S.autotune(A.PREC_C64, nsrc)
+        note("grid %d^3 tuned: %s" % (inf["grid"], inf["tuning"]))
     for w in range(a.warmup):
         hz.factor_many(solvers, A.PREC_C64)
         hz.fgmres_many(solvers, fs, m=a.m, restarts=max(1, a.max_iters // a.m), tol=a.tol, prec=A.PREC_C64,

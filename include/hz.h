@@ -110,6 +110,8 @@ typedef struct {
                            preconditioner and operator apply (0: all of them, at most 16) */
   int32_t graph;        /* FGMRES: 1 = replay each Arnoldi step's preconditioner + operator
                            launches from a captured HIP graph */
+  int32_t autotune;     /* FGMRES: 1 = run hz_autotune for (prec, batch size) before the first step */
+  int32_t reserved;
 } hz_krylov_opts;
 
 typedef struct {
@@ -124,6 +126,21 @@ typedef struct {
   int32_t reserved;
   double t_solve_s;  /* wall time of the Krylov loop */
 } hz_krylov_stats;
+
+/* Launch choices picked by hz_autotune for one (precision, right-hand-side count). */
+enum {
+  HZ_SOLVE_DEFAULT = 0, /* streaming multi-RHS solve where instantiated, else chunked */
+  HZ_SOLVE_STREAM = 1,  /* factor tiles streamed through an LDS ring, up to 8 RHS per pass */
+  HZ_SOLVE_CHUNKED = 2  /* chunked GEMV kernel, any width */
+};
+typedef struct {
+  int32_t solve_kernel; /* HZ_SOLVE_* */
+  int32_t group_cap;    /* right-hand sides per streaming pass: 8, 4, 2 or 1 */
+  int32_t fft_lines;    /* (x,y) lines per z-FFT workgroup: 8, 16 or 32 */
+  int32_t from_cache;   /* 1 when read from the HZ_TUNE_CACHE file */
+  double solve_ms;      /* plane solve of nrhs vectors with the choice */
+  double fft_ms;        /* one forward z-FFT of nrhs vectors with the choice */
+} hz_tune_info;
 
 /* ---------------------------------------------------------------- library */
 const char* hz_last_error(void);
@@ -193,6 +210,15 @@ int hz_fgmres(hz_problem* p, int nrhs, const hz_c128* f, hz_c128* u, const hz_kr
  * right-hand side.  Replaces BCGSTAB (templates.h:60; BcGSTAB.cpp:6-666). */
 int hz_bicgstab(hz_problem* p, int prec, int nrhs, const hz_c128* f, hz_c128* u, int maxit, double tol,
                 int tol_abs, int mask, double* hist, int hist_cap, hz_krylov_stats* stats);
+
+/* Time the launch variants of the preconditioner for nrhs right-hand sides
+ * (plane-solve kernel family and right-hand-side group size, z-FFT lines per
+ * workgroup) on this problem's factors and keep the fastest for later calls
+ * with that precision and count.  Needs hz_precond_factor(prec).  With the
+ * environment variable HZ_TUNE_CACHE naming a file, choices are read from and
+ * appended to it (key: grid, precision, nrhs).  info (optional) receives the
+ * choice.  No reference counterpart (PARDISO's iparm defaults are fixed). */
+int hz_autotune(hz_problem* p, int prec, int nrhs, hz_tune_info* info);
 
 /* ------------------------------------------------------ several problems */
 /* Factor nprob problems (e.g. several frequencies on grids of different sizes)

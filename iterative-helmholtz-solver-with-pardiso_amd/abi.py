@@ -10,6 +10,7 @@ HZ_OK = 0
 ERRORS = {1: "HZ_ERR_ARG", 2: "HZ_ERR_HIP", 3: "HZ_ERR_STATE", 4: "HZ_ERR_NOMEM", 5: "HZ_ERR_NOT_CONVERGED",
           6: "HZ_ERR_BREAKDOWN", 7: "HZ_ERR_SINGULAR", 8: "HZ_ERR_NONFINITE"}
 PREC_FP64, PREC_C64 = 0, 1
+SOLVE_DEFAULT, SOLVE_STREAM, SOLVE_CHUNKED = 0, 1, 2
 DRIVER_FGMRES, DRIVER_GMRES_REF = 0, 1
 ORTHO_REF_CONJ, ORTHO_MGS, ORTHO_MGS_SEQ = 0, 1, 2
 MASK_FULL, MASK_PHYS = 0, 1
@@ -48,6 +49,19 @@ class KrylovOpts(C.Structure):
         ("prec", C.c_int32),
         ("batch", C.c_int32),
         ("graph", C.c_int32),
+        ("autotune", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class TuneInfo(C.Structure):
+    _fields_ = [
+        ("solve_kernel", C.c_int32),
+        ("group_cap", C.c_int32),
+        ("fft_lines", C.c_int32),
+        ("from_cache", C.c_int32),
+        ("solve_ms", C.c_double),
+        ("fft_ms", C.c_double),
     ]
 
 
@@ -80,6 +94,7 @@ SIGNATURES = {
     "hz_problem_info": (_i, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "hz_precond_factor": (_i, [_vp, _i]),
     "hz_precond_factor_multi": (_i, [_i, C.POINTER(C.c_void_p), _i]),
+    "hz_autotune": (_i, [_vp, _i, _i, C.POINTER(TuneInfo)]),
     "hz_fgmres_multi": (_i, [_i, C.POINTER(C.c_void_p), C.POINTER(C.c_int), C.POINTER(C.c_void_p),
                              C.POINTER(C.c_void_p), C.POINTER(KrylovOpts), C.POINTER(C.POINTER(KrylovStats))]),
     "hz_precond_apply": (_i, [_vp, _i, _i, _vp, _vp]),

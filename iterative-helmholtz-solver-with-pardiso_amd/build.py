@@ -17,6 +17,7 @@ SOURCES = [
     "hz_setup.cpp",
     "hz_abi.cpp",
     "hz_ref_shim.cpp",
+    "hz_tune.cpp",
     "k_fft.hip",
     "k_stencil.hip",
     "k_blocktri.hip",

@@ -297,6 +297,7 @@ static void fgmres_lockstep(hz_problem* p, int R, const double2* f, double2* x, 
         if (e) (void)hipGraphExecDestroy(e);
     }
   } graphs;
+  if (o.autotune && !p->tuned.count({o.prec, R})) autotune(p, o.prec, R, nullptr);
   if (o.graph) {
     graphs.exec.assign(m, nullptr);
     graphs.uses.assign(m, 0);
@@ -923,6 +924,13 @@ static void misfit_solve_forward(hz_problem* p, int s0, int nb, double2* us) {
   for (int q = 0; q < nb; ++q)
     HZ_CHECK(st[q].converged, HZ_ERR_NOT_CONVERGED, "forward solve %d: relative residual %.3e", s0 + q,
              st[q].rel_res);
+}
+
+int hz_autotune(hz_problem* p, int prec, int nrhs, hz_tune_info* info) {
+  HZ_API_BEGIN
+  HZ_CHECK(p, HZ_ERR_ARG, "null problem");
+  autotune(p, prec, nrhs, info);
+  HZ_API_END
 }
 
 int hz_precond_factor_multi(int nprob, hz_problem* const* probs, int prec) {

@@ -5,7 +5,9 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/hz.h"
@@ -55,6 +57,14 @@ struct FftPlan {
   DevBuf tw32;  // float2[n]
 };
 
+// Launch choices of one (precision, right-hand-side count), set by the
+// autotuner; the defaults are used until it has run.
+struct Tuning {
+  int solve_family = 0;  // HZ_SOLVE_DEFAULT / _STREAM / _CHUNKED
+  int group_cap = 8;     // largest right-hand-side group of the streaming solve (8, 4, 2, 1)
+  int fft_lines = 16;    // (x,y) lines per z-FFT workgroup (8, 16, 32)
+};
+
 // Plane factors of one precision.
 struct Factors {
   bool ready = false;
@@ -94,6 +104,22 @@ struct hz_problem {
   hz::DevBuf red;              // reduction partials
   hz::DevBuf scal;             // device scalars (double2)
 
+  // autotuned launch choices per (precision, right-hand sides)
+  std::map<std::pair<int, int>, hz::Tuning> tuned;
+  hz::Tuning tuning(int prec, int nrhs) const {
+    auto it = tuned.find({prec, nrhs});
+    if (it != tuned.end()) return it->second;
+    // a tuned count with the same group structure serves larger batches too
+    hz::Tuning best;
+    int best_n = 0;
+    for (const auto& kv : tuned)
+      if (kv.first.first == prec && kv.first.second <= nrhs && kv.first.second > best_n) {
+        best = kv.second;
+        best_n = kv.first.second;
+      }
+    return best;
+  }
+
   // Krylov storage
   hz::DevBuf kV, kZ, kx, kr, kw, kaux;
   int k_m = 0, k_nrhs = 0;
@@ -123,6 +149,9 @@ struct hz_problem {
 
 namespace hz {
 
+// ------------------------------------------------------------ autotuner
+void autotune(hz_problem* p, int prec, int nrhs, hz_tune_info* info);
+
 // ------------------------------------------------------------ setup (host)
 void setup_tables(hz_problem* p, const double* c_host);
 
@@ -133,6 +162,11 @@ void setup_tables(hz_problem* p, const double* c_host);
 // with w (double2) and dL (double2) read at the same index.
 void launch_fft_z(hz_problem* p, int prec, int sign, int nrhs, const void* in, int in_prec, void* out,
                   int out_prec, int epi, const double2* w, const double2* dL, hipStream_t s);
+// the same with an explicit number of lines per workgroup (8, 16, 32)
+void launch_fft_z_lines(hz_problem* p, int lines, int prec, int sign, int nrhs, const void* in, int in_prec,
+                        void* out, int out_prec, int epi, const double2* w, const double2* dL, hipStream_t s);
+// whether `lines` lines of length nz fit the LDS-resident FFT in precision prec
+bool fft_lines_fit(int prec, int nz, int lines);
 
 // S7 stencil y = A u (adjoint: A^H u); epi 1: y = f - A u.
 void launch_stencil(hz_problem* p, int nrhs, const double2* u, double2* y, bool adjoint, const double2* f,
@@ -148,6 +182,11 @@ void set_max_lds(const void* fn, size_t bytes);
 void launch_factor(hz_problem* p, int prec, hipStream_t s);
 // in-place plane solves on z-transformed data (prec 0: double2, 1: float2).
 void launch_plane_solve(hz_problem* p, int prec, int nrhs, void* x, bool adjoint, hipStream_t s);
+// the same with explicit launch choices (autotuner)
+void launch_plane_solve_with(hz_problem* p, int prec, int nrhs, void* x, bool adjoint, const Tuning& t,
+                             hipStream_t s);
+// whether the streaming multi-RHS solve is instantiated for this width/precision
+bool stream_solve_available(int prec, int nx);
 
 // ---- vector kernels (FP64 complex)
 int reduce_blocks(int64_t n);

@@ -1070,33 +1070,30 @@ void launch_mr(int ny, int nz, int groups, hipStream_t s, const void* G, void* x
   HZ_HIP(hipGetLastError());
 }
 
-// All nrhs vectors: as many full groups of NRMAX right-hand sides as fit, then
-// the remainder in groups of 4, 2 and 1 (each smaller than NRMAX); returns
-// false when NX has no instantiation.
+// All nrhs vectors: as many full groups of the cap (<= NRMAX) as fit, then
+// the remainder in groups of 4, 2 and 1; returns false when NX has no
+// instantiation.
 template <typename T, int RPL, int NCG, int TCOL, int NX, int NRMAX>
-bool try_launch_mr(int n, int ny, int nz, int nrhs, hipStream_t s, const void* G, void* x, const double2* cmy,
-                   const double2* cpy, int64_t n2, int64_t N) {
+bool try_launch_mr(int n, int ny, int nz, int nrhs, int cap, hipStream_t s, const void* G, void* x,
+                   const double2* cmy, const double2* cpy, int64_t n2, int64_t N) {
   if (n != NX) return false;
   typedef typename Cx<T>::t C;
   C* xp = (C*)x;
   int done = 0;
-  const int full = nrhs / NRMAX;
-  if (full > 0) {
-    launch_mr<T, RPL, NCG, TCOL, NX, NRMAX>(ny, nz, full, s, G, xp, cmy, cpy, n2, N);
-    done = full * NRMAX;
-  }
-  if (NRMAX > 4 && nrhs - done >= 4) {
-    launch_mr<T, RPL, NCG, TCOL, NX, (NRMAX > 4 ? 4 : 1)>(ny, nz, 1, s, G, xp + done * N, cmy, cpy, n2, N);
-    done += 4;
-  }
-  if (NRMAX > 2 && nrhs - done >= 2) {
-    launch_mr<T, RPL, NCG, TCOL, NX, (NRMAX > 2 ? 2 : 1)>(ny, nz, 1, s, G, xp + done * N, cmy, cpy, n2, N);
-    done += 2;
-  }
-  if (NRMAX > 1 && nrhs - done >= 1) {
-    launch_mr<T, RPL, NCG, TCOL, NX, 1>(ny, nz, 1, s, G, xp + done * N, cmy, cpy, n2, N);
-    done += 1;
-  }
+  auto groups = [&](int g, int count) {
+    switch (g) {
+      case 8: launch_mr<T, RPL, NCG, TCOL, NX, (NRMAX >= 8 ? 8 : 1)>(ny, nz, count, s, G, xp + done * N, cmy, cpy, n2, N); break;
+      case 4: launch_mr<T, RPL, NCG, TCOL, NX, (NRMAX >= 4 ? 4 : 1)>(ny, nz, count, s, G, xp + done * N, cmy, cpy, n2, N); break;
+      case 2: launch_mr<T, RPL, NCG, TCOL, NX, 2>(ny, nz, count, s, G, xp + done * N, cmy, cpy, n2, N); break;
+      default: launch_mr<T, RPL, NCG, TCOL, NX, 1>(ny, nz, count, s, G, xp + done * N, cmy, cpy, n2, N); break;
+    }
+    done += g * count;
+  };
+  int g = 1;
+  while (g * 2 <= std::min(cap, NRMAX)) g *= 2;  // largest power of two <= cap
+  if (nrhs / g > 0) groups(g, nrhs / g);
+  for (int r = g / 2; r >= 1; r /= 2)
+    if (nrhs - done >= r) groups(r, 1);
   HZ_CHECK(done == nrhs, HZ_ERR_STATE, "plane solve covered %d of %d right-hand sides", done, nrhs);
   return true;
 }
@@ -1141,23 +1138,42 @@ SolveKernel solve_kernel_choice() {
 }
 
 // The multi-RHS kernel for the grid widths it is instantiated for.
-bool launch_mr_any(int prec, int n, int ny, int nz, int nrhs, hipStream_t s, const void* G, void* x,
+#define HZ_MR_WIDTHS(X)                                                                      \
+  X(HZ_PREC_C64, float, 2, 4, 20, 200, 8)                                                     \
+  X(HZ_PREC_C64, float, 2, 4, 16, 192, 8)                                                     \
+  X(HZ_PREC_C64, float, 2, 4, 20, 160, 8)                                                     \
+  X(HZ_PREC_C64, float, 2, 8, 32, 128, 8)                                                     \
+  X(HZ_PREC_C64, float, 2, 8, 32, 96, 8)                                                      \
+  X(HZ_PREC_C64, float, 2, 16, 32, 64, 8)                                                     \
+  X(HZ_PREC_FP64, double, 1, 2, 10, 200, 4)                                                   \
+  X(HZ_PREC_FP64, double, 1, 2, 8, 192, 4)                                                    \
+  X(HZ_PREC_FP64, double, 1, 2, 10, 160, 4)                                                   \
+  X(HZ_PREC_FP64, double, 1, 4, 16, 128, 4)                                                   \
+  X(HZ_PREC_FP64, double, 1, 4, 16, 96, 4)                                                    \
+  X(HZ_PREC_FP64, double, 1, 8, 32, 64, 4)
+
+bool launch_mr_any(int prec, int n, int ny, int nz, int nrhs, int cap, hipStream_t s, const void* G, void* x,
                    const double2* cmy, const double2* cpy, int64_t n2, int64_t N) {
   if (reinterpret_cast<uintptr_t>(x) % 16 != 0) return false;
-  if (prec == HZ_PREC_C64)
-    return try_launch_mr<float, 2, 4, 20, 200, 8>(n, ny, nz, nrhs, s, G, x, cmy, cpy, n2, N) ||
-           try_launch_mr<float, 2, 4, 16, 192, 8>(n, ny, nz, nrhs, s, G, x, cmy, cpy, n2, N) ||
-           try_launch_mr<float, 2, 4, 20, 160, 8>(n, ny, nz, nrhs, s, G, x, cmy, cpy, n2, N) ||
-           try_launch_mr<float, 2, 8, 32, 128, 8>(n, ny, nz, nrhs, s, G, x, cmy, cpy, n2, N) ||
-           try_launch_mr<float, 2, 8, 32, 96, 8>(n, ny, nz, nrhs, s, G, x, cmy, cpy, n2, N) ||
-           try_launch_mr<float, 2, 16, 32, 64, 8>(n, ny, nz, nrhs, s, G, x, cmy, cpy, n2, N);
-  return try_launch_mr<double, 1, 2, 10, 200, 4>(n, ny, nz, nrhs, s, G, x, cmy, cpy, n2, N) ||
-         try_launch_mr<double, 1, 2, 8, 192, 4>(n, ny, nz, nrhs, s, G, x, cmy, cpy, n2, N) ||
-         try_launch_mr<double, 1, 2, 10, 160, 4>(n, ny, nz, nrhs, s, G, x, cmy, cpy, n2, N) ||
-         try_launch_mr<double, 1, 4, 16, 128, 4>(n, ny, nz, nrhs, s, G, x, cmy, cpy, n2, N) ||
-         try_launch_mr<double, 1, 4, 16, 96, 4>(n, ny, nz, nrhs, s, G, x, cmy, cpy, n2, N) ||
-         try_launch_mr<double, 1, 8, 32, 64, 4>(n, ny, nz, nrhs, s, G, x, cmy, cpy, n2, N);
+#define HZ_MR_TRY(P, T, RPL, NCG, TCOL, NX, NRMAX)                                                        \
+  if (prec == P && try_launch_mr<T, RPL, NCG, TCOL, NX, NRMAX>(n, ny, nz, nrhs, cap, s, G, x, cmy, cpy, n2, N)) \
+    return true;
+  HZ_MR_WIDTHS(HZ_MR_TRY)
+#undef HZ_MR_TRY
+  return false;
 }
+
+}  // namespace
+
+bool stream_solve_available(int prec, int nx) {
+#define HZ_MR_HAS(P, T, RPL, NCG, TCOL, NX, NRMAX) \
+  if (prec == P && nx == NX) return true;
+  HZ_MR_WIDTHS(HZ_MR_HAS)
+#undef HZ_MR_HAS
+  return false;
+}
+
+namespace {
 
 PlaneTabs plane_tabs(hz_problem* p) {
   PlaneTabs t;
@@ -1217,6 +1233,11 @@ void launch_factor(hz_problem* p, int prec, hipStream_t s) {
 }
 
 void launch_plane_solve(hz_problem* p, int prec, int nrhs, void* x, bool adjoint, hipStream_t s) {
+  launch_plane_solve_with(p, prec, nrhs, x, adjoint, p->tuning(prec, nrhs), s);
+}
+
+void launch_plane_solve_with(hz_problem* p, int prec, int nrhs, void* x, bool adjoint, const Tuning& t,
+                             hipStream_t s) {
   Factors& f = p->fac[prec];
   HZ_CHECK(f.ready, HZ_ERR_STATE, "preconditioner of precision %d not factored", prec);
   const int n = (int)p->nx, ny = (int)p->ny;
@@ -1225,9 +1246,11 @@ void launch_plane_solve(hz_problem* p, int prec, int nrhs, void* x, bool adjoint
   const double2* cmy = p->tab(1, 0);
   const double2* cpy = p->tab(1, 1);
   dim3 grid((unsigned)p->nz, (unsigned)nrhs);
-  const SolveKernel sk = solve_kernel_choice();
+  SolveKernel sk = solve_kernel_choice();
+  if (sk == SK_DEFAULT && t.solve_family == HZ_SOLVE_STREAM) sk = SK_MR;
+  if (sk == SK_DEFAULT && t.solve_family == HZ_SOLVE_CHUNKED) sk = SK_FAST;
   if (!adjoint && ny >= 1 && (sk == SK_DEFAULT || sk == SK_MR) &&
-      launch_mr_any(prec, n, ny, (int)p->nz, nrhs, s, f.G.ptr, x, cmy, cpy, p->n2, p->N))
+      launch_mr_any(prec, n, ny, (int)p->nz, nrhs, t.group_cap, s, f.G.ptr, x, cmy, cpy, p->n2, p->N))
     return;
   if (!adjoint && ny >= 1 && sk == SK_DMA) {
     // LDS-DMA streaming kernel for the configurations it is instantiated for

@@ -160,10 +160,9 @@ __global__ __launch_bounds__(256) void k_fft_z(const TI* __restrict__ in, TO* __
   }
 }
 
-template <typename T, typename TI, typename TO, int EPI>
-void launch_typed(hz_problem* p, int sign, int nrhs, const void* in, void* out, const double2* w,
+template <typename T, typename TI, typename TO, int EPI, int LINES>
+void launch_lines(hz_problem* p, int sign, int nrhs, const void* in, void* out, const double2* w,
                   const double2* dL, hipStream_t s) {
-  constexpr int LINES = 16;
   FftArgs a;
   a.n = p->fft.n;
   a.nstages = p->fft.nstages;
@@ -181,23 +180,45 @@ void launch_typed(hz_problem* p, int sign, int nrhs, const void* in, void* out, 
   HZ_HIP(hipGetLastError());
 }
 
+template <typename T, typename TI, typename TO, int EPI>
+void launch_typed(hz_problem* p, int lines, int sign, int nrhs, const void* in, void* out, const double2* w,
+                  const double2* dL, hipStream_t s) {
+  if (lines == 32 && fft_lines_fit(sizeof(T) == 8 ? HZ_PREC_FP64 : HZ_PREC_C64, p->fft.n, 32))
+    launch_lines<T, TI, TO, EPI, 32>(p, sign, nrhs, in, out, w, dL, s);
+  else if (lines == 8)
+    launch_lines<T, TI, TO, EPI, 8>(p, sign, nrhs, in, out, w, dL, s);
+  else
+    launch_lines<T, TI, TO, EPI, 16>(p, sign, nrhs, in, out, w, dL, s);
+}
+
 }  // namespace
+
+bool fft_lines_fit(int prec, int nz, int lines) {
+  const size_t c = prec == HZ_PREC_FP64 ? 16 : 8;
+  return 2 * (size_t)lines * nz * c <= 160 * 1024;
+}
 
 void launch_fft_z(hz_problem* p, int prec, int sign, int nrhs, const void* in, int in_prec, void* out,
                   int out_prec, int epi, const double2* w, const double2* dL, hipStream_t s) {
+  launch_fft_z_lines(p, p->tuning(prec, nrhs).fft_lines, prec, sign, nrhs, in, in_prec, out, out_prec, epi, w, dL,
+                     s);
+}
+
+void launch_fft_z_lines(hz_problem* p, int lines, int prec, int sign, int nrhs, const void* in, int in_prec,
+                        void* out, int out_prec, int epi, const double2* w, const double2* dL, hipStream_t s) {
   HZ_CHECK(!(epi && out_prec != 0), HZ_ERR_ARG, "fft epilogue needs a complex128 output");
   if (prec == HZ_PREC_FP64) {
     HZ_CHECK(in_prec == 0 && out_prec == 0, HZ_ERR_ARG, "FP64 FFT takes complex128 in and out");
-    if (epi) launch_typed<double, double2, double2, 1>(p, sign, nrhs, in, out, w, dL, s);
-    else launch_typed<double, double2, double2, 0>(p, sign, nrhs, in, out, w, dL, s);
+    if (epi) launch_typed<double, double2, double2, 1>(p, lines, sign, nrhs, in, out, w, dL, s);
+    else launch_typed<double, double2, double2, 0>(p, lines, sign, nrhs, in, out, w, dL, s);
     return;
   }
-  if (in_prec == 0 && out_prec == 1) launch_typed<float, double2, float2, 0>(p, sign, nrhs, in, out, w, dL, s);
+  if (in_prec == 0 && out_prec == 1) launch_typed<float, double2, float2, 0>(p, lines, sign, nrhs, in, out, w, dL, s);
   else if (in_prec == 1 && out_prec == 0) {
-    if (epi) launch_typed<float, float2, double2, 1>(p, sign, nrhs, in, out, w, dL, s);
-    else launch_typed<float, float2, double2, 0>(p, sign, nrhs, in, out, w, dL, s);
-  } else if (in_prec == 1 && out_prec == 1) launch_typed<float, float2, float2, 0>(p, sign, nrhs, in, out, w, dL, s);
-  else launch_typed<float, double2, double2, 0>(p, sign, nrhs, in, out, w, dL, s);
+    if (epi) launch_typed<float, float2, double2, 1>(p, lines, sign, nrhs, in, out, w, dL, s);
+    else launch_typed<float, float2, double2, 0>(p, lines, sign, nrhs, in, out, w, dL, s);
+  } else if (in_prec == 1 && out_prec == 1) launch_typed<float, float2, float2, 0>(p, lines, sign, nrhs, in, out, w, dL, s);
+  else launch_typed<float, double2, double2, 0>(p, lines, sign, nrhs, in, out, w, dL, s);
 }
 
 }  // namespace hz

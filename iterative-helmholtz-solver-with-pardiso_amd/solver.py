@@ -100,6 +100,14 @@ class Solver:
     def factor(self, prec=PREC_FP64):
         self._run("hz_precond_factor", int(prec))
 
+    def autotune(self, prec=PREC_C64, nrhs=1):
+        """Time the preconditioner's launch variants for nrhs right-hand sides
+        and keep the fastest; returns the choice."""
+        info = abi.TuneInfo()
+        self._run("hz_autotune", int(prec), int(nrhs), C.byref(info))
+        return dict(solve_kernel=info.solve_kernel, group_cap=info.group_cap, fft_lines=info.fft_lines,
+                    from_cache=bool(info.from_cache), solve_ms=info.solve_ms, fft_ms=info.fft_ms)
+
     def precond_apply(self, x, prec=PREC_FP64, adjoint=False):
         x, nr = self._vec(x)
         out = _torch().empty_like(x)
@@ -151,7 +159,8 @@ class Solver:
 
     # ----------------------------------------------------------- drivers
     def fgmres(self, f, m=30, restarts=20, tol=1e-8, driver=DRIVER_FGMRES, prec=PREC_FP64, ortho=ORTHO_MGS,
-               mask=MASK_FULL, monitor=0, breakdown=0.0, hist_cap=8192, check=False, batch=0, graph=False):
+               mask=MASK_FULL, monitor=0, breakdown=0.0, hist_cap=8192, check=False, batch=0, graph=False,
+               autotune=False):
         """Solve for every row of f (shape (N,) or (nrhs, N)); FGMRES advances
         `batch` right-hand sides in lockstep (0: up to 16), `graph` replays each
         Arnoldi step's launches from a captured HIP graph."""
@@ -161,7 +170,7 @@ class Solver:
         o = abi.KrylovOpts(driver=int(driver), m=int(m), max_restarts=int(restarts), ortho=int(ortho),
                            mask=int(mask), monitor=int(monitor), tol=float(tol), breakdown=float(breakdown),
                            hist=hist.ctypes.data_as(C.POINTER(C.c_double)), hist_cap=hist_cap, prec=int(prec),
-                           batch=int(batch), graph=int(bool(graph)))
+                           batch=int(batch), graph=int(bool(graph)), autotune=int(bool(autotune)))
         st = (abi.KrylovStats * nr)()
         _torch().cuda.synchronize()
         rc = abi.lib().hz_fgmres(self.h, nr, _ptr(f), _ptr(u), C.byref(o), st)
@@ -214,7 +223,7 @@ def factor_many(solvers, prec=PREC_C64):
 
 
 def fgmres_many(solvers, fs, m=30, restarts=20, tol=1e-8, prec=PREC_C64, ortho=ORTHO_MGS, mask=MASK_FULL,
-                batch=0, graph=False, hist_cap=8192):
+                batch=0, graph=False, autotune=False, hist_cap=8192):
     """FGMRES on several problems at once: fs[i] (complex128 CUDA tensor of
     shape (N_i,) or (nrhs_i, N_i)) is solved on solvers[i]; every problem runs
     its lockstep batches on its own stream.  Returns [(u_i, [stats...])]."""
@@ -230,7 +239,8 @@ def fgmres_many(solvers, fs, m=30, restarts=20, tol=1e-8, prec=PREC_C64, ortho=O
     hist = np.zeros(hist_cap)
     o = abi.KrylovOpts(driver=DRIVER_FGMRES, m=int(m), max_restarts=int(restarts), ortho=int(ortho), mask=int(mask),
                        monitor=0, tol=float(tol), breakdown=0.0, hist=hist.ctypes.data_as(C.POINTER(C.c_double)),
-                       hist_cap=hist_cap, prec=int(prec), batch=int(batch), graph=int(bool(graph)))
+                       hist_cap=hist_cap, prec=int(prec), batch=int(batch), graph=int(bool(graph)),
+                       autotune=int(bool(autotune)))
     hs = (C.c_void_p * n)(*[S.h.value for S in solvers])
     nra = (C.c_int * n)(*nr)
     fa = (C.c_void_p * n)(*[f.data_ptr() for f in fv])

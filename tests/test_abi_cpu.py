@@ -46,8 +46,10 @@ def test_struct_layouts_match_header(tmp_path):
 #include <stdio.h>
 #include "hz.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(hz_desc), sizeof(hz_krylov_opts), sizeof(hz_krylov_stats),
-         offsetof(hz_desc, c_bg), offsetof(hz_krylov_opts, hist), offsetof(hz_krylov_stats, t_solve_s));
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(hz_desc), sizeof(hz_krylov_opts),
+         sizeof(hz_krylov_stats), offsetof(hz_desc, c_bg), offsetof(hz_krylov_opts, hist),
+         offsetof(hz_krylov_stats, t_solve_s), offsetof(hz_krylov_opts, batch), offsetof(hz_krylov_opts, autotune),
+         sizeof(hz_tune_info), offsetof(hz_tune_info, fft_ms));
   return 0;
 }
 """)
@@ -56,7 +58,9 @@ int main(void) {
     got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
     A = hz.abi
     want = [ctypes.sizeof(A.HzDesc), ctypes.sizeof(A.KrylovOpts), ctypes.sizeof(A.KrylovStats),
-            A.HzDesc.c_bg.offset, A.KrylovOpts.hist.offset, A.KrylovStats.t_solve_s.offset]
+            A.HzDesc.c_bg.offset, A.KrylovOpts.hist.offset, A.KrylovStats.t_solve_s.offset,
+            A.KrylovOpts.batch.offset, A.KrylovOpts.autotune.offset, ctypes.sizeof(A.TuneInfo),
+            A.TuneInfo.fft_ms.offset]
     assert got == want
 
 

@@ -188,6 +188,36 @@ def test_precond_wide_planes_multi_rhs_vs_oracle(hz, oracle, nx):
             assert e < tol, (prec, q, e)
 
 
+def test_autotune_choices_and_cache(hz, tmp_path, monkeypatch):
+    """The autotuner picks a valid launch configuration per (precision, RHS
+    count), the tuned preconditioner gives the untuned one's result, and the
+    cache file returns the same choice for a new problem of the same grid."""
+    pr = hz.problem
+    d = pr.cube_geometry((128, 12, 8), (8, 2, 2), 40.0, beta=0.5, pml_c=30.0)
+    c = 0.8 + 0.4 * np.random.default_rng(4).random(d.N)
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("HZ_TUNE_CACHE", str(cache))
+    S = hz.Solver(d, c)
+    x = _dev(rand_vec(d.N, 77, 8))
+    for prec, tol in ((hz.abi.PREC_C64, 1e-6), (hz.abi.PREC_FP64, 1e-13)):
+        S.factor(prec)
+        y0 = S.precond_apply(x, prec)
+        t = S.autotune(prec, 8)
+        print("tuned prec %d: %s" % (prec, t))
+        assert t["solve_kernel"] in (hz.abi.SOLVE_STREAM, hz.abi.SOLVE_CHUNKED) and not t["from_cache"]
+        assert t["group_cap"] in (1, 2, 4, 8) and t["fft_lines"] in (8, 16, 32)
+        assert t["solve_ms"] > 0 and t["fft_ms"] > 0
+        y1 = S.precond_apply(x, prec)
+        assert (torch.linalg.vector_norm(y1 - y0) / torch.linalg.vector_norm(y0)).item() < tol
+    S2 = hz.Solver(d, c)
+    S2.factor(hz.abi.PREC_C64)
+    t2 = S2.autotune(hz.abi.PREC_C64, 8)
+    assert t2["from_cache"]
+    first = [l for l in cache.read_text().splitlines() if " 1 8 |" in l][0]
+    fam, cap, lines = (int(v) for v in first.split("|")[1].split())
+    assert (t2["solve_kernel"], t2["group_cap"], t2["fft_lines"]) == (fam, cap, lines)
+
+
 def test_plane_solve_groups_match_single_rhs(hz):
     """A batch of 11 right-hand sides (groups of 8, 2, 1) gives each vector's
     single-RHS solve."""

@@ -555,121 +555,6 @@ __global__ __launch_bounds__(1024) void k_solve_fast(const typename Cx<T>::t* __
   }
 }
 
-// Prefetching variant with a compile-time column count.  The column groups
-// divide nx exactly, so every worker thread owns CPG whole columns of each block
-// and needs no bounds checks: its loads are (uniform column base) + (one lane
-// byte offset).  The columns are walked in chunks of PF loads through a ring of
-// two register buffers: chunk c+1 is issued before chunk c is consumed, and the
-// first chunk of the NEXT block is issued right after the last FMA of the
-// current one, before the partial-sum write, the barriers and the serial
-// epilogue of the step.  512 threads per workgroup leave 256 VGPRs per thread
-// for the two buffers.
-constexpr int PF_THREADS = 512;
-
-template <typename T, int RPL, int CPG>
-__global__ __launch_bounds__(PF_THREADS) void k_solve_pf(const typename Cx<T>::t* __restrict__ G,
-                                                         typename Cx<T>::t* __restrict__ X,
-                                                         const double2* __restrict__ cmy,
-                                                         const double2* __restrict__ cpy, int n, int ny, int64_t n2,
-                                                         int64_t N, int nru, int ncg) {
-  typedef typename Cx<T>::t C;
-  typedef typename Unit<T>::t U;
-  constexpr int PF = CPG < 20 ? CPG : 20;
-  constexpr int NCH = (CPG + PF - 1) / PF;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  C* v = reinterpret_cast<C*>(smem_raw);  // n
-  C* part = v + n;                        // ncg * n
-  const int k = blockIdx.x;
-  const int64_t rhs = blockIdx.y;
-  C* x = X + rhs * N + (int64_t)k * n2;
-  const size_t nn = (size_t)n * n;
-  const C* Gk = G + (size_t)k * ny * nn;
-  const int tid = threadIdx.x;
-  const int g = tid / nru, ru = tid - g * nru;
-  const int c0 = g * CPG;
-  const bool worker = g < ncg;
-  // lane offset inside a column slice, in bytes (32-bit, unsigned): the loads
-  // are (uniform column base) + zext(offset), the scalar-base addressing form
-  const unsigned toff = (unsigned)((ru * RPL + n * c0) * (int)sizeof(C));
-  const size_t colbytes = (size_t)n * sizeof(C);
-  const int nsteps = 2 * ny - 1;
-
-  U buf[2][PF];
-  T acc[2 * RPL];
-  auto issue = [&](const C* B, int ch, U* b) {
-    const unsigned char* base = reinterpret_cast<const unsigned char*>(B) + (size_t)(ch * PF) * colbytes;
-#pragma unroll
-    for (int i = 0; i < PF; ++i)
-      if (ch * PF + i < CPG) b[i] = *reinterpret_cast<const U*>(base + (size_t)i * colbytes + toff);
-  };
-  auto consume = [&](int ch, const U* b) {
-#pragma unroll
-    for (int i = 0; i < PF; ++i)
-      if (ch * PF + i < CPG) unit_fma<T, RPL>(b[i], v[c0 + ch * PF + i], acc);
-  };
-  auto blk_of = [&](int s) { return s <= ny - 1 ? s : 2 * ny - 2 - s; };
-
-  for (int i = tid; i < n; i += blockDim.x) v[i] = x[i];
-  if (worker) issue(Gk, 0, buf[0]);
-  __syncthreads();
-
-  for (int s = 0; s < nsteps; ++s) {
-    const C* B = Gk + (size_t)blk_of(s) * nn;
-    C pre;
-    pre.x = 0; pre.y = 0;
-    int slot_next = -1;
-    if (s < ny - 1) slot_next = s + 1;
-    else if (2 * ny - 2 - s > 0) slot_next = 2 * ny - 3 - s;
-    if (tid < n && slot_next >= 0) pre = x[(size_t)n * slot_next + tid];
-#pragma unroll
-    for (int i = 0; i < 2 * RPL; ++i) acc[i] = 0;
-    if (worker) {
-#pragma unroll
-      for (int ch = 0; ch < NCH; ++ch) {
-        if (ch + 1 < NCH) issue(B, ch + 1, buf[(ch + 1) & 1]);
-        consume(ch, buf[ch & 1]);
-      }
-      if (s + 1 < nsteps) issue(Gk + (size_t)blk_of(s + 1) * nn, 0, buf[0]);
-#pragma unroll
-      for (int r = 0; r < RPL; ++r) {
-        C o;
-        o.x = acc[2 * r];
-        o.y = acc[2 * r + 1];
-        part[(size_t)g * n + ru * RPL + r] = o;
-      }
-    }
-    __syncthreads();
-    if (tid < n) {
-      C y;
-      y.x = 0; y.y = 0;
-      for (int q = 0; q < ncg; ++q) {
-        const C t = part[(size_t)q * n + tid];
-        y.x += t.x;
-        y.y += t.y;
-      }
-      C nv;
-      if (s < ny - 1) {
-        const double2 cm = cmy[s + 1];
-        nv.x = pre.x - (T)(cm.x * y.x - cm.y * y.y);
-        nv.y = pre.y - (T)(cm.x * y.y + cm.y * y.x);
-        x[(size_t)n * (s + 1) + tid] = nv;
-      } else {
-        const int j = 2 * ny - 2 - s;
-        x[(size_t)n * j + tid] = y;
-        if (j > 0) {
-          const double2 cp = cpy[j - 1];
-          nv.x = pre.x - (T)(cp.x * y.x - cp.y * y.y);
-          nv.y = pre.y - (T)(cp.x * y.y + cp.y * y.x);
-        } else {
-          nv = y;
-        }
-      }
-      v[tid] = nv;
-    }
-    __syncthreads();
-  }
-}
-
 // ------------------------------------------------------------ LDS-DMA solve
 // The same forward/backward sweep with the factor blocks streamed through LDS
 // by the gfx950 direct-to-LDS loads (global_load_lds_dwordx4).  A block is one
@@ -1098,41 +983,14 @@ bool try_launch_mr(int n, int ny, int nz, int nrhs, int cap, hipStream_t s, cons
   return true;
 }
 
-// Column groups of the prefetching kernel: the largest divisor of n whose
-// groups fit 1024 threads; returns the columns per group (0: no divisor > 0).
-int pf_columns(int n, int nru, int* ncg_out) {
-  const int maxg = PF_THREADS / nru;
-  for (int g = std::min(maxg, n); g >= 1; --g)
-    if (n % g == 0) {
-      *ncg_out = g;
-      return n / g;
-    }
-  return 0;
-}
-
-template <typename T, int RPL, int CPG>
-bool try_launch_pf(int cpg, int ncg, int nru, dim3 grid, hipStream_t s, const void* G, void* x, const double2* cmy,
-                   const double2* cpy, int n, int ny, int64_t n2, int64_t N) {
-  if (cpg != CPG) return false;
-  typedef typename Cx<T>::t C;
-  const size_t lds = sizeof(C) * (size_t)n * (1 + ncg);
-  set_max_lds((const void*)&k_solve_pf<T, RPL, CPG>, lds);
-  hipLaunchKernelGGL((k_solve_pf<T, RPL, CPG>), grid, dim3(PF_THREADS), lds, s, (const C*)G, (C*)x, cmy, cpy, n, ny,
-                     n2, N, nru, ncg);
-  HZ_HIP(hipGetLastError());
-  return true;
-}
-
-// Plane-solve kernel family: HZ_SOLVE_KERNEL=mr|dma|pf|fast picks one for
-// timing; the default is the multi-RHS LDS-DMA kernel where it is
-// instantiated and the chunked kernel elsewhere.
-enum SolveKernel { SK_DEFAULT, SK_MR, SK_DMA, SK_PF, SK_FAST };
+// Plane-solve kernel family: HZ_SOLVE_KERNEL=mr|dma|fast forces one for
+// timing; otherwise the tuned (or default) choice applies.
+enum SolveKernel { SK_DEFAULT, SK_MR, SK_DMA, SK_FAST };
 SolveKernel solve_kernel_choice() {
   const char* v = getenv("HZ_SOLVE_KERNEL");
   if (!v || !v[0]) return SK_DEFAULT;
   if (!strcmp(v, "mr")) return SK_MR;
   if (!strcmp(v, "dma")) return SK_DMA;
-  if (!strcmp(v, "pf")) return SK_PF;
   if (!strcmp(v, "fast")) return SK_FAST;
   return SK_DEFAULT;
 }
@@ -1164,6 +1022,11 @@ bool launch_mr_any(int prec, int n, int ny, int nz, int nrhs, int cap, hipStream
 }
 
 }  // namespace
+
+bool single_ring_available(int prec, int nx) {
+  (void)prec;
+  return nx == 200;
+}
 
 bool stream_solve_available(int prec, int nx) {
 #define HZ_MR_HAS(P, T, RPL, NCG, TCOL, NX, NRMAX) \
@@ -1249,11 +1112,14 @@ void launch_plane_solve_with(hz_problem* p, int prec, int nrhs, void* x, bool ad
   SolveKernel sk = solve_kernel_choice();
   if (sk == SK_DEFAULT && t.solve_family == HZ_SOLVE_STREAM) sk = SK_MR;
   if (sk == SK_DEFAULT && t.solve_family == HZ_SOLVE_CHUNKED) sk = SK_FAST;
+  if (sk == SK_DEFAULT && t.solve_family == HZ_SOLVE_STREAM_1) sk = SK_DMA;
+  // one right-hand side on a width with the single-RHS ring: that kernel by default
+  if (sk == SK_DEFAULT && nrhs == 1 && single_ring_available(prec, n)) sk = SK_DMA;
   if (!adjoint && ny >= 1 && (sk == SK_DEFAULT || sk == SK_MR) &&
       launch_mr_any(prec, n, ny, (int)p->nz, nrhs, t.group_cap, s, f.G.ptr, x, cmy, cpy, p->n2, p->N))
     return;
-  if (!adjoint && ny >= 1 && sk == SK_DMA) {
-    // LDS-DMA streaming kernel for the configurations it is instantiated for
+  if (!adjoint && ny >= 1 && sk == SK_DMA && reinterpret_cast<uintptr_t>(x) % 16 == 0) {
+    // single-RHS LDS-DMA ring for the configurations it is instantiated for
     const void* Gp = f.G.ptr;
     if (prec == HZ_PREC_C64 &&
         (try_launch_dma<float, 2, 5, 20, 200>(n, ny, grid, s, Gp, x, cmy, cpy, p->n2, p->N)))
@@ -1262,34 +1128,7 @@ void launch_plane_solve_with(hz_problem* p, int prec, int nrhs, void* x, bool ad
         (try_launch_dma<double, 1, 2, 10, 200>(n, ny, grid, s, Gp, x, cmy, cpy, p->n2, p->N)))
       return;
   }
-  if (!adjoint && ny >= 1 && sk == SK_PF) {
-    // prefetching kernel for the column splits it is instantiated for
-    if (prec == HZ_PREC_C64 && n % 2 == 0) {
-      const int nru = n / 2;
-      int ncg = 0;
-      const int cpg = pf_columns(n, nru, &ncg);
-      const void* G = f.G.ptr;
-      if (try_launch_pf<float, 2, 40>(cpg, ncg, nru, grid, s, G, x, cmy, cpy, n, ny, p->n2, p->N) ||
-          try_launch_pf<float, 2, 48>(cpg, ncg, nru, grid, s, G, x, cmy, cpy, n, ny, p->n2, p->N) ||
-          try_launch_pf<float, 2, 32>(cpg, ncg, nru, grid, s, G, x, cmy, cpy, n, ny, p->n2, p->N) ||
-          try_launch_pf<float, 2, 16>(cpg, ncg, nru, grid, s, G, x, cmy, cpy, n, ny, p->n2, p->N) ||
-          try_launch_pf<float, 2, 12>(cpg, ncg, nru, grid, s, G, x, cmy, cpy, n, ny, p->n2, p->N) ||
-          try_launch_pf<float, 2, 4>(cpg, ncg, nru, grid, s, G, x, cmy, cpy, n, ny, p->n2, p->N))
-        return;
-    }
-    if (prec == HZ_PREC_FP64) {
-      const int nru = n;
-      int ncg = 0;
-      const int cpg = pf_columns(n, nru, &ncg);
-      const void* G = f.G.ptr;
-      if (try_launch_pf<double, 1, 100>(cpg, ncg, nru, grid, s, G, x, cmy, cpy, n, ny, p->n2, p->N) ||
-          try_launch_pf<double, 1, 96>(cpg, ncg, nru, grid, s, G, x, cmy, cpy, n, ny, p->n2, p->N) ||
-          try_launch_pf<double, 1, 80>(cpg, ncg, nru, grid, s, G, x, cmy, cpy, n, ny, p->n2, p->N) ||
-          try_launch_pf<double, 1, 32>(cpg, ncg, nru, grid, s, G, x, cmy, cpy, n, ny, p->n2, p->N) ||
-          try_launch_pf<double, 1, 24>(cpg, ncg, nru, grid, s, G, x, cmy, cpy, n, ny, p->n2, p->N) ||
-          try_launch_pf<double, 1, 8>(cpg, ncg, nru, grid, s, G, x, cmy, cpy, n, ny, p->n2, p->N))
-        return;
-    }
+  if (!adjoint && ny >= 1) {
     // generic chunked kernel for every other split
     constexpr int CH = 8;
     if (prec == HZ_PREC_C64 && n % 2 == 0) {

@@ -63,7 +63,25 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))
     os.replace(tmp, LIB)
+    build_ref_driver()
     return LIB
+
+
+DRIVER_SRC = os.path.join(HERE, "..", "tests", "ref_shim_driver.cpp")
+DRIVER = os.path.join(OBJ, "ref_shim_driver")
+
+
+def build_ref_driver():
+    """The test caller of the reference-signature layer (tests/ref_shim_driver.cpp),
+    a host-only C++ program linked against libhz.so."""
+    if not os.path.exists(DRIVER_SRC):
+        return None
+    cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(HERE, "..", "include"), DRIVER_SRC, "-o", DRIVER,
+           "-L" + HERE, "-lhz", "-Wl,-rpath," + HERE]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("driver build failed: %s\n%s" % (" ".join(cmd), r.stderr))
+    return DRIVER
 
 
 if __name__ == "__main__":

@@ -7,6 +7,8 @@ Tolerances (relative 2-norm errors):
   Krylov solutions vs the oracle's                           1e-8 (FP64), 1e-6 (mixed)
   iteration counts                                           within +-1 of the oracle
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -276,6 +278,39 @@ def test_gmres_ref_matches_reference_path(hz, oracle, homo):
         assert _rel(_host(u), u_o) < 1e-10
 
 
+@pytest.mark.parametrize("homo", [1, 0])
+def test_reference_signatures_drop_in(hz, oracle, tmp_path, homo):
+    """A C++ caller written against the reference's own signatures (hz_ref.h:
+    Solve3DSparseUsingFT_CSR, ApplyCoeffMatrixA_CSR, Multiply3DSparseUsingFT,
+    FGMRES), linked against libhz.so, gives the oracle's results on the
+    reference grid (1e-10; GMRES within the reference's own tolerance)."""
+    import subprocess
+    drv = os.path.join(os.path.dirname(hz.abi.LIB_PATH), "_build", "ref_shim_driver")
+    if not os.path.exists(drv):
+        pytest.fail("the reference-signature test driver is not built (run __graft_entry__.build())")
+    n_nopml, nz_nopml, pml, spg, length, nu, beta, c_z = 8, 8, 4, 3, 240.0, 4.0, 0.5, 1280.0
+    pr = hz.problem
+    d = pr.ref_geometry(n_nopml=n_nopml, pml=pml, spg=spg, nz_nopml=nz_nopml, length=(length,) * 3, nu=nu,
+                        beta=beta, c_z=c_z, homo=bool(homo))
+    c = pr.ref_sound_speed(d)
+    P = oracle.Problem(d, c)
+    f = pr.point_source(d)
+    fin = tmp_path / "f.bin"
+    f.astype(np.complex128).tofile(fin)
+    out = str(tmp_path / "res")
+    r = subprocess.run([drv, str(n_nopml), str(nz_nopml), str(pml), str(spg), repr(length), repr(nu), repr(beta),
+                        str(homo), repr(c_z), "10", "5", "1e-8", str(fin), out], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, (r.returncode, r.stdout, r.stderr)
+    load = lambda name: np.fromfile(out + "_%s.bin" % name, dtype=np.complex128)
+    assert _rel(load("precond"), P.precond_apply(f)) < 1e-10
+    assert _rel(load("L0"), P.L0_apply(f)) < 1e-10
+    dl = load("dl")
+    assert _rel(load("applyA"), f - dl * P.precond_apply(f)) < 1e-10
+    u_o, st_o = P.gmres_ref(f, m=10, rest=5, thresh=1e-12, tol=1e-8, mask=1, ortho=0)
+    assert _rel(load("fgmres"), u_o) < 1e-8
+
+
 def test_fgmres_fp64_and_mixed_vs_oracle(hz, oracle):
     d, c = s7_case(hz, n=(20, 18, 22), npml=4, smooth=True)
     S, P = hz.Solver(d, c), oracle.Problem(d, c)
@@ -369,6 +404,40 @@ def test_fgmres_many_problems_match_separate_solves(hz):
         for q in range(f.shape[0]):
             assert st[q].converged and st[q].iters == st1[q].iters
         assert (torch.linalg.vector_norm(u - u1) / torch.linalg.vector_norm(u1)).item() < 1e-12
+
+
+def test_config1_point_source_matches_green_function(hz):
+    """64^3 incl. 8-point PML, c = 1, omega = 2 pi 6.4 (10 points per
+    wavelength), f = +1/h^3 at the centre node: FGMRES to 1e-8 in FP64 and in
+    mixed precision; away from the source and the layers the solution is
+    -e^{ikr}/(4 pi r) to discretisation accuracy, and the two precisions agree
+    to 1e-6."""
+    pr = hz.problem
+    n, npml = 64, 8
+    h = 1.0 / n
+    om = pr.omega_for_ppw(1.0, h, 10)
+    assert abs(om - 2 * np.pi * 6.4) < 1e-9
+    d = pr.cube_geometry(n, npml, om, beta=1.0, pml_c=3.0 * np.log(1e6) / (2.0 * npml * h))
+    c = np.ones(d.N)
+    idx = (n // 2 - 1,) * 3
+    f = _dev(pr.point_source(d, idx, amplitude=1.0 / h ** 3))
+    S = hz.Solver(d, c)
+    S.factor(hz.abi.PREC_FP64)
+    S.factor(hz.abi.PREC_C64)
+    u64, st64 = S.fgmres(f, m=30, restarts=100, tol=1e-8, prec=hz.abi.PREC_FP64)
+    u32, st32 = S.fgmres(f, m=30, restarts=100, tol=1e-8, prec=hz.abi.PREC_C64)
+    assert st64.converged and st32.converged
+    G = pr.green_function(d, idx, c=1.0)
+    X = np.arange(n)
+    I, J, K = np.meshgrid(X, X, X, indexing="ij")
+    r = np.sqrt((I - idx[0]) ** 2 + (J - idx[1]) ** 2 + (K - idx[2]) ** 2).transpose(2, 1, 0).reshape(-1) * h
+    sel = d.phys_mask() & (r >= 3 * h) & (r <= 0.3)
+    for u in (u64, u32):
+        uh = _host(u)
+        err = np.linalg.norm(uh[sel] + G[sel]) / np.linalg.norm(G[sel])
+        print("config 1: iterations %d / %d, Green's function error %.3e" % (st64.iters, st32.iters, err))
+        assert err < 0.1, err
+    assert (torch.linalg.vector_norm(u32 - u64) / torch.linalg.vector_norm(u64)).item() < 1e-6
 
 
 @pytest.mark.parametrize("mode", ["ref", "s7"])

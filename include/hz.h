@@ -132,7 +132,8 @@ enum {
   HZ_SOLVE_DEFAULT = 0, /* streaming multi-RHS solve where instantiated, else chunked */
   HZ_SOLVE_STREAM = 1,  /* factor tiles streamed through an LDS ring, up to 8 RHS per pass */
   HZ_SOLVE_CHUNKED = 2, /* chunked GEMV kernel, any width */
-  HZ_SOLVE_STREAM_1 = 3 /* single-RHS ring with an LDS cross-group reduction (200-wide planes) */
+  HZ_SOLVE_STREAM_1 = 3, /* single-RHS ring with an LDS cross-group reduction (200-wide planes) */
+  HZ_SOLVE_STREAM_MFMA = 4 /* streaming solve with the 8-RHS block product on FP32 MFMA (complex64) */
 };
 typedef struct {
   int32_t solve_kernel; /* HZ_SOLVE_* */

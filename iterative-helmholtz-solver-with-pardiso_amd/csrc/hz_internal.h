@@ -187,6 +187,8 @@ void launch_plane_solve_with(hz_problem* p, int prec, int nrhs, void* x, bool ad
                              hipStream_t s);
 // whether the streaming multi-RHS solve is instantiated for this width/precision
 bool stream_solve_available(int prec, int nx);
+// whether the 8-RHS MFMA streaming kernel is instantiated for this width/precision
+bool mfma_solve_available(int prec, int nx);
 // whether the single-RHS ring kernel is instantiated for this width/precision
 bool single_ring_available(int prec, int nx);
 

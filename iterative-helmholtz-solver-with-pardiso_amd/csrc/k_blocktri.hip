@@ -945,6 +945,218 @@ __global__ __launch_bounds__(dma::NT) void k_solve_mr(const typename Cx<T>::t* _
   __builtin_amdgcn_s_waitcnt(dma::wait_vm(0));
 }
 
+// ------------------------------------------- 8-RHS streaming solve on MFMA
+// Same LDS ring as k_solve_mr, for groups of 8 complex64 right-hand sides, with
+// the block product Y = G V (n x 8 complex) on the matrix cores: per 16-row x
+// 4-column tile of G two v_mfma_f32_16x16x4f32 accumulate
+//   C += Re(G) [Re V | Im V]  +  Im(G) [-Im V | Re V]  =  [Re Y | Im Y]
+// into one 16 x 16 FP32 tile (columns 0-7 real parts of the 8 vectors, 8-15
+// imaginary parts).  Operand layout of 16x16x4 f32 (wave64): A lane l holds
+// A[l % 16][l / 16], B lane l holds B[l / 16][l % 16], C lane l holds rows
+// 4 (l / 16) + 0..3 of column l % 16.  The B operands of a k-slice come from
+// the vectors in LDS once per wave and serve every row tile of the wave.
+// Wave w owns row tiles w and w + 8.
+typedef float hz_v4f __attribute__((ext_vector_type(4)));
+
+template <int NX, int TCOL>
+struct MfShape {
+  static constexpr int n = NX;
+  static constexpr int RT = (n + 15) / 16;                    // row tiles
+  static constexpr int KS = TCOL / 4;                         // k-slices per tile
+  static constexpr int tps = n / TCOL;
+  static constexpr int units = TCOL * n / 2;                  // 16-byte units per tile
+  static constexpr int R = (units + dma::NT - 1) / dma::NT;
+  static constexpr int WE = (tps < 3 ? tps : 3) * R;
+  static constexpr int TPW = (RT + 7) / 8;                    // row tiles per wave (max)
+  static constexpr size_t lds = 4 * (size_t)R * dma::NT * 16 + (size_t)n * 8 * sizeof(float2) +
+                                (size_t)8 * TPW * 64 * 16;
+  static_assert(n % TCOL == 0 && TCOL % 4 == 0 && n % 2 == 0, "tile split");
+  static_assert(TPW <= 2, "row tiles per wave");
+  // the last row tile may read up to 15 rows past the block inside the slot
+  static_assert((size_t)(TCOL - 1) * n + 16 * RT <= (size_t)R * dma::NT * 2, "slot slack for the last row tile");
+  static_assert(2 * R <= 63 && WE <= 63, "vmcnt range");
+  static_assert(lds <= 160 * 1024, "LDS budget");
+};
+
+template <int NX, int TCOL>
+__global__ __launch_bounds__(dma::NT) void k_solve_mfma8(const float2* __restrict__ G, float2* __restrict__ X,
+                                                         const double2* __restrict__ cmy,
+                                                         const double2* __restrict__ cpy, int ny, int64_t n2,
+                                                         int64_t N) {
+  typedef MfShape<NX, TCOL> S;
+  typedef float4 U;
+  constexpr int n = S::n, RT = S::RT, KS = S::KS, tps = S::tps, units = S::units, R = S::R, TPW = S::TPW;
+  constexpr int NR = 8;
+  __shared__ U s0[R * dma::NT], s1[R * dma::NT], s2[R * dma::NT], s3[R * dma::NT];
+  __shared__ U pbuf[8 * TPW * 64];
+  __shared__ float2 v[n * NR];
+
+  const int k = blockIdx.x;
+  const int64_t rhs0 = (int64_t)blockIdx.y * NR;
+  float2* xk = X + rhs0 * N + (int64_t)k * n2;
+  const size_t nn = (size_t)n * n;
+  const float2* Gk = G + (size_t)k * ny * nn;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6;
+  const int li = lane & 15, lg = lane >> 4;  // operand row/column index, k / row-group index
+  const int nsteps = 2 * ny - 1;
+  const int TT = nsteps * tps;
+  auto blk_of = [&](int s) { return s <= ny - 1 ? s : 2 * ny - 2 - s; };
+
+#define HZ_MF_TILE(TC_, SLOT)                                                                           \
+  {                                                                                                     \
+    const int tc_ = (TC_) >= TT ? 0 : (TC_); /* past the end: re-read tile 0 */                        \
+    const int s_ = tc_ / tps, t_ = tc_ - s_ * tps;                                                      \
+    const unsigned char* base_ = reinterpret_cast<const unsigned char*>(Gk + (size_t)blk_of(s_) * nn) + \
+                                 (size_t)t_ * TCOL * n * sizeof(float2);                                \
+    _Pragma("unroll") for (int r_ = 0; r_ < R; ++r_) {                                                  \
+      const int p_ = r_ * dma::NT + tid;                                                                \
+      const int u_ = p_ < units ? p_ : units - 1;                                                       \
+      __builtin_amdgcn_global_load_lds((const void*)(base_ + (size_t)u_ * 16),                          \
+                                       (HZ_LDS void*)(&SLOT[r_ * dma::NT + wv * 64]), 16, 0, 0);        \
+    }                                                                                                   \
+  }
+  // the next line's operand for this wave's epilogue: lane l loads unit
+  // 8 rt + 2 g + (l & 1) of right-hand side q, consumer (g, q) = (l >> 1) / 8, % 8
+#define HZ_MF_LINE(J_)                                                                                  \
+  {                                                                                                     \
+    _Pragma("unroll") for (int a_ = 0; a_ < TPW; ++a_) {                                                \
+      const int rt_ = wv + 8 * a_;                                                                      \
+      if (rt_ < RT) {                                                                                   \
+        const int c_ = lane >> 1, g_ = c_ >> 3, q_ = c_ & 7;                                            \
+        int u_ = 8 * rt_ + 2 * g_ + (lane & 1);                                                         \
+        u_ = u_ < n / 2 ? u_ : n / 2 - 1;                                                               \
+        const float2* src_ = xk + (int64_t)q_ * N + (size_t)n * (J_) + 2 * (size_t)u_;                  \
+        __builtin_amdgcn_global_load_lds((const void*)src_, (HZ_LDS void*)(&pbuf[(wv * TPW + a_) * 64]), \
+                                         16, 0, 0);                                                     \
+      }                                                                                                 \
+    }                                                                                                   \
+  }
+  hz_v4f acc[TPW];
+#define HZ_MF_EPILOGUE(S_)                                                                                \
+  {                                                                                                       \
+    const int se_ = (S_);                                                                                 \
+    __builtin_amdgcn_s_waitcnt(dma::wait_vm(S::WE));                                                      \
+    __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);                                                          \
+    dma::raw_barrier(); /* every wave is done reading v of this step */                                  \
+    _Pragma("unroll") for (int a_ = 0; a_ < TPW; ++a_) {                                                  \
+      const int rt_ = wv + 8 * a_;                                                                        \
+      float im_[4];                                                                                       \
+      _Pragma("unroll") for (int r_ = 0; r_ < 4; ++r_) im_[r_] = __shfl_xor(acc[a_][r_], 8);             \
+      if (rt_ < RT && li < 8) {                                                                           \
+        const int q_ = li;                                                                                \
+        const U p0_ = pbuf[(wv * TPW + a_) * 64 + 2 * (8 * lg + q_)];                                     \
+        const U p1_ = pbuf[(wv * TPW + a_) * 64 + 2 * (8 * lg + q_) + 1];                                 \
+        const float2 pre_[4] = {make_float2(p0_.x, p0_.y), make_float2(p0_.z, p0_.w),                   \
+                                make_float2(p1_.x, p1_.y), make_float2(p1_.z, p1_.w)};                  \
+        float2 out_[4];                                                                                   \
+        _Pragma("unroll") for (int r_ = 0; r_ < 4; ++r_) {                                                \
+          const int row_ = 16 * rt_ + 4 * lg + r_;                                                        \
+          const float2 y_ = make_float2(acc[a_][r_], im_[r_]);                                            \
+          float2 nv_;                                                                                     \
+          if (se_ < ny - 1) {                                                                             \
+            const double2 cm = cmy[se_ + 1];                                                              \
+            nv_.x = pre_[r_].x - (float)(cm.x * y_.x - cm.y * y_.y);                                      \
+            nv_.y = pre_[r_].y - (float)(cm.x * y_.y + cm.y * y_.x);                                      \
+            out_[r_] = nv_;                                                                               \
+          } else {                                                                                        \
+            const int j = 2 * ny - 2 - se_;                                                               \
+            out_[r_] = y_;                                                                                \
+            if (j > 0) {                                                                                  \
+              const double2 cp = cpy[j - 1];                                                              \
+              nv_.x = pre_[r_].x - (float)(cp.x * y_.x - cp.y * y_.y);                                    \
+              nv_.y = pre_[r_].y - (float)(cp.x * y_.y + cp.y * y_.x);                                    \
+            } else {                                                                                      \
+              nv_ = y_;                                                                                   \
+            }                                                                                             \
+          }                                                                                               \
+          if (row_ < n) v[row_ * NR + q_] = nv_;                                                          \
+        }                                                                                                 \
+        const int line_ = se_ < ny - 1 ? se_ + 1 : 2 * ny - 2 - se_;                                      \
+        float2* xq_ = xk + (int64_t)q_ * N + (size_t)n * line_ + 16 * rt_ + 4 * lg;                       \
+        if (16 * rt_ + 4 * lg + 1 < n)                                                                    \
+          *reinterpret_cast<U*>(xq_) = make_float4(out_[0].x, out_[0].y, out_[1].x, out_[1].y);           \
+        if (16 * rt_ + 4 * lg + 3 < n)                                                                    \
+          *reinterpret_cast<U*>(xq_ + 2) = make_float4(out_[2].x, out_[2].y, out_[3].x, out_[3].y);       \
+      }                                                                                                   \
+    }                                                                                                     \
+    __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);                                                          \
+  }
+#define HZ_MF_STAGE(TC_, SLOT, NEXT)                                                          \
+  {                                                                                           \
+    const int Tc = (TC_);                                                                     \
+    const int s = Tc / tps, t = Tc - s * tps;                                                 \
+    __builtin_amdgcn_s_waitcnt(dma::wait_vm(2 * R));                                          \
+    __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);                                              \
+    dma::raw_barrier();                                                                       \
+    if (Tc < TT && t == 0) {                                                                  \
+      _Pragma("unroll") for (int a = 0; a < TPW; ++a) acc[a] = hz_v4f{0.f, 0.f, 0.f, 0.f};    \
+      int slot_line = 0;                                                                      \
+      if (s < ny - 1) slot_line = s + 1;                                                      \
+      else if (2 * ny - 2 - s > 0) slot_line = 2 * ny - 3 - s;                                \
+      HZ_MF_LINE(slot_line)                                                                   \
+    }                                                                                         \
+    if (Tc < TT) {                                                                            \
+      const float2* sl = reinterpret_cast<const float2*>(SLOT);                               \
+      _Pragma("unroll") for (int ks = 0; ks < KS; ++ks) {                                     \
+        const float2 vv = v[(t * TCOL + 4 * ks + lg) * NR + (li & 7)];                        \
+        const float b1 = li < 8 ? vv.x : vv.y;                                                \
+        const float b2 = li < 8 ? -vv.y : vv.x;                                               \
+        _Pragma("unroll") for (int a = 0; a < TPW; ++a) {                                     \
+          const int rt = wv + 8 * a;                                                          \
+          if (rt < RT) {                                                                      \
+            const float2 ga = sl[(4 * ks + lg) * n + 16 * rt + li];                           \
+            acc[a] = __builtin_amdgcn_mfma_f32_16x16x4f32(ga.x, b1, acc[a], 0, 0, 0);         \
+            acc[a] = __builtin_amdgcn_mfma_f32_16x16x4f32(ga.y, b2, acc[a], 0, 0, 0);         \
+          }                                                                                   \
+        }                                                                                     \
+      }                                                                                       \
+    }                                                                                         \
+    HZ_MF_TILE(Tc + 3, NEXT)                                                                  \
+    if (Tc < TT && t == tps - 1) HZ_MF_EPILOGUE(s)                                            \
+  }
+
+  for (int i = tid; i < n * NR; i += dma::NT) {
+    const int q = i % NR, row = i / NR;
+    v[i] = xk[(int64_t)q * N + row];
+  }
+  __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
+  HZ_MF_TILE(0, s0)
+  HZ_MF_TILE(1, s1)
+  HZ_MF_TILE(2, s2)
+  for (int T0 = 0; T0 < TT; T0 += 4) {
+    HZ_MF_STAGE(T0, s0, s3)
+    HZ_MF_STAGE(T0 + 1, s1, s0)
+    HZ_MF_STAGE(T0 + 2, s2, s1)
+    HZ_MF_STAGE(T0 + 3, s3, s2)
+  }
+#undef HZ_MF_STAGE
+#undef HZ_MF_EPILOGUE
+#undef HZ_MF_LINE
+#undef HZ_MF_TILE
+  __builtin_amdgcn_s_waitcnt(dma::wait_vm(0));
+}
+
+template <int NX, int TCOL>
+bool try_launch_mfma8(int n, int ny, int nz, int groups, hipStream_t s, const void* G, void* x, const double2* cmy,
+                      const double2* cpy, int64_t n2, int64_t N) {
+  if (n != NX) return false;
+  hipLaunchKernelGGL((k_solve_mfma8<NX, TCOL>), dim3((unsigned)nz, (unsigned)groups), dim3(dma::NT), 0, s,
+                     (const float2*)G, (float2*)x, cmy, cpy, ny, n2, N);
+  HZ_HIP(hipGetLastError());
+  return true;
+}
+
+bool launch_mfma8_any(int n, int ny, int nz, int groups, hipStream_t s, const void* G, void* x, const double2* cmy,
+                      const double2* cpy, int64_t n2, int64_t N) {
+  return try_launch_mfma8<200, 20>(n, ny, nz, groups, s, G, x, cmy, cpy, n2, N) ||
+         try_launch_mfma8<192, 16>(n, ny, nz, groups, s, G, x, cmy, cpy, n2, N) ||
+         try_launch_mfma8<160, 20>(n, ny, nz, groups, s, G, x, cmy, cpy, n2, N) ||
+         try_launch_mfma8<128, 32>(n, ny, nz, groups, s, G, x, cmy, cpy, n2, N) ||
+         try_launch_mfma8<96, 32>(n, ny, nz, groups, s, G, x, cmy, cpy, n2, N) ||
+         try_launch_mfma8<64, 32>(n, ny, nz, groups, s, G, x, cmy, cpy, n2, N);
+}
+
 // Launch of one right-hand-side group size NR over nrhs_g * NR vectors.
 template <typename T, int RPL, int NCG, int TCOL, int NX, int NR>
 void launch_mr(int ny, int nz, int groups, hipStream_t s, const void* G, void* x, const double2* cmy,
@@ -983,15 +1195,16 @@ bool try_launch_mr(int n, int ny, int nz, int nrhs, int cap, hipStream_t s, cons
   return true;
 }
 
-// Plane-solve kernel family: HZ_SOLVE_KERNEL=mr|dma|fast forces one for
+// Plane-solve kernel family: HZ_SOLVE_KERNEL=mr|dma|fast|mfma forces one for
 // timing; otherwise the tuned (or default) choice applies.
-enum SolveKernel { SK_DEFAULT, SK_MR, SK_DMA, SK_FAST };
+enum SolveKernel { SK_DEFAULT, SK_MR, SK_DMA, SK_FAST, SK_MFMA };
 SolveKernel solve_kernel_choice() {
   const char* v = getenv("HZ_SOLVE_KERNEL");
   if (!v || !v[0]) return SK_DEFAULT;
   if (!strcmp(v, "mr")) return SK_MR;
   if (!strcmp(v, "dma")) return SK_DMA;
   if (!strcmp(v, "fast")) return SK_FAST;
+  if (!strcmp(v, "mfma")) return SK_MFMA;
   return SK_DEFAULT;
 }
 
@@ -1022,6 +1235,10 @@ bool launch_mr_any(int prec, int n, int ny, int nz, int nrhs, int cap, hipStream
 }
 
 }  // namespace
+
+bool mfma_solve_available(int prec, int nx) {
+  return prec == HZ_PREC_C64 && (nx == 200 || nx == 192 || nx == 160 || nx == 128 || nx == 96 || nx == 64);
+}
 
 bool single_ring_available(int prec, int nx) {
   (void)prec;
@@ -1113,10 +1330,24 @@ void launch_plane_solve_with(hz_problem* p, int prec, int nrhs, void* x, bool ad
   if (sk == SK_DEFAULT && t.solve_family == HZ_SOLVE_STREAM) sk = SK_MR;
   if (sk == SK_DEFAULT && t.solve_family == HZ_SOLVE_CHUNKED) sk = SK_FAST;
   if (sk == SK_DEFAULT && t.solve_family == HZ_SOLVE_STREAM_1) sk = SK_DMA;
+  if (sk == SK_DEFAULT && t.solve_family == HZ_SOLVE_STREAM_MFMA) sk = SK_MFMA;
+  if (!adjoint && ny >= 1 && sk == SK_MFMA && prec == HZ_PREC_C64 && nrhs >= 8 && mfma_solve_available(prec, n) &&
+      reinterpret_cast<uintptr_t>(x) % 16 == 0) {
+    // groups of 8 on the matrix cores, the remainder on the VALU streaming kernel
+    const int g = nrhs / 8;
+    launch_mfma8_any(n, ny, (int)p->nz, g, s, f.G.ptr, x, cmy, cpy, p->n2, p->N);
+    const int rest = nrhs - 8 * g;
+    if (rest == 0) return;
+    x = reinterpret_cast<float2*>(x) + (size_t)8 * g * p->N;
+    nrhs = rest;
+    grid = dim3((unsigned)p->nz, (unsigned)nrhs);
+    sk = SK_MR;
+  }
   // one right-hand side on a width with the single-RHS ring: that kernel by default
   if (sk == SK_DEFAULT && nrhs == 1 && single_ring_available(prec, n)) sk = SK_DMA;
-  if (!adjoint && ny >= 1 && (sk == SK_DEFAULT || sk == SK_MR) &&
-      launch_mr_any(prec, n, ny, (int)p->nz, nrhs, t.group_cap, s, f.G.ptr, x, cmy, cpy, p->n2, p->N))
+  if (!adjoint && ny >= 1 && (sk == SK_DEFAULT || sk == SK_MR || sk == SK_MFMA) &&
+      launch_mr_any(prec, n, ny, (int)p->nz, nrhs, sk == SK_MFMA ? 4 : t.group_cap, s, f.G.ptr, x, cmy, cpy, p->n2,
+                    p->N))
     return;
   if (!adjoint && ny >= 1 && sk == SK_DMA && reinterpret_cast<uintptr_t>(x) % 16 == 0) {
     // single-RHS LDS-DMA ring for the configurations it is instantiated for

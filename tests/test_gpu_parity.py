@@ -190,6 +190,32 @@ def test_precond_wide_planes_multi_rhs_vs_oracle(hz, oracle, nx):
             assert e < tol, (prec, q, e)
 
 
+@pytest.mark.parametrize("nx", [64, 96, 128, 160, 192, 200])
+def test_plane_solve_mfma_matches_vector_kernel(hz, oracle, monkeypatch, nx):
+    """The 8-RHS block product on the matrix cores (groups of 8, remainder on the
+    vector kernel) gives the vector kernel's plane solves (FP32, 1e-5) and the
+    oracle's complex64-accurate preconditioner (1e-4)."""
+    print("case mfma nx=%d" % nx, flush=True)
+    pr = hz.problem
+    d = pr.cube_geometry((nx, 10, 6), (8, 2, 2), pr.omega_for_ppw(0.8, 1.0 / nx, 8), beta=0.5, pml_c=30.0)
+    c = 0.8 + 0.4 * np.random.default_rng(nx + 1).random(d.N)
+    S = hz.Solver(d, c)
+    S.factor(hz.abi.PREC_C64)
+    x = torch.from_numpy(rand_vec(d.N, 60 + nx, 11)).to(torch.complex64).cuda()
+    monkeypatch.setenv("HZ_SOLVE_KERNEL", "mr")
+    ref = S.plane_solve(x.clone(), hz.abi.PREC_C64)
+    monkeypatch.setenv("HZ_SOLVE_KERNEL", "mfma")
+    got = S.plane_solve(x.clone(), hz.abi.PREC_C64)
+    for q in range(11):
+        e = (torch.linalg.vector_norm(got[q] - ref[q]) / torch.linalg.vector_norm(ref[q])).item()
+        assert e < 1e-5, (q, e)
+    xb = rand_vec(d.N, 70 + nx, 8)
+    y = _host(S.precond_apply(_dev(xb), hz.abi.PREC_C64))
+    P = oracle.Problem(d, c)
+    for q in (0, 7):
+        assert _rel(y[q], P.precond_apply(xb[q])) < 1e-4
+
+
 def test_autotune_choices_and_cache(hz, tmp_path, monkeypatch):
     """The autotuner picks a valid launch configuration per (precision, RHS
     count), the tuned preconditioner gives the untuned one's result, and the

@@ -8,6 +8,8 @@
 // g = w - deltaL * (L0^{-1} w) epilogue into the backward pass
 // (functions.cpp:5571-5574).  Forward is unscaled with e^{-2 pi i jk/n},
 // backward scales by 1/n (DFTI settings of functions.cpp:5864-5869).
+#include <cstdlib>
+
 #include "hz_internal.h"
 
 namespace hz {
@@ -37,44 +39,54 @@ struct FftArgs {
   int radix[24];
 };
 
+// LDS row of one z index: LINES lines plus padding, so that the strided rows a
+// Stockham stage writes (stride p rows in the first stage) spread over the banks.
+template <typename T, int LINES>
+struct Lds {
+  static constexpr int LD = LINES + (sizeof(T) == 4 ? 2 : 1);
+};
+
 template <typename T, int LINES>
 __device__ void stockham_lds(typename Cx<T>::t* x, typename Cx<T>::t* y, const FftArgs& a,
                              const typename Cx<T>::t* __restrict__ tw, int sign,
                              typename Cx<T>::t** result) {
   typedef typename Cx<T>::t C;
+  constexpr int LD = Lds<T, LINES>::LD;
   const int n = a.n;
   int len = n, s = 1;
   for (int st = 0; st < a.nstages; ++st) {
     const int p = a.radix[st];
     const int m = len / p;
     const int nbf = LINES * (n / p);
+    const float inv_s = 1.0f / (float)s;
     for (int b = threadIdx.x; b < nbf; b += blockDim.x) {
       const int l = b % LINES;
       const int qt = b / LINES;
-      const int t = qt % s;
-      const int q = qt / s;
+      // q = qt / s exactly: (qt + 1/2) / s stays below the next integer for qt, s < 2^12
+      const int q = (int)(((float)qt + 0.5f) * inv_s);
+      const int t = qt - q * s;
       const int in0 = t + s * q;
       const int out0 = t + s * p * q;
       C v[5];
       if (p == 4) {
-        C a0 = x[(in0) * LINES + l], a1 = x[(in0 + s * m) * LINES + l];
-        C a2 = x[(in0 + 2 * s * m) * LINES + l], a3 = x[(in0 + 3 * s * m) * LINES + l];
+        C a0 = x[(in0) * LD + l], a1 = x[(in0 + s * m) * LD + l];
+        C a2 = x[(in0 + 2 * s * m) * LD + l], a3 = x[(in0 + 3 * s * m) * LD + l];
         C t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = cmul_si(csub(a1, a3), sign);
         v[0] = cadd(t0, t2); v[1] = cadd(t1, t3); v[2] = csub(t0, t2); v[3] = csub(t1, t3);
       } else if (p == 2) {
-        C a0 = x[in0 * LINES + l], a1 = x[(in0 + s * m) * LINES + l];
+        C a0 = x[in0 * LD + l], a1 = x[(in0 + s * m) * LD + l];
         v[0] = cadd(a0, a1); v[1] = csub(a0, a1);
       } else if (p == 3) {
         const T h = (T)0.86602540378443864676372317075294;
-        C a0 = x[in0 * LINES + l], a1 = x[(in0 + s * m) * LINES + l], a2 = x[(in0 + 2 * s * m) * LINES + l];
+        C a0 = x[in0 * LD + l], a1 = x[(in0 + s * m) * LD + l], a2 = x[(in0 + 2 * s * m) * LD + l];
         C sm = cadd(a1, a2), df = cmul_si(cscale(csub(a1, a2), h), sign);
         C base = csub(a0, cscale(sm, (T)0.5));
         v[0] = cadd(a0, sm); v[1] = cadd(base, df); v[2] = csub(base, df);
       } else if (p == 5) {
         const T c1 = (T)0.30901699437494742410229341718282, c2 = (T)-0.80901699437494742410229341718282;
         const T s1 = (T)0.95105651629515357211643933337938, s2 = (T)0.58778525229247312916870595463907;
-        C a0 = x[in0 * LINES + l], a1 = x[(in0 + s * m) * LINES + l], a2 = x[(in0 + 2 * s * m) * LINES + l];
-        C a3 = x[(in0 + 3 * s * m) * LINES + l], a4 = x[(in0 + 4 * s * m) * LINES + l];
+        C a0 = x[in0 * LD + l], a1 = x[(in0 + s * m) * LD + l], a2 = x[(in0 + 2 * s * m) * LD + l];
+        C a3 = x[(in0 + 3 * s * m) * LD + l], a4 = x[(in0 + 4 * s * m) * LD + l];
         C b1 = cadd(a1, a4), b2 = cadd(a2, a3), d1 = csub(a1, a4), d2 = csub(a2, a3);
         C r1 = cadd(a0, cadd(cscale(b1, c1), cscale(b2, c2)));
         C r2 = cadd(a0, cadd(cscale(b1, c2), cscale(b2, c1)));
@@ -83,10 +95,11 @@ __device__ void stockham_lds(typename Cx<T>::t* x, typename Cx<T>::t* y, const F
         v[0] = cadd(a0, cadd(b1, b2)); v[1] = cadd(r1, i1); v[4] = csub(r1, i1); v[2] = cadd(r2, i2); v[3] = csub(r2, i2);
       }
       if (p <= 5) {
-        for (int k = 0; k < p; ++k) {
-          C w = tw[(q * k * s) % n];
+        y[out0 * LD + l] = v[0];
+        for (int k = 1; k < p; ++k) {
+          C w = tw[q * k * s];  // q s k < n: no wrap
           if (sign > 0) w.y = -w.y;
-          y[(out0 + s * k) * LINES + l] = (k == 0) ? v[0] : cmul(v[k], w);
+          y[(out0 + s * k) * LD + l] = cmul(v[k], w);
         }
       } else {
         // generic prime radix: direct sum with table twiddles
@@ -96,11 +109,11 @@ __device__ void stockham_lds(typename Cx<T>::t* x, typename Cx<T>::t* y, const F
           for (int r = 0; r < p; ++r) {
             C w = tw[(r * k * np) % n];
             if (sign > 0) w.y = -w.y;
-            acc = cadd(acc, cmul(x[(in0 + r * s * m) * LINES + l], w));
+            acc = cadd(acc, cmul(x[(in0 + r * s * m) * LD + l], w));
           }
-          C w = tw[(q * k * s) % n];
+          C w = tw[q * k * s];
           if (sign > 0) w.y = -w.y;
-          y[(out0 + s * k) * LINES + l] = cmul(acc, w);
+          y[(out0 + s * k) * LD + l] = cmul(acc, w);
         }
       }
     }
@@ -114,25 +127,28 @@ __device__ void stockham_lds(typename Cx<T>::t* x, typename Cx<T>::t* y, const F
 
 template <typename T, typename TI, typename TO, int EPI, int LINES>
 __global__ __launch_bounds__(256) void k_fft_z(const TI* __restrict__ in, TO* __restrict__ out, int64_t n2,
-                                               int64_t N, FftArgs a, const typename Cx<T>::t* __restrict__ tw,
+                                               int64_t N, FftArgs a, const typename Cx<T>::t* __restrict__ twg,
                                                int sign, T scale, const double2* __restrict__ w,
                                                const double2* __restrict__ dL) {
   typedef typename Cx<T>::t C;
+  constexpr int LD = Lds<T, LINES>::LD;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  C* buf0 = reinterpret_cast<C*>(smem_raw);
-  C* buf1 = buf0 + LINES * a.n;
   const int n = a.n;
+  C* buf0 = reinterpret_cast<C*>(smem_raw);
+  C* buf1 = buf0 + LD * n;
+  C* tw = buf1 + LD * n;  // twiddle table staged in LDS
   const int64_t rhs = blockIdx.y;
   const int64_t w0 = (int64_t)blockIdx.x * LINES;
   const TI* src = in + rhs * N;
   TO* dst = out + rhs * N;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = twg[i];
   for (int idx = threadIdx.x; idx < LINES * n; idx += blockDim.x) {
     const int l = idx % LINES, z = idx / LINES;
     const int64_t g = w0 + l;
     C v;
     v.x = 0; v.y = 0;
     if (g < n2) v = load_c<C>(src + g + n2 * z);
-    buf0[z * LINES + l] = v;
+    buf0[z * LD + l] = v;
   }
   __syncthreads();
   C* res;
@@ -141,7 +157,7 @@ __global__ __launch_bounds__(256) void k_fft_z(const TI* __restrict__ in, TO* __
     const int l = idx % LINES, z = idx / LINES;
     const int64_t g = w0 + l;
     if (g >= n2) continue;
-    C v = res[z * LINES + l];
+    C v = res[z * LD + l];
     const int64_t gi = g + n2 * z;
     if (EPI == 1) {
       const double2 ww = w[rhs * N + gi], dd = dL[gi];
@@ -169,7 +185,7 @@ void launch_lines(hz_problem* p, int sign, int nrhs, const void* in, void* out, 
   for (int i = 0; i < 24; ++i) a.radix[i] = i < a.nstages ? p->fft.radix[i] : 1;
   const typename Cx<T>::t* tw =
       sizeof(T) == 8 ? (const typename Cx<T>::t*)p->fft.tw64.ptr : (const typename Cx<T>::t*)p->fft.tw32.ptr;
-  size_t lds = 2 * (size_t)LINES * a.n * sizeof(typename Cx<T>::t);
+  size_t lds = (2 * (size_t)Lds<T, LINES>::LD + 1) * a.n * sizeof(typename Cx<T>::t);
   HZ_CHECK(lds <= 160 * 1024, HZ_ERR_ARG, "nz=%d too long for the LDS-resident FFT", a.n);
   dim3 grid((unsigned)((p->n2 + LINES - 1) / LINES), (unsigned)nrhs);
   if (lds > 64 * 1024)
@@ -195,13 +211,19 @@ void launch_typed(hz_problem* p, int lines, int sign, int nrhs, const void* in, 
 
 bool fft_lines_fit(int prec, int nz, int lines) {
   const size_t c = prec == HZ_PREC_FP64 ? 16 : 8;
-  return 2 * (size_t)lines * nz * c <= 160 * 1024;
+  const size_t ld = lines + (prec == HZ_PREC_FP64 ? 1 : 2);
+  return (2 * ld + 1) * nz * c <= 160 * 1024;
 }
 
 void launch_fft_z(hz_problem* p, int prec, int sign, int nrhs, const void* in, int in_prec, void* out,
                   int out_prec, int epi, const double2* w, const double2* dL, hipStream_t s) {
-  launch_fft_z_lines(p, p->tuning(prec, nrhs).fft_lines, prec, sign, nrhs, in, in_prec, out, out_prec, epi, w, dL,
-                     s);
+  // HZ_FFT_LINES=8|16|32 forces the lines per workgroup (timing and tests)
+  int lines = p->tuning(prec, nrhs).fft_lines;
+  if (const char* e = getenv("HZ_FFT_LINES")) {
+    const int v = atoi(e);
+    if ((v == 8 || v == 16 || v == 32) && fft_lines_fit(prec, (int)p->nz, v)) lines = v;
+  }
+  launch_fft_z_lines(p, lines, prec, sign, nrhs, in, in_prec, out, out_prec, epi, w, dL, s);
 }
 
 void launch_fft_z_lines(hz_problem* p, int lines, int prec, int sign, int nrhs, const void* in, int in_prec,

@@ -66,6 +66,24 @@ def rand_vec(N, seed, nrhs=None):
 
 
 # ----------------------------------------------------------------- FFT
+@pytest.mark.parametrize("lines", ["8", "32"])
+@pytest.mark.parametrize("nz", [7, 64, 96, 200])
+def test_fft_z_lines_variants_vs_numpy(hz, monkeypatch, nz, lines):
+    """Every lines-per-workgroup variant the autotuner may pick (16 is the
+    default, covered below): FP64 and complex64 transforms vs numpy."""
+    monkeypatch.setenv("HZ_FFT_LINES", lines)
+    d = hz.problem.cube_geometry((9, 5, nz), 0, 10.0)
+    S = hz.Solver(d, np.ones(d.N))
+    x = rand_vec(d.N, 3 * nz + 1, 2)
+    X = x.reshape(2, *d.shape_zyx)
+    ref = np.fft.fft(X, axis=1).reshape(2, -1)
+    for prec, tol in ((hz.abi.PREC_FP64, 1e-13), (hz.abi.PREC_C64, 2e-6)):
+        y = _host(S.fft_z(_dev(x), -1, prec))
+        assert _rel(y, ref) < tol, (prec, _rel(y, ref))
+        back = _host(S.fft_z(_dev(ref), +1, prec))
+        assert _rel(back, x) < tol
+
+
 @pytest.mark.parametrize("nz", [1, 2, 7, 12, 16, 20, 25, 30, 49, 64, 96, 150, 200])
 def test_fft_z_vs_numpy(hz, nz):
     print("case fft nz=%d" % nz, flush=True)

@@ -6,8 +6,9 @@
 // grid are dropped).  The adjoint uses the tables acm(i) = conj(cp(i-1)),
 // acp(i) = conj(cm(i+1)) and the conjugated diagonal.  Each thread owns one
 // (x,y) column of a z-chunk and streams it upward, keeping u(k-1), u(k), u(k+1)
-// in registers; x/y neighbours come through L1/L2.  The launch is tiled 64 x 4
-// in (x,y) with blockIdx remapped so that each XCD walks a contiguous z slab.
+// in registers; x/y neighbours come through L1/L2.  A tile is a run of 256
+// consecutive (x,y) columns (full waves for any nx), with blockIdx remapped so
+// that each XCD walks a contiguous range of tiles.
 //
 // k_plane5: the per-kz 5-point operator M_k on z-transformed data
 //   (M_k x)(i,j) = (cdg_x + cdg_y + kappa(i,j) - lambda_k) x + x/y couplings,
@@ -30,32 +31,32 @@ struct AxisTabs {
   const double2* d;
 };
 
-constexpr int SX = 64, SY = 4, KZ = 16;
+constexpr int ST = 256, KZ = 16;  // threads per tile (a run of ST consecutive (x,y) columns), z chunk
 
 template <bool ADJ, bool RESID>
-__global__ __launch_bounds__(SX* SY) void k_stencil7(const double2* __restrict__ u, double2* __restrict__ y,
-                                                     const double2* __restrict__ f, const double* __restrict__ mass,
-                                                     AxisTabs tx, AxisTabs ty, AxisTabs tz, int nx, int ny, int nz,
-                                                     int64_t N, int gx, int gy, int gz) {
+__global__ __launch_bounds__(ST) void k_stencil7(const double2* __restrict__ u, double2* __restrict__ y,
+                                                 const double2* __restrict__ f, const double* __restrict__ mass,
+                                                 AxisTabs tx, AxisTabs ty, AxisTabs tz, int nx, int ny, int nz,
+                                                 int64_t N, int gxy, int gz) {
   // blockIdx remap: consecutive linear ids go to different XCDs (round robin
-  // over 8); give each XCD a contiguous range of tiles so that z-neighbour
-  // planes of a tile stay in the same L2.
-  const int nblk = gx * gy * gz;
+  // over 8); give each XCD a contiguous range of tiles so that the y-neighbour
+  // columns of a tile (nx columns away, in the neighbouring tiles) stay in the
+  // same L2.
+  const int nblk = gxy * gz;
   const int lin = blockIdx.x;
   const int per = (nblk + 7) / 8;
   const int xcd = lin % 8, slot = lin / 8;
   const int tile = xcd * per + slot;
   if (tile >= nblk) return;
-  const int bx = tile % gx;
-  const int by = (tile / gx) % gy;
-  const int bz_all = tile / (gx * gy);
+  const int bxy = tile % gxy;
+  const int bz_all = tile / gxy;
   const int zch = (nz + KZ - 1) / KZ;
   const int rhs = bz_all / zch, bz = bz_all % zch;
 
-  const int i = bx * SX + threadIdx.x;
-  const int j = by * SY + threadIdx.y;
-  if (i >= nx || j >= ny) return;
   const int64_t n2 = (int64_t)nx * ny;
+  const int64_t ij = (int64_t)bxy * ST + threadIdx.x;  // one (x,y) column per thread, x fastest
+  if (ij >= n2) return;
+  const int j = (int)(ij / nx), i = (int)(ij - (int64_t)j * nx);
   const int k0 = bz * KZ;
   const int k1 = min(nz, k0 + KZ);
   u += (int64_t)rhs * N;
@@ -65,19 +66,23 @@ __global__ __launch_bounds__(SX* SY) void k_stencil7(const double2* __restrict__
   const double2 cmx = tx.m[i], cpx = tx.p[i], cmy = ty.m[j], cpy = ty.p[j];
   double2 dxy = make_double2(tx.d[i].x + ty.d[j].x, tx.d[i].y + ty.d[j].y);
   const bool hxm = i > 0, hxp = i < nx - 1, hym = j > 0, hyp = j < ny - 1;
-  int64_t l = i + (int64_t)nx * j + n2 * k0;
+  int64_t l = ij + n2 * k0;
   double2 um = k0 > 0 ? u[l - n2] : make_double2(0, 0);
   double2 uc = u[l];
   for (int k = k0; k < k1; ++k, l += n2) {
     const double2 up = (k + 1 < nz) ? u[l + n2] : make_double2(0, 0);
+    const double2 xm = hxm ? u[l - 1] : make_double2(0, 0);
+    const double2 xp = hxp ? u[l + 1] : make_double2(0, 0);
+    const double2 ym = hym ? u[l - nx] : make_double2(0, 0);
+    const double2 yp = hyp ? u[l + nx] : make_double2(0, 0);
     double2 dg = make_double2(dxy.x + tz.d[k].x, dxy.y + tz.d[k].y);
     if (ADJ) dg.y = -dg.y;
     dg.x += mass[l];
     double2 s = cmul(dg, uc);
-    if (hxm) s = cfma(cmx, u[l - 1], s);
-    if (hxp) s = cfma(cpx, u[l + 1], s);
-    if (hym) s = cfma(cmy, u[l - nx], s);
-    if (hyp) s = cfma(cpy, u[l + nx], s);
+    s = cfma(cmx, xm, s);
+    s = cfma(cpx, xp, s);
+    s = cfma(cmy, ym, s);
+    s = cfma(cpy, yp, s);
     s = cfma(tz.m[k], um, s);  // tz.m[0] multiplies a zero
     s = cfma(tz.p[k], up, s);
     if (RESID) {
@@ -128,18 +133,18 @@ void launch_stencil(hz_problem* p, int nrhs, const double2* u, double2* y, bool 
                     hipStream_t s) {
   HZ_CHECK(p->d.mode == HZ_MODE_S7, HZ_ERR_STATE, "7-point stencil needs HZ_MODE_S7");
   const int nx = (int)p->nx, ny = (int)p->ny, nz = (int)p->nz;
-  const int gx = (nx + SX - 1) / SX, gy = (ny + SY - 1) / SY, gz = ((nz + KZ - 1) / KZ) * nrhs;
-  const int nblk = gx * gy * gz;
+  const int gxy = (int)((p->n2 + ST - 1) / ST), gz = ((nz + KZ - 1) / KZ) * nrhs;
+  const int nblk = gxy * gz;
   const int launch = ((nblk + 7) / 8) * 8;
   AxisTabs tx = tabs(p, 0, adjoint), ty = tabs(p, 1, adjoint), tz = tabs(p, 2, adjoint);
   const double* mass = p->d_mass.as<double>();
-  dim3 blk(SX, SY);
+  dim3 blk(ST);
   if (adjoint) {
-    if (f) hipLaunchKernelGGL((k_stencil7<true, true>), dim3(launch), blk, 0, s, u, y, f, mass, tx, ty, tz, nx, ny, nz, p->N, gx, gy, gz);
-    else hipLaunchKernelGGL((k_stencil7<true, false>), dim3(launch), blk, 0, s, u, y, f, mass, tx, ty, tz, nx, ny, nz, p->N, gx, gy, gz);
+    if (f) hipLaunchKernelGGL((k_stencil7<true, true>), dim3(launch), blk, 0, s, u, y, f, mass, tx, ty, tz, nx, ny, nz, p->N, gxy, gz);
+    else hipLaunchKernelGGL((k_stencil7<true, false>), dim3(launch), blk, 0, s, u, y, f, mass, tx, ty, tz, nx, ny, nz, p->N, gxy, gz);
   } else {
-    if (f) hipLaunchKernelGGL((k_stencil7<false, true>), dim3(launch), blk, 0, s, u, y, f, mass, tx, ty, tz, nx, ny, nz, p->N, gx, gy, gz);
-    else hipLaunchKernelGGL((k_stencil7<false, false>), dim3(launch), blk, 0, s, u, y, f, mass, tx, ty, tz, nx, ny, nz, p->N, gx, gy, gz);
+    if (f) hipLaunchKernelGGL((k_stencil7<false, true>), dim3(launch), blk, 0, s, u, y, f, mass, tx, ty, tz, nx, ny, nz, p->N, gxy, gz);
+    else hipLaunchKernelGGL((k_stencil7<false, false>), dim3(launch), blk, 0, s, u, y, f, mass, tx, ty, tz, nx, ny, nz, p->N, gxy, gz);
   }
   HZ_HIP(hipGetLastError());
 }

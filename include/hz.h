@@ -138,7 +138,8 @@ enum {
 typedef struct {
   int32_t solve_kernel; /* HZ_SOLVE_* */
   int32_t group_cap;    /* right-hand sides per streaming pass: 8, 4, 2 or 1 */
-  int32_t fft_lines;    /* (x,y) lines per z-FFT workgroup: 8, 16 or 32 */
+  int32_t fft_lines;    /* (x,y) lines per z-FFT workgroup: 8, 16 or 32 (LDS Stockham), 0 for the
+                           register-resident kernel of nz = 64, 96, 128, 160, 192, 200 */
   int32_t from_cache;   /* 1 when read from the HZ_TUNE_CACHE file */
   double solve_ms;      /* plane solve of nrhs vectors with the choice */
   double fft_ms;        /* one forward z-FFT of nrhs vectors with the choice */

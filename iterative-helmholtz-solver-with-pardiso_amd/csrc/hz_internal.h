@@ -62,7 +62,7 @@ struct FftPlan {
 struct Tuning {
   int solve_family = 0;  // HZ_SOLVE_DEFAULT / _STREAM / _CHUNKED
   int group_cap = 8;     // largest right-hand-side group of the streaming solve (8, 4, 2, 1)
-  int fft_lines = 16;    // (x,y) lines per z-FFT workgroup (8, 16, 32)
+  int fft_lines = -1;    // z-FFT: 0 register-resident kernel, 8/16/32 lines per LDS workgroup, -1 automatic
 };
 
 // Plane factors of one precision.

@@ -136,7 +136,7 @@ void autotune(hz_problem* p, int prec, int nrhs, hz_tune_info* info) {
     }
     // forward z-FFT (complex128 in, the precision's plane layout out)
     best_fft = -1;
-    for (int lines : {8, 16, 32}) {
+    for (int lines : {0, 8, 16, 32}) {
       if (!fft_lines_fit(prec, (int)p->nz, lines)) continue;
       const double ms = time_ms(p, [&] {
         launch_fft_z_lines(p, lines, prec, -1, nrhs, p->w_b.ptr, 0, p->w_a.ptr, prec == HZ_PREC_C64 ? 1 : 0, 0,

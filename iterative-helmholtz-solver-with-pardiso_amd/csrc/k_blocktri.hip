@@ -1331,6 +1331,13 @@ void launch_plane_solve_with(hz_problem* p, int prec, int nrhs, void* x, bool ad
   if (sk == SK_DEFAULT && t.solve_family == HZ_SOLVE_CHUNKED) sk = SK_FAST;
   if (sk == SK_DEFAULT && t.solve_family == HZ_SOLVE_STREAM_1) sk = SK_DMA;
   if (sk == SK_DEFAULT && t.solve_family == HZ_SOLVE_STREAM_MFMA) sk = SK_MFMA;
+
+  // untuned defaults: complex64 groups of 8 on the matrix cores, one
+  // right-hand side on the single-RHS ring where they are instantiated
+  if (sk == SK_DEFAULT && t.solve_family == HZ_SOLVE_DEFAULT && prec == HZ_PREC_C64 && nrhs >= 8 &&
+      mfma_solve_available(prec, n))
+    sk = SK_MFMA;
+  if (sk == SK_DEFAULT && nrhs == 1 && single_ring_available(prec, n)) sk = SK_DMA;
   if (!adjoint && ny >= 1 && sk == SK_MFMA && prec == HZ_PREC_C64 && nrhs >= 8 && mfma_solve_available(prec, n) &&
       reinterpret_cast<uintptr_t>(x) % 16 == 0) {
     // groups of 8 on the matrix cores, the remainder on the VALU streaming kernel
@@ -1343,8 +1350,6 @@ void launch_plane_solve_with(hz_problem* p, int prec, int nrhs, void* x, bool ad
     grid = dim3((unsigned)p->nz, (unsigned)nrhs);
     sk = SK_MR;
   }
-  // one right-hand side on a width with the single-RHS ring: that kernel by default
-  if (sk == SK_DEFAULT && nrhs == 1 && single_ring_available(prec, n)) sk = SK_DMA;
   if (!adjoint && ny >= 1 && (sk == SK_DEFAULT || sk == SK_MR || sk == SK_MFMA) &&
       launch_mr_any(prec, n, ny, (int)p->nz, nrhs, sk == SK_MFMA ? 4 : t.group_cap, s, f.G.ptr, x, cmy, cpy, p->n2,
                     p->N))

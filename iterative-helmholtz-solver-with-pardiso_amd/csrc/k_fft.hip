@@ -176,6 +176,183 @@ __global__ __launch_bounds__(256) void k_fft_z(const TI* __restrict__ in, TO* __
   }
 }
 
+// ------------------------------------------- register-resident z-FFT
+// For n = N1 * N2 with both factors small: z = z1 + N1 z2, k = k2 + N2 k1,
+//   X[k2 + N2 k1] = sum_z1 W_n^{z1 k2} W_N1^{z1 k1} sum_z2 x[z1 + N1 z2] W_N2^{z2 k2}.
+// A workgroup owns LINES consecutive (x,y) lines; thread (l, z1) loads the N2
+// samples z1 + N1 z2 of line l (each load instruction reads LINES consecutive
+// lines), transforms them in registers, applies W_n^{z1 k2} and stores them to
+// LDS; after one barrier thread (l, k2) reads the N1 values of its column,
+// transforms them in registers and writes X[k2 + N2 k1].  The in-register
+// DFTs are compile-time mixed-radix Cooley-Tukey with constant twiddles.
+
+// out[k] = sum_j in[j] e^{sign 2 pi i j k / N}  (in place, compile-time N)
+template <typename T, int N, int SIGN>
+struct RegDft {
+  typedef typename Cx<T>::t C;
+  static constexpr int P = (N % 4 == 0 && N != 4) ? 4 : (N % 2 == 0 && N != 2) ? 2 : (N % 3 == 0 && N != 3) ? 3
+                           : (N % 5 == 0 && N != 5) ? 5 : N;
+  static constexpr int M = N / P;
+  __device__ __forceinline__ static void run(C* v) {
+    if constexpr (P == N) {
+      RegDft<T, N, SIGN>::prime(v);
+    } else {
+      // decimation in time: sub-transforms of the P residues, twiddles, P-point butterflies
+      C t[N];
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        C sub[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) sub[m] = v[p + P * m];
+        RegDft<T, M, SIGN>::run(sub);
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+          const double ang = (double)SIGN * 2.0 * 3.14159265358979323846 * (double)(p * k) / (double)N;
+          C w;
+          w.x = (T)cos(ang);
+          w.y = (T)sin(ang);
+          t[p * M + k] = (p * k == 0) ? sub[k] : cmul(sub[k], w);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < M; ++k) {
+        C col[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) col[p] = t[p * M + k];
+        RegDft<T, P, SIGN>::run(col);
+#pragma unroll
+        for (int q = 0; q < P; ++q) v[k + M * q] = col[q];
+      }
+    }
+  }
+  __device__ __forceinline__ static void prime(C* v) {
+    if constexpr (N == 1) {
+    } else if constexpr (N == 2) {
+      const C a = v[0], b = v[1];
+      v[0] = cadd(a, b);
+      v[1] = csub(a, b);
+    } else if constexpr (N == 4) {
+      const C t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]), t2 = cadd(v[1], v[3]);
+      const C t3 = cmul_si(csub(v[1], v[3]), SIGN);
+      v[0] = cadd(t0, t2); v[1] = cadd(t1, t3); v[2] = csub(t0, t2); v[3] = csub(t1, t3);
+    } else if constexpr (N == 3) {
+      const T h = (T)0.86602540378443864676372317075294;
+      const C sm = cadd(v[1], v[2]), df = cmul_si(cscale(csub(v[1], v[2]), h), SIGN);
+      const C base = csub(v[0], cscale(sm, (T)0.5));
+      v[0] = cadd(v[0], sm); v[1] = cadd(base, df); v[2] = csub(base, df);
+    } else if constexpr (N == 5) {
+      const T c1 = (T)0.30901699437494742410229341718282, c2 = (T)-0.80901699437494742410229341718282;
+      const T s1 = (T)0.95105651629515357211643933337938, s2 = (T)0.58778525229247312916870595463907;
+      const C a0 = v[0];
+      const C b1 = cadd(v[1], v[4]), b2 = cadd(v[2], v[3]), d1 = csub(v[1], v[4]), d2 = csub(v[2], v[3]);
+      const C r1 = cadd(a0, cadd(cscale(b1, c1), cscale(b2, c2)));
+      const C r2 = cadd(a0, cadd(cscale(b1, c2), cscale(b2, c1)));
+      const C i1 = cmul_si(cadd(cscale(d1, s1), cscale(d2, s2)), SIGN);
+      const C i2 = cmul_si(csub(cscale(d1, s2), cscale(d2, s1)), SIGN);
+      v[0] = cadd(a0, cadd(b1, b2)); v[1] = cadd(r1, i1); v[4] = csub(r1, i1); v[2] = cadd(r2, i2); v[3] = csub(r2, i2);
+    } else {
+      static_assert(N <= 5, "in-register DFT: unsupported prime factor");
+    }
+  }
+};
+
+template <typename T, typename TI, typename TO, int EPI, int N1, int N2, int LINES, int SIGN>
+__global__ __launch_bounds__(LINES * N1) void k_fft_reg(const TI* __restrict__ in, TO* __restrict__ out, int64_t n2,
+                                                        int64_t N, const typename Cx<T>::t* __restrict__ twg,
+                                                        T scale, const double2* __restrict__ w,
+                                                        const double2* __restrict__ dL) {
+  typedef typename Cx<T>::t C;
+  constexpr int n = N1 * N2;
+  __shared__ C A[n * LINES];  // A[(z1 N2 + k2) LINES + l]
+  __shared__ C tw[n];         // exp(-2 pi i t / n)
+  const int tid = threadIdx.x;
+  const int l = tid % LINES, z1 = tid / LINES;
+  const int64_t rhs = blockIdx.y;
+  const int64_t g = (int64_t)blockIdx.x * LINES + l;
+  const bool live = g < n2;
+  const TI* src = in + rhs * N;
+  TO* dst = out + rhs * N;
+  for (int i = tid; i < n; i += LINES * N1) tw[i] = twg[i];
+  // step A: N2-point transforms of the samples z1 + N1 z2
+  C v[N2];
+#pragma unroll
+  for (int z2 = 0; z2 < N2; ++z2) {
+    C x;
+    x.x = 0; x.y = 0;
+    if (live) x = load_c<C>(src + g + n2 * (int64_t)(z1 + N1 * z2));
+    v[z2] = x;
+  }
+  RegDft<T, N2, SIGN>::run(v);
+  __syncthreads();  // twiddle table staged
+#pragma unroll
+  for (int k2 = 0; k2 < N2; ++k2) {
+    C wt = tw[z1 * k2];  // z1 k2 < n: no wrap
+    if (SIGN > 0) wt.y = -wt.y;
+    A[(z1 * N2 + k2) * LINES + l] = (k2 == 0) ? v[k2] : cmul(v[k2], wt);
+  }
+  __syncthreads();
+  // step C: N1-point transforms of the columns k2
+  for (int item = tid; item < LINES * N2; item += LINES * N1) {
+    const int lc = item % LINES, k2 = item / LINES;
+    const int64_t gc = (int64_t)blockIdx.x * LINES + lc;
+    C u[N1];
+#pragma unroll
+    for (int a = 0; a < N1; ++a) u[a] = A[(a * N2 + k2) * LINES + lc];
+    RegDft<T, N1, SIGN>::run(u);
+    if (gc >= n2) continue;
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) {
+      const int64_t gi = gc + n2 * (int64_t)(k2 + N2 * k1);
+      const C vv = u[k1];
+      if (EPI == 1) {
+        const double2 ww = w[rhs * N + gi], dd = dL[gi];
+        const double vx = (double)vv.x * (double)scale, vy = (double)vv.y * (double)scale;
+        TO r;
+        r.x = ww.x - (dd.x * vx - dd.y * vy);
+        r.y = ww.y - (dd.x * vy + dd.y * vx);
+        dst[gi] = r;
+      } else {
+        TO r;
+        r.x = vv.x * scale;
+        r.y = vv.y * scale;
+        dst[gi] = r;
+      }
+    }
+  }
+}
+
+template <typename T, typename TI, typename TO, int EPI, int N1, int N2>
+bool try_fft_reg(hz_problem* p, int sign, int nrhs, const void* in, void* out, const double2* w, const double2* dL,
+                 hipStream_t s) {
+  if (p->fft.n != N1 * N2) return false;
+  constexpr int LINES = sizeof(T) == 4 ? 64 : 32;
+  const typename Cx<T>::t* tw =
+      sizeof(T) == 8 ? (const typename Cx<T>::t*)p->fft.tw64.ptr : (const typename Cx<T>::t*)p->fft.tw32.ptr;
+  dim3 grid((unsigned)((p->n2 + LINES - 1) / LINES), (unsigned)nrhs);
+  const T scale = sign > 0 ? (T)(1.0 / (double)(N1 * N2)) : (T)1;
+  if (sign < 0)
+    hipLaunchKernelGGL((k_fft_reg<T, TI, TO, EPI, N1, N2, LINES, -1>), grid, dim3(LINES * N1), 0, s, (const TI*)in,
+                       (TO*)out, p->n2, p->N, tw, scale, w, dL);
+  else
+    hipLaunchKernelGGL((k_fft_reg<T, TI, TO, EPI, N1, N2, LINES, 1>), grid, dim3(LINES * N1), 0, s, (const TI*)in,
+                       (TO*)out, p->n2, p->N, tw, scale, w, dL);
+  HZ_HIP(hipGetLastError());
+  return true;
+}
+
+// lengths of the configurations: 64 = 8 x 8, 96 = 8 x 12, 128 = 8 x 16,
+// 160 = 10 x 16, 192 = 12 x 16, 200 = 8 x 25
+template <typename T, typename TI, typename TO, int EPI>
+bool launch_reg(hz_problem* p, int sign, int nrhs, const void* in, void* out, const double2* w, const double2* dL,
+                hipStream_t s) {
+  return try_fft_reg<T, TI, TO, EPI, 8, 25>(p, sign, nrhs, in, out, w, dL, s) ||
+         try_fft_reg<T, TI, TO, EPI, 12, 16>(p, sign, nrhs, in, out, w, dL, s) ||
+         try_fft_reg<T, TI, TO, EPI, 10, 16>(p, sign, nrhs, in, out, w, dL, s) ||
+         try_fft_reg<T, TI, TO, EPI, 8, 16>(p, sign, nrhs, in, out, w, dL, s) ||
+         try_fft_reg<T, TI, TO, EPI, 8, 12>(p, sign, nrhs, in, out, w, dL, s) ||
+         try_fft_reg<T, TI, TO, EPI, 8, 8>(p, sign, nrhs, in, out, w, dL, s);
+}
+
 template <typename T, typename TI, typename TO, int EPI, int LINES>
 void launch_lines(hz_problem* p, int sign, int nrhs, const void* in, void* out, const double2* w,
                   const double2* dL, hipStream_t s) {
@@ -199,6 +376,7 @@ void launch_lines(hz_problem* p, int sign, int nrhs, const void* in, void* out, 
 template <typename T, typename TI, typename TO, int EPI>
 void launch_typed(hz_problem* p, int lines, int sign, int nrhs, const void* in, void* out, const double2* w,
                   const double2* dL, hipStream_t s) {
+  if (lines == 0 && launch_reg<T, TI, TO, EPI>(p, sign, nrhs, in, out, w, dL, s)) return;
   if (lines == 32 && fft_lines_fit(sizeof(T) == 8 ? HZ_PREC_FP64 : HZ_PREC_C64, p->fft.n, 32))
     launch_lines<T, TI, TO, EPI, 32>(p, sign, nrhs, in, out, w, dL, s);
   else if (lines == 8)
@@ -209,7 +387,12 @@ void launch_typed(hz_problem* p, int lines, int sign, int nrhs, const void* in, 
 
 }  // namespace
 
+bool fft_reg_available(int nz) {
+  return nz == 200 || nz == 192 || nz == 160 || nz == 128 || nz == 96 || nz == 64;
+}
+
 bool fft_lines_fit(int prec, int nz, int lines) {
+  if (lines == 0) return fft_reg_available(nz);
   const size_t c = prec == HZ_PREC_FP64 ? 16 : 8;
   const size_t ld = lines + (prec == HZ_PREC_FP64 ? 1 : 2);
   return (2 * ld + 1) * nz * c <= 160 * 1024;
@@ -217,11 +400,13 @@ bool fft_lines_fit(int prec, int nz, int lines) {
 
 void launch_fft_z(hz_problem* p, int prec, int sign, int nrhs, const void* in, int in_prec, void* out,
                   int out_prec, int epi, const double2* w, const double2* dL, hipStream_t s) {
-  // HZ_FFT_LINES=8|16|32 forces the lines per workgroup (timing and tests)
+  // HZ_FFT_LINES=0|8|16|32 forces the lines per workgroup, 0 being the
+  // register-resident kernel (timing and tests)
   int lines = p->tuning(prec, nrhs).fft_lines;
+  if (lines < 0) lines = fft_reg_available((int)p->nz) ? 0 : 16;
   if (const char* e = getenv("HZ_FFT_LINES")) {
     const int v = atoi(e);
-    if ((v == 8 || v == 16 || v == 32) && fft_lines_fit(prec, (int)p->nz, v)) lines = v;
+    if ((v == 0 || v == 8 || v == 16 || v == 32) && fft_lines_fit(prec, (int)p->nz, v)) lines = v;
   }
   launch_fft_z_lines(p, lines, prec, sign, nrhs, in, in_prec, out, out_prec, epi, w, dL, s);
 }

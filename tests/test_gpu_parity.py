@@ -66,11 +66,13 @@ def rand_vec(N, seed, nrhs=None):
 
 
 # ----------------------------------------------------------------- FFT
-@pytest.mark.parametrize("lines", ["8", "32"])
-@pytest.mark.parametrize("nz", [7, 64, 96, 200])
+@pytest.mark.parametrize("lines", ["0", "8", "32"])
+@pytest.mark.parametrize("nz", [7, 64, 96, 128, 160, 192, 200])
 def test_fft_z_lines_variants_vs_numpy(hz, monkeypatch, nz, lines):
-    """Every lines-per-workgroup variant the autotuner may pick (16 is the
-    default, covered below): FP64 and complex64 transforms vs numpy."""
+    """Every z-FFT variant the autotuner may pick (16 lines per workgroup is the
+    default, covered below; 0 is the register-resident kernel of the
+    configuration lengths): FP64 and complex64 transforms vs numpy, on a line
+    count that is not a multiple of the workgroup's."""
     monkeypatch.setenv("HZ_FFT_LINES", lines)
     d = hz.problem.cube_geometry((9, 5, nz), 0, 10.0)
     S = hz.Solver(d, np.ones(d.N))
@@ -97,6 +99,24 @@ def test_fft_z_vs_numpy(hz, nz):
     assert _rel(_host(S.fft_z(_dev(x), +1, hz.abi.PREC_FP64)), want_b) < 1e-14
     assert _rel(_host(S.fft_z(_dev(x), -1, hz.abi.PREC_C64)), want_f) < 2e-6
     assert _rel(_host(S.fft_z(_dev(x), +1, hz.abi.PREC_C64)), want_b) < 2e-6
+
+
+def test_fft_register_kernel_ref_epilogue(hz, monkeypatch):
+    """The register-resident FFT's fused epilogue g = w - deltaL L0^{-1} w (REF
+    mode, nz = 64) equals the LDS Stockham kernel's."""
+    pr = hz.problem
+    d = pr.ref_geometry(n_nopml=8, pml=3, spg=6, nz_nopml=52, length=(240.0,) * 3)
+    assert d.n[2] == 64
+    S = hz.Solver(d, pr.ref_sound_speed(d))
+    S.factor(hz.abi.PREC_FP64)
+    S.factor(hz.abi.PREC_C64)
+    x = _dev(rand_vec(d.N, 17, 2))
+    for prec, tol in ((hz.abi.PREC_FP64, 1e-13), (hz.abi.PREC_C64, 1e-6)):
+        monkeypatch.setenv("HZ_FFT_LINES", "16")
+        a = S.op_pre_apply(x, prec)
+        monkeypatch.setenv("HZ_FFT_LINES", "0")
+        b = S.op_pre_apply(x, prec)
+        assert (torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(a)).item() < tol
 
 
 def test_fft_z_multi_rhs_and_ragged_lines(hz):
@@ -250,8 +270,8 @@ def test_autotune_choices_and_cache(hz, tmp_path, monkeypatch):
         y0 = S.precond_apply(x, prec)
         t = S.autotune(prec, 8)
         print("tuned prec %d: %s" % (prec, t))
-        assert t["solve_kernel"] in (hz.abi.SOLVE_STREAM, hz.abi.SOLVE_CHUNKED) and not t["from_cache"]
-        assert t["group_cap"] in (1, 2, 4, 8) and t["fft_lines"] in (8, 16, 32)
+        assert t["solve_kernel"] in (hz.abi.SOLVE_STREAM, hz.abi.SOLVE_CHUNKED, hz.abi.SOLVE_STREAM_MFMA) and not t["from_cache"]
+        assert t["group_cap"] in (1, 2, 4, 8) and t["fft_lines"] in (0, 8, 16, 32)
         assert t["solve_ms"] > 0 and t["fft_ms"] > 0
         y1 = S.precond_apply(x, prec)
         assert (torch.linalg.vector_norm(y1 - y0) / torch.linalg.vector_norm(y0)).item() < tol

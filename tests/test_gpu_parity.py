@@ -552,6 +552,60 @@ def test_misfit_gradient_vs_oracle(hz, oracle):
     assert _rel(g, g_o) < 1e-6
 
 
+@pytest.mark.slow
+def test_adjoint_gradient_vs_finite_differences_20_directions(hz):
+    """The configuration-5 check at reduced size (48^3 incl. 8-point PML):
+    c = 1 + 3 seeded +-10% Gaussian blobs smoothed by sigma 0.15, 16 sources on
+    a 4 x 4 grid at k = npml + 4, receivers at every physical node of that
+    plane.  The mixed-precision adjoint gradient (solves to 1e-8) matches
+    central finite differences of J along 20 seeded smooth directions
+    (epsilon = 1e-6 relative, FP64 solves to 1e-11) within 1e-4."""
+    pr = hz.problem
+    n, npml = 48, 8
+    h = 1.0 / n
+    om = pr.omega_for_ppw(1.0, h, 10.0)
+    d = pr.cube_geometry(n, npml, om, beta=1.0, pml_c=3.0 * np.log(1e6) / (2.0 * npml * h))
+    rng = np.random.default_rng(5)
+    lo, hi = (npml + 1) * h, (n - npml) * h
+    centres = [tuple(rng.uniform(lo + 0.1, hi - 0.1, size=3)) for _ in range(3)]
+    amps = [0.1 * (1 if rng.random() < 0.5 else -1) for _ in range(3)]
+    c_true = pr.gaussian_blobs(d, centres, amps, 0.08)
+    c_cur = pr.gaussian_smooth(c_true, d, 0.15)
+    kz = npml + 4
+    xs = [npml + int((i + 0.5) * (n - 2 * npml) / 4) for i in range(4)]
+    f = np.zeros((16, d.N), complex)
+    for q, (i, j) in enumerate([(i, j) for j in xs for i in xs]):
+        f[q, pr.flat_index(d, i, j, kz)] = 1.0 / h ** 3
+    rec = np.array([pr.flat_index(d, i, j, kz) for j in range(npml, n - npml) for i in range(npml, n - npml)])
+    fd = _dev(f)
+    St = hz.Solver(d, c_true)
+    St.factor(hz.abi.PREC_FP64)
+    ut, stt = St.fgmres(fd, m=40, restarts=50, tol=1e-12, prec=hz.abi.PREC_FP64)
+    data = _host(ut)[:, rec]
+    St.close()
+    S = hz.Solver(d, c_cur)
+    S.factor(hz.abi.PREC_C64)
+    S.misfit_forward(fd, rec, data, m=40, restarts=50, tol=1e-8, prec=hz.abi.PREC_C64)
+    g = _host(S.misfit_backward())
+    S.close()
+    m0 = 1.0 / c_cur ** 2
+    worst = 0.0
+    for k in range(20):
+        dm = pr.gaussian_smooth(np.random.default_rng(500 + k).standard_normal(d.N), d, 0.05)
+        dm *= 1e-6 * np.linalg.norm(m0) / np.linalg.norm(dm)
+        Jpm = []
+        for sgn in (1.0, -1.0):
+            Sp = hz.Solver(d, 1.0 / np.sqrt(m0 + sgn * dm))
+            Sp.factor(hz.abi.PREC_FP64)
+            Jpm.append(Sp.misfit_forward(fd, rec, data, m=40, restarts=50, tol=1e-11, prec=hz.abi.PREC_FP64))
+            Sp.close()
+        dj_fd = (Jpm[0] - Jpm[1]) / 2.0
+        dj = float(np.dot(g, dm))
+        worst = max(worst, abs(dj - dj_fd) / abs(dj_fd))
+    print("adjoint vs finite differences, 20 directions: worst relative error %.2e" % worst)
+    assert worst < 1e-4
+
+
 def test_error_paths(hz):
     """Apply before factor, bad sizes and a non-convergent solve return errors."""
     d, c = s7_case(hz)

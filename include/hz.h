@@ -133,7 +133,8 @@ enum {
   HZ_SOLVE_STREAM = 1,  /* factor tiles streamed through an LDS ring, up to 8 RHS per pass */
   HZ_SOLVE_CHUNKED = 2, /* chunked GEMV kernel, any width */
   HZ_SOLVE_STREAM_1 = 3, /* single-RHS ring with an LDS cross-group reduction (200-wide planes) */
-  HZ_SOLVE_STREAM_MFMA = 4 /* streaming solve with the 8-RHS block product on FP32 MFMA (complex64) */
+  HZ_SOLVE_STREAM_MFMA = 4, /* streaming solve with the 8-RHS block product on FP32 MFMA (complex64) */
+  HZ_SOLVE_SYM = 5          /* symmetric packed factors (upper block triangle of G_j D_j), one RHS per pass */
 };
 typedef struct {
   int32_t solve_kernel; /* HZ_SOLVE_* */

@@ -70,6 +70,10 @@ struct Factors {
   bool ready = false;
   DevBuf G;        // nz * ny * nx * nx complex, [k][j] blocks, column-major nx x nx
   size_t elem = 0; // bytes per complex element
+  // symmetric packed form of G_j D_j (upper block triangle of 8 x 8 tiles,
+  // block columns paired J / T-1-J), for the widths it is built for
+  bool sym_ready = false;
+  DevBuf Gsym;
 };
 
 }  // namespace hz
@@ -95,6 +99,8 @@ struct hz_problem {
   hz::DevBuf d_dL;       // double2[N] delta-L (REF)
   hz::DevBuf d_dLc;      // double2[N] conj(delta-L) (REF adjoint)
   hz::DevBuf d_sig;      // double2[ny] Schur coupling cm_y(j) cp_y(j-1)
+  std::vector<double2> symd[2];  // symmetrising scalings dx (nx), dy (ny)
+  hz::DevBuf d_symd;     // double2: dx[nx] | dy[ny] | 1/dx[nx] | 1/dy[ny]
 
   hz::FftPlan fft;
   hz::Factors fac[2];
@@ -187,6 +193,8 @@ void launch_plane_solve_with(hz_problem* p, int prec, int nrhs, void* x, bool ad
                              hipStream_t s);
 // whether the streaming multi-RHS solve is instantiated for this width/precision
 bool stream_solve_available(int prec, int nx);
+// whether the symmetric packed streaming kernel is instantiated for this width/precision
+bool sym_solve_available(int prec, int nx);
 // whether the 8-RHS MFMA streaming kernel is instantiated for this width/precision
 bool mfma_solve_available(int prec, int nx);
 // whether the single-RHS ring kernel is instantiated for this width/precision

@@ -221,6 +221,28 @@ void setup_tables(hz_problem* p, const double* c) {
   p->d_sig.alloc(p->ny * sizeof(double2));
   HZ_HIP(hipMemcpy(p->d_sig.ptr, sig.data(), p->ny * sizeof(double2), hipMemcpyHostToDevice));
 
+  // Diagonal scalings D = diag(dx(i) dy(j)) with D^{-1} M_k complex symmetric:
+  // d(i) / d(i-1) = cm(i) / cp(i-1) per axis (the reference's symmetry check,
+  // Tests.cpp:686-757), so the block inverses G_j D_j are symmetric.
+  {
+    std::vector<double2> sd;
+    for (int a = 0; a < 2; ++a) {
+      const int64_t n = d.n[a];
+      std::vector<zc> dd(n, zc(1.0, 0.0));
+      for (int64_t i = 1; i < n; ++i) {
+        const zc den = zz(p->cp[a][i - 1]);
+        dd[i] = std::abs(den) > 0.0 ? dd[i - 1] * zz(p->cm[a][i]) / den : dd[i - 1];
+      }
+      p->symd[a].resize(n);
+      for (int64_t i = 0; i < n; ++i) p->symd[a][i] = d2(dd[i]);
+    }
+    for (int a = 0; a < 2; ++a) sd.insert(sd.end(), p->symd[a].begin(), p->symd[a].end());
+    for (int a = 0; a < 2; ++a)
+      for (const double2& v : p->symd[a]) sd.push_back(d2(1.0 / zz(v)));
+    p->d_symd.alloc(sd.size() * sizeof(double2));
+    HZ_HIP(hipMemcpy(p->d_symd.ptr, sd.data(), sd.size() * sizeof(double2), hipMemcpyHostToDevice));
+  }
+
   if (d.mode == HZ_MODE_S7) {
     std::vector<double> mass(p->N);
     double w2 = d.omega * d.omega;

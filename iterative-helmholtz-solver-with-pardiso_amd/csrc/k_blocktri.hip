@@ -1157,6 +1157,407 @@ bool launch_mfma8_any(int n, int ny, int nz, int groups, hipStream_t s, const vo
          try_launch_mfma8<64, 32>(n, ny, nz, groups, s, G, x, cmy, cpy, n2, N);
 }
 
+// ------------------------------------------ symmetric packed plane solve
+// D_j^{-1} M_k is complex symmetric for D_j = diag(dx(i) dy(j)), hence so is
+// K_j = G_j D_j, and G_j v = K_j (D_j^{-1} v).  The packed form keeps the upper
+// block triangle of K_j in TB x TB tiles, block column J holding rows
+// [0, (J+1) TB) column-major, with block columns J and T-1-J stored next to
+// each other so that every pair (one DMA tile) has the same size (T+1) TB^2:
+// 52% of the full block at T = 25.  A stored element K(r, c) feeds y_r += K v'_c
+// and, below the diagonal tile, also y_c += K v'_r; the second product is a
+// dot over the rows of column c, reduced across the wave with xor shuffles and
+// accumulated per wave in LDS.  v' = D_j^{-1} v is formed when the next vector
+// is written.
+template <int NX, int TB>
+struct SymLayout {
+  static constexpr int T = NX / TB;                     // block columns
+  static constexpr int NPAIR = T / 2;                   // pair tiles
+  static constexpr int TPS = (T + 1) / 2;               // tiles per block (with the middle one when T is odd)
+  static constexpr int PAIR = (T + 1) * TB * TB;        // complex per pair tile
+  static constexpr int BS = TB * TB * T * (T + 1) / 2;  // complex per packed block
+  static_assert(NX % TB == 0, "tile split");
+};
+
+// Gsym[(k ny + j) BS + ...] = packed G_j D_j of plane k
+template <typename C, int NX, int TB>
+__global__ __launch_bounds__(256) void k_pack_sym(const C* __restrict__ G, C* __restrict__ P,
+                                                  const double2* __restrict__ dx, const double2* __restrict__ dy,
+                                                  int ny) {
+  typedef SymLayout<NX, TB> L;
+  const int blk = blockIdx.y;
+  const int j = blk % ny;
+  const int t = blockIdx.x;
+  const int J1 = t, J2 = L::T - 1 - t;
+  const bool two = J2 != J1;
+  const int H1 = (J1 + 1) * TB, H2 = (J2 + 1) * TB;
+  const int sz = H1 * TB + (two ? H2 * TB : 0);
+  const C* Gb = G + (size_t)blk * NX * NX;
+  C* Pb = P + (size_t)blk * L::BS + (size_t)t * L::PAIR;
+  const double2 dyj = dy[j];
+  for (int e = threadIdx.x; e < sz; e += blockDim.x) {
+    int J, cl, r;
+    if (e < H1 * TB) {
+      J = J1; cl = e / H1; r = e - cl * H1;
+    } else {
+      const int e2 = e - H1 * TB;
+      J = J2; cl = e2 / H2; r = e2 - cl * H2;
+    }
+    const int c = J * TB + cl;
+    const C g = Gb[(size_t)c * NX + r];
+    const double2 dc = dx[c];
+    const double sx = dc.x * dyj.x - dc.y * dyj.y, sy = dc.x * dyj.y + dc.y * dyj.x;
+    C o;
+    o.x = (double)g.x * sx - (double)g.y * sy;
+    o.y = (double)g.x * sy + (double)g.y * sx;
+    Pb[e] = o;
+  }
+}
+
+// Sum of a float over each 16-lane row of a wave (every lane of the row gets
+// it): DPP adds with quad xor 1, quad xor 2, half-row mirror and row mirror.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float row16_sum(float x) {
+  x += dpp_f<0xB1>(x);   // quad_perm [1,0,3,2]
+  x += dpp_f<0x4E>(x);   // quad_perm [2,3,0,1]
+  x += dpp_f<0x141>(x);  // row_half_mirror
+  x += dpp_f<0x140>(x);  // row_mirror
+  return x;
+}
+
+template <typename T, int RPL, int NCG, int TB, int NX>
+struct SymShape {
+  typedef SymLayout<NX, TB> L;
+  typedef typename Cx<T>::t C;
+  static constexpr int S = 8;                               // ring slots
+  static constexpr int n = NX, nru = n / RPL;
+  static constexpr int RUP = dma::NT / NCG;                 // thread slots per column group
+  static constexpr int CPB = TB / NCG;                      // columns per thread per block column
+  static constexpr int units = L::PAIR / RPL;               // 16-byte units per pair tile
+  static constexpr int R = (units + dma::NT - 1) / dma::NT;
+  static constexpr int PU = n * (int)sizeof(C) / 16;        // units of one y-line
+  static constexpr size_t lds = (size_t)S * R * dma::NT * 16 + (size_t)NCG * n * sizeof(C) +
+                                4 * (size_t)n * sizeof(C) + 3 * 64 * 16;
+  static_assert(nru <= RUP && RUP % 64 == 0 && TB % NCG == 0 && TB % RPL == 0, "thread split");
+  static_assert(2 * TB * 32 == dma::NT, "transposed jobs: 32 lanes per column of a tile");
+  static_assert(PU <= 128, "line DMA of two waves");
+  static_assert((S - 1) * R <= 63, "vmcnt range");
+  static_assert(L::TPS >= S - 1, "the epilogue wait assumes a step of at least S-1 tiles");
+  static_assert(lds <= 160 * 1024, "LDS budget");
+};
+
+template <typename T, int RPL, int NCG, int TB, int NX>
+__global__ __launch_bounds__(dma::NT) void k_solve_sym(const typename Cx<T>::t* __restrict__ P,
+                                                       typename Cx<T>::t* __restrict__ X,
+                                                       const double2* __restrict__ cmy,
+                                                       const double2* __restrict__ cpy,
+                                                       const double2* __restrict__ symd, int ny, int64_t n2,
+                                                       int64_t N) {
+  typedef SymShape<T, RPL, NCG, TB, NX> S_;
+  typedef SymLayout<NX, TB> L;
+  typedef typename Cx<T>::t C;
+  typedef typename Unit<T>::t U;
+  constexpr int n = S_::n, nru = S_::nru, RUP = S_::RUP, CPB = S_::CPB, R = S_::R, PU = S_::PU;
+  constexpr int TPS = L::TPS, NPAIR = L::NPAIR, PAIR = L::PAIR, BS = L::BS, T_ = L::T;
+  constexpr int TIT = ((T_ - 1) * TB / RPL + 31) / 32;  // transposed-job iterations (max)
+  __shared__ U s0[R * dma::NT], s1[R * dma::NT], s2[R * dma::NT], s3[R * dma::NT];
+  __shared__ U s4[R * dma::NT], s5[R * dma::NT], s6[R * dma::NT], s7[R * dma::NT];
+  __shared__ C part[NCG * n];
+  __shared__ C tdot[2 * n];  // transposed products y_c of the current step, two half sums
+  __shared__ C v[n];
+  // next-line operand (waves 0-1) and the step's coefficients (wave 2: cm_y, cp_y,
+  // 1/dy), all fetched by DMA at the start of the step so that the epilogue
+  // issues no vector loads behind the tiles in flight
+  __shared__ U pbuf[3 * 64];
+  __shared__ C sdxi[n];  // 1/dx
+  const C* pline = reinterpret_cast<const C*>(pbuf);
+  const double2* pscal = reinterpret_cast<const double2*>(&pbuf[128]);
+
+  const int k = blockIdx.x;
+  const int64_t rhs = blockIdx.y;
+  C* x = X + rhs * N + (int64_t)k * n2;
+  const C* Pk = P + (size_t)k * ny * BS;
+  const double2* dxi = symd + n + ny;
+  const double2* dyi = symd + 2 * n + ny;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6;
+  const int cg = tid / RUP, ru = tid - cg * RUP;
+  const bool worker = ru < nru;
+  const int nsteps = 2 * ny - 1;
+  const int TT = nsteps * TPS;
+  auto blk_of = [&](int s) { return s <= ny - 1 ? s : 2 * ny - 2 - s; };
+  // v' = v D_j^{-1}: the scaling of row r for block j (dyi[j] is wave-uniform)
+  auto dinv = [&](int r, int j) {
+    const C a = sdxi[r];
+    const double2 b = dyi[j];
+    return make_double2((double)a.x * b.x - (double)a.y * b.y, (double)a.x * b.y + (double)a.y * b.x);
+  };
+
+#define HZ_SY_TILE(TC_, SLOT)                                                                         \
+  {                                                                                                   \
+    const int tc_ = (TC_) >= TT ? 0 : (TC_); /* past the end: re-read tile 0 */                      \
+    const int s_ = tc_ / TPS, t_ = tc_ - s_ * TPS;                                                    \
+    const int tu_ = (t_ < NPAIR ? PAIR : (NPAIR + 1) * TB * TB) / RPL;                                \
+    const unsigned char* base_ =                                                                      \
+        reinterpret_cast<const unsigned char*>(Pk + (size_t)blk_of(s_) * BS + (size_t)t_ * PAIR);     \
+    _Pragma("unroll") for (int r_ = 0; r_ < R; ++r_) {                                                \
+      const int p_ = r_ * dma::NT + tid;                                                              \
+      const int u_ = p_ < tu_ ? p_ : tu_ - 1;                                                         \
+      __builtin_amdgcn_global_load_lds((const void*)(base_ + (size_t)u_ * 16),                        \
+                                       (HZ_LDS void*)(&SLOT[r_ * dma::NT + wv * 64]), 16, 0, 0);      \
+    }                                                                                                 \
+  }
+#define HZ_SY_LINE(J_, SE_)                                                                             \
+  {                                                                                                     \
+    if (wv < 2) {                                                                                       \
+      const unsigned char* base_ = reinterpret_cast<const unsigned char*>(x + (size_t)n * (J_));        \
+      const int u_ = min(tid, PU - 1);                                                                  \
+      __builtin_amdgcn_global_load_lds((const void*)(base_ + (size_t)u_ * 16),                          \
+                                       (HZ_LDS void*)(&pbuf[wv * 64]), 16, 0, 0);                       \
+    } else if (wv == 2) {                                                                               \
+      const int sn_ = min((SE_) + 1, nsteps - 1);                                                       \
+      const int jb_ = min(max(2 * ny - 3 - (SE_), 0), ny - 1);                                          \
+      const double2* src_ = lane == 1 ? cpy + jb_ : (lane == 2 ? dyi + blk_of(sn_) : cmy + min((SE_) + 1, ny - 1)); \
+      __builtin_amdgcn_global_load_lds((const void*)src_, (HZ_LDS void*)(&pbuf[128]), 16, 0, 0);        \
+    }                                                                                                   \
+  }
+  T acc[2 * RPL];
+#define HZ_SY_EPILOGUE(S_STEP)                                                                     \
+  {                                                                                                \
+    const int se_ = (S_STEP);                                                                      \
+    if (worker) {                                                                                  \
+      _Pragma("unroll") for (int r = 0; r < RPL; ++r) {                                            \
+        C o;                                                                                       \
+        o.x = acc[2 * r];                                                                          \
+        o.y = acc[2 * r + 1];                                                                      \
+        part[cg * n + ru * RPL + r] = o;                                                           \
+      }                                                                                            \
+    }                                                                                              \
+    __builtin_amdgcn_s_waitcnt(dma::wait_vm((S_::S - 1) * R));                                     \
+    __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);                                                   \
+    dma::raw_barrier();                                                                            \
+    if (tid < n) {                                                                                 \
+      C y;                                                                                         \
+      y.x = 0; y.y = 0;                                                                            \
+      _Pragma("unroll") for (int q = 0; q < NCG; ++q) {                                            \
+        const C u = part[q * n + tid];                                                             \
+        y.x += u.x; y.y += u.y;                                                                    \
+      }                                                                                            \
+      {                                                                                            \
+        const C u0 = tdot[tid], u1 = tdot[n + tid];                                                \
+        y.x += u0.x + u1.x; y.y += u0.y + u1.y;                                                    \
+      }                                                                                            \
+      const C pre = pline[tid];                                                                    \
+      C nv;                                                                                        \
+      if (se_ < ny - 1) {                                                                          \
+        const double2 cm = pscal[0];                                                               \
+        nv.x = pre.x - (T)(cm.x * y.x - cm.y * y.y);                                               \
+        nv.y = pre.y - (T)(cm.x * y.y + cm.y * y.x);                                               \
+        x[(size_t)n * (se_ + 1) + tid] = nv;                                                       \
+      } else {                                                                                     \
+        const int j = 2 * ny - 2 - se_;                                                            \
+        x[(size_t)n * j + tid] = y;                                                                \
+        if (j > 0) {                                                                               \
+          const double2 cp = pscal[1];                                                             \
+          nv.x = pre.x - (T)(cp.x * y.x - cp.y * y.y);                                             \
+          nv.y = pre.y - (T)(cp.x * y.y + cp.y * y.x);                                             \
+        } else {                                                                                   \
+          nv = y;                                                                                  \
+        }                                                                                          \
+      }                                                                                            \
+      if (se_ + 1 < nsteps) {                                                                      \
+        const C a_ = sdxi[tid];                                                                    \
+        const double2 b_ = pscal[2];                                                               \
+        const double2 di = make_double2((double)a_.x * b_.x - (double)a_.y * b_.y,                 \
+                                        (double)a_.x * b_.y + (double)a_.y * b_.x);                \
+        C sv;                                                                                      \
+        sv.x = (T)((double)nv.x * di.x - (double)nv.y * di.y);                                     \
+        sv.y = (T)((double)nv.x * di.y + (double)nv.y * di.x);                                     \
+        v[tid] = sv;                                                                               \
+      }                                                                                            \
+    }                                                                                              \
+    __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);                                                   \
+  }
+#define HZ_SY_BLOCKCOL(SLOT, J_, UBASE_)                                                           \
+  {                                                                                                \
+    const int Jb_ = (J_);                                                                          \
+    const int Hu_ = (Jb_ + 1) * TB / RPL; /* units per stored column */                           \
+    if (worker && ru < Hu_) {                                                                      \
+      _Pragma("unroll") for (int cc = 0; cc < CPB; ++cc) {                                         \
+        const int cl = cg * CPB + cc;                                                              \
+        unit_fma<T, RPL>(SLOT[(UBASE_) + cl * Hu_ + ru], v[Jb_ * TB + cl], acc);                   \
+      }                                                                                            \
+    }                                                                                              \
+  }
+  // transposed products of the tile's off-diagonal rows: job (column, chunk),
+  // 32 chunks per column (all 512 threads for the tile's 16 columns); each
+  // 16-lane DPP row sums its chunks and its first lane stores one half of y_c
+  // (every column of a block gets exactly one value per half per step)
+#define HZ_SY_TRANSPOSED(SLOT, J1_, J2_)                                                           \
+  {                                                                                                \
+    const int job_ = tid >> 5, ch_ = tid & 31;                                                     \
+    const int bsel_ = job_ / TB, cl_ = job_ - bsel_ * TB;                                          \
+    if (bsel_ == 0 || (J2_) != (J1_)) {                                                            \
+      const int Jb_ = bsel_ == 0 ? (J1_) : (J2_);                                                  \
+      const int Hu_ = (Jb_ + 1) * TB / RPL;                                                        \
+      const int ub_ = bsel_ == 0 ? 0 : ((J1_) + 1) * TB * TB / RPL;                                \
+      const int nu_ = Jb_ * TB / RPL; /* off-diagonal units of the column */                      \
+      T d0_ = 0, d1_ = 0;                                                                          \
+      U a_[TIT];                                                                                   \
+      U vv_[TIT];                                                                                  \
+      _Pragma("unroll") for (int it_ = 0; it_ < TIT; ++it_) {                                      \
+        const int u_ = ch_ + 32 * it_;                                                             \
+        const int uc_ = u_ < nu_ ? u_ : 0;                                                         \
+        a_[it_] = SLOT[ub_ + cl_ * Hu_ + uc_];                                                     \
+        vv_[it_] = reinterpret_cast<const U*>(v)[uc_];                                             \
+      }                                                                                            \
+      _Pragma("unroll") for (int it_ = 0; it_ < TIT; ++it_) {                                      \
+        if (ch_ + 32 * it_ < nu_) {                                                                \
+          const C* ag_ = reinterpret_cast<const C*>(&a_[it_]);                                     \
+          const C* vg_ = reinterpret_cast<const C*>(&vv_[it_]);                                    \
+          _Pragma("unroll") for (int i_ = 0; i_ < RPL; ++i_) {                                     \
+            d0_ += ag_[i_].x * vg_[i_].x - ag_[i_].y * vg_[i_].y;                                  \
+            d1_ += ag_[i_].x * vg_[i_].y + ag_[i_].y * vg_[i_].x;                                  \
+          }                                                                                        \
+        }                                                                                          \
+      }                                                                                            \
+      d0_ = row16_sum(d0_);                                                                        \
+      d1_ = row16_sum(d1_);                                                                        \
+      if ((ch_ & 15) == 0) {                                                                       \
+        C o_;                                                                                      \
+        o_.x = d0_;                                                                                \
+        o_.y = d1_;                                                                                \
+        tdot[(ch_ >> 4) * n + Jb_ * TB + cl_] = o_;                                                \
+      }                                                                                            \
+    }                                                                                              \
+  }
+#define HZ_SY_STAGE(TC_, SLOT, NEXT)                                                      \
+  {                                                                                       \
+    const int Tc = (TC_);                                                                 \
+    const int s = Tc / TPS, t = Tc - s * TPS;                                             \
+    __builtin_amdgcn_s_waitcnt(dma::wait_vm((S_::S - 2) * R));                            \
+    __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);                                          \
+    dma::raw_barrier();                                                                   \
+    if (Tc < TT && t == 0) {                                                              \
+      _Pragma("unroll") for (int i = 0; i < 2 * RPL; ++i) acc[i] = 0;                     \
+      int slot_line = 0;                                                                  \
+      if (s < ny - 1) slot_line = s + 1;                                                  \
+      else if (2 * ny - 2 - s > 0) slot_line = 2 * ny - 3 - s;                            \
+      HZ_SY_LINE(slot_line, s)                                                            \
+    }                                                                                     \
+    if (Tc < TT) {                                                                        \
+      const int J1 = t, J2 = T_ - 1 - t;                                                  \
+      HZ_SY_BLOCKCOL(SLOT, J1, 0)                                                         \
+      if (J2 != J1) HZ_SY_BLOCKCOL(SLOT, J2, (J1 + 1) * TB * TB / RPL)                    \
+      HZ_SY_TRANSPOSED(SLOT, J1, J2)                                                      \
+    }                                                                                     \
+    HZ_SY_TILE(Tc + S_::S - 1, NEXT)                                                      \
+    if (Tc < TT && t == TPS - 1) HZ_SY_EPILOGUE(s)                                        \
+  }
+
+  for (int i = tid; i < 2 * n; i += dma::NT) {
+    C z;
+    z.x = 0; z.y = 0;
+    tdot[i] = z;
+  }
+  for (int i = tid; i < n; i += dma::NT) {
+    const double2 a = dxi[i];
+    C c;
+    c.x = (T)a.x;
+    c.y = (T)a.y;
+    sdxi[i] = c;
+  }
+  __syncthreads();
+  for (int i = tid; i < n; i += dma::NT) {
+    const C x0 = x[i];
+    const double2 di = dinv(i, blk_of(0));
+    C sv;
+    sv.x = (T)((double)x0.x * di.x - (double)x0.y * di.y);
+    sv.y = (T)((double)x0.x * di.y + (double)x0.y * di.x);
+    v[i] = sv;
+  }
+  __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
+  HZ_SY_TILE(0, s0)
+  HZ_SY_TILE(1, s1)
+  HZ_SY_TILE(2, s2)
+  HZ_SY_TILE(3, s3)
+  HZ_SY_TILE(4, s4)
+  HZ_SY_TILE(5, s5)
+  HZ_SY_TILE(6, s6)
+  for (int T0 = 0; T0 < TT; T0 += 8) {
+    HZ_SY_STAGE(T0, s0, s7)
+    HZ_SY_STAGE(T0 + 1, s1, s0)
+    HZ_SY_STAGE(T0 + 2, s2, s1)
+    HZ_SY_STAGE(T0 + 3, s3, s2)
+    HZ_SY_STAGE(T0 + 4, s4, s3)
+    HZ_SY_STAGE(T0 + 5, s5, s4)
+    HZ_SY_STAGE(T0 + 6, s6, s5)
+    HZ_SY_STAGE(T0 + 7, s7, s6)
+  }
+#undef HZ_SY_STAGE
+#undef HZ_SY_TRANSPOSED
+#undef HZ_SY_BLOCKCOL
+#undef HZ_SY_EPILOGUE
+#undef HZ_SY_LINE
+#undef HZ_SY_TILE
+  __builtin_amdgcn_s_waitcnt(dma::wait_vm(0));
+}
+
+template <typename T, int RPL, int NCG, int TB, int NX>
+bool try_pack_sym(hz_problem* p, int prec, hipStream_t s) {
+  if (p->nx != NX) return false;
+  typedef typename Cx<T>::t C;
+  typedef SymLayout<NX, TB> L;
+  Factors& f = p->fac[prec];
+  const size_t bytes = (size_t)p->nz * p->ny * L::BS * sizeof(C);
+  if (f.Gsym.bytes != bytes) {
+    f.Gsym.release();
+    f.Gsym.alloc(bytes);
+  }
+  const double2* dx = p->d_symd.as<double2>();
+  const double2* dy = dx + NX;
+  hipLaunchKernelGGL((k_pack_sym<C, NX, TB>), dim3(L::TPS, (unsigned)(p->nz * p->ny)), dim3(256), 0, s, f.G.as<C>(),
+                     f.Gsym.as<C>(), dx, dy, (int)p->ny);
+  HZ_HIP(hipGetLastError());
+  return true;
+}
+
+template <typename T, int RPL, int NCG, int TB, int NX>
+bool try_solve_sym(hz_problem* p, int prec, int nrhs, void* x, hipStream_t s) {
+  if (p->nx != NX) return false;
+  typedef typename Cx<T>::t C;
+  const double2* cmy = p->tab(1, 0);
+  const double2* cpy = p->tab(1, 1);
+  hipLaunchKernelGGL((k_solve_sym<T, RPL, NCG, TB, NX>), dim3((unsigned)p->nz, (unsigned)nrhs), dim3(dma::NT), 0, s,
+                     p->fac[prec].Gsym.as<C>(), (C*)x, cmy, cpy, p->d_symd.as<double2>(), (int)p->ny, p->n2, p->N);
+  HZ_HIP(hipGetLastError());
+  return true;
+}
+
+#define HZ_SYM_WIDTHS(X)                 \
+  X(HZ_PREC_C64, float, 2, 4, 8, 200)    \
+  X(HZ_PREC_C64, float, 2, 4, 8, 192)    \
+  X(HZ_PREC_C64, float, 2, 4, 8, 160)    \
+  X(HZ_PREC_C64, float, 2, 4, 8, 128)
+
+bool pack_sym_any(hz_problem* p, int prec, hipStream_t s) {
+#define HZ_SYM_PACK(P, T, RPL, NCG, TB, NX) \
+  if (prec == P && try_pack_sym<T, RPL, NCG, TB, NX>(p, prec, s)) return true;
+  HZ_SYM_WIDTHS(HZ_SYM_PACK)
+#undef HZ_SYM_PACK
+  return false;
+}
+
+bool solve_sym_any(hz_problem* p, int prec, int nrhs, void* x, hipStream_t s) {
+#define HZ_SYM_SOLVE(P, T, RPL, NCG, TB, NX) \
+  if (prec == P && try_solve_sym<T, RPL, NCG, TB, NX>(p, prec, nrhs, x, s)) return true;
+  HZ_SYM_WIDTHS(HZ_SYM_SOLVE)
+#undef HZ_SYM_SOLVE
+  return false;
+}
+
 // Launch of one right-hand-side group size NR over nrhs_g * NR vectors.
 template <typename T, int RPL, int NCG, int TCOL, int NX, int NR>
 void launch_mr(int ny, int nz, int groups, hipStream_t s, const void* G, void* x, const double2* cmy,
@@ -1195,9 +1596,9 @@ bool try_launch_mr(int n, int ny, int nz, int nrhs, int cap, hipStream_t s, cons
   return true;
 }
 
-// Plane-solve kernel family: HZ_SOLVE_KERNEL=mr|dma|fast|mfma forces one for
+// Plane-solve kernel family: HZ_SOLVE_KERNEL=mr|dma|fast|mfma|sym forces one for
 // timing; otherwise the tuned (or default) choice applies.
-enum SolveKernel { SK_DEFAULT, SK_MR, SK_DMA, SK_FAST, SK_MFMA };
+enum SolveKernel { SK_DEFAULT, SK_MR, SK_DMA, SK_FAST, SK_MFMA, SK_SYM };
 SolveKernel solve_kernel_choice() {
   const char* v = getenv("HZ_SOLVE_KERNEL");
   if (!v || !v[0]) return SK_DEFAULT;
@@ -1205,6 +1606,7 @@ SolveKernel solve_kernel_choice() {
   if (!strcmp(v, "dma")) return SK_DMA;
   if (!strcmp(v, "fast")) return SK_FAST;
   if (!strcmp(v, "mfma")) return SK_MFMA;
+  if (!strcmp(v, "sym")) return SK_SYM;
   return SK_DEFAULT;
 }
 
@@ -1235,6 +1637,10 @@ bool launch_mr_any(int prec, int n, int ny, int nz, int nrhs, int cap, hipStream
 }
 
 }  // namespace
+
+bool sym_solve_available(int prec, int nx) {
+  return prec == HZ_PREC_C64 && (nx == 200 || nx == 192 || nx == 160 || nx == 128);
+}
 
 bool mfma_solve_available(int prec, int nx) {
   return prec == HZ_PREC_C64 && (nx == 200 || nx == 192 || nx == 160 || nx == 128 || nx == 96 || nx == 64);
@@ -1309,6 +1715,12 @@ void launch_factor(hz_problem* p, int prec, hipStream_t s) {
   HZ_HIP(hipMemcpyAsync(&hs, st.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
   HZ_HIP(hipStreamSynchronize(s));
   HZ_CHECK(hs == 0, HZ_ERR_SINGULAR, "zero pivot while factoring a plane");
+  // symmetric packed copy for the widths the packed solve is instantiated for
+  f.sym_ready = false;
+  if (sym_solve_available(prec, n) && !getenv("HZ_NO_SYM_PACK")) {
+    f.sym_ready = pack_sym_any(p, prec, s);
+    HZ_HIP(hipStreamSynchronize(s));
+  }
   f.ready = true;
 }
 
@@ -1331,6 +1743,13 @@ void launch_plane_solve_with(hz_problem* p, int prec, int nrhs, void* x, bool ad
   if (sk == SK_DEFAULT && t.solve_family == HZ_SOLVE_CHUNKED) sk = SK_FAST;
   if (sk == SK_DEFAULT && t.solve_family == HZ_SOLVE_STREAM_1) sk = SK_DMA;
   if (sk == SK_DEFAULT && t.solve_family == HZ_SOLVE_STREAM_MFMA) sk = SK_MFMA;
+  if (sk == SK_DEFAULT && t.solve_family == HZ_SOLVE_SYM) sk = SK_SYM;
+  // untuned default for one right-hand side: the symmetric packed stream where it is built
+  if (sk == SK_DEFAULT && t.solve_family == HZ_SOLVE_DEFAULT && nrhs == 1 && f.sym_ready) sk = SK_SYM;
+  if (!adjoint && ny >= 1 && sk == SK_SYM && f.sym_ready && reinterpret_cast<uintptr_t>(x) % 16 == 0 &&
+      solve_sym_any(p, prec, nrhs, x, s))
+    return;
+  if (sk == SK_SYM) sk = SK_DEFAULT;
 
   // untuned defaults: complex64 groups of 8 on the matrix cores, one
   // right-hand side on the single-RHS ring where they are instantiated

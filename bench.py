@@ -127,8 +127,9 @@ def main():
 
     # ---- roofline of the dominant kernel: the complex64 plane solve, timed with
     # HIP events on the problem's stream.  Algorithmic bytes per launch: the
-    # 2 ny - 1 block reads of nx^2 complex64 per plane plus four passes over the
-    # plane vectors (read b, write t, read t, write x).
+    # 2 ny - 1 block reads per plane (nx^2 complex64, or the packed upper block
+    # triangle for the symmetric kernel) plus four passes over the plane vectors
+    # (read b, write t, read t, write x).
     nx, ny, nz = d.n
     N = d.N
     buf = torch.randn(N, dtype=torch.complex64, device="cuda")
@@ -141,7 +142,8 @@ def main():
     e1.record(stream)
     torch.cuda.synchronize()
     ms_kernel = e0.elapsed_time(e1) / reps
-    bytes_kernel = (2 * ny - 1) * nx * nx * 8 * nz + 4 * 8 * N
+    fam = tuning.get("solve_kernel", 0)
+    bytes_kernel = plane_solve_bytes(d.n, fam)
     achieved = bytes_kernel / (ms_kernel * 1e-3) / 1e9
 
     out = {
@@ -174,7 +176,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(n),
+            "traffic": measured_traffic(n, fam),
             "bytes_per_launch": bytes_kernel,
             "ms_per_launch": round(ms_kernel, 4),
         },
@@ -203,6 +205,8 @@ def main():
 def plane_solve_kernel(nx, tuning):
     """Name of the complex64 single-RHS plane-solve kernel the tuned choice launches."""
     fam = tuning.get("solve_kernel", 0)
+    if fam == 5:
+        return "k_solve_sym<float,2,4,8,%d>" % nx
     if fam == 3:
         return "k_solve_dma<float,2,5,20,200>"
     if fam == 1:
@@ -382,12 +386,27 @@ def bench_adjoint(a, hz, np, torch):
     print(json.dumps(out), flush=True)
 
 
-def measured_traffic(n):
+def plane_solve_bytes(dims, fam):
+    """Algorithmic HBM bytes of one complex64 single-RHS plane solve: 2 ny - 1
+    block reads per plane (the full nx^2 block, or for the symmetric kernel the
+    upper block triangle of 8x8 tiles, 64 T (T+1)/2 entries with T = nx/8) and
+    four passes over the nx ny nz vector."""
+    nx, ny, nz = dims
+    if fam == 5:
+        t = nx // 8
+        blk = 64 * t * (t + 1) // 2
+    else:
+        blk = nx * nx
+    return (2 * ny - 1) * blk * 8 * nz + 4 * 8 * nx * ny * nz
+
+
+def measured_traffic(n, fam):
     """HBM bytes per launch of the plane solve from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE passes (scaled by a 1 GiB copy), for the 200^3 grid."""
-    if n != 200:
+    names = {3: "k_solve_dma_c64_200cube_pmc.md", 5: "k_solve_sym_c64_200cube_pmc.md"}
+    if n != 200 or fam not in names:
         return None
-    path = os.path.join(ROOT, "profiles", "plane_solve_c64_200cube_pmc.md")
+    path = os.path.join(ROOT, "profiles", names[fam])
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
@@ -404,12 +423,21 @@ def kernel_only(S, d, A, torch):
     dst.copy_(src)
     torch.cuda.synchronize()
     buf = torch.randn(d.N, dtype=torch.complex64, device="cuda")
-    for _ in range(5):
-        A.check(A.lib().hz_plane_solve(S.h, A.PREC_C64, 0, 1, buf.data_ptr()), "hz_plane_solve")
-    A.check(A.lib().hz_problem_sync(S.h), "sync")
-    nx, ny, nz = d.n
+    # 5 launches of each single-RHS kernel that fits the grid: symmetric packed
+    # (family 5), then the full-block LDS ring (family 3, 200-wide planes)
+    fams = [5, 3] if d.n[0] == 200 else [5]
+    saved = os.environ.get("HZ_SOLVE_KERNEL")
+    for fam in fams:
+        os.environ["HZ_SOLVE_KERNEL"] = {5: "sym", 3: "dma"}[fam]
+        for _ in range(5):
+            A.check(A.lib().hz_plane_solve(S.h, A.PREC_C64, 0, 1, buf.data_ptr()), "hz_plane_solve")
+        A.check(A.lib().hz_problem_sync(S.h), "sync")
+    if saved is None:
+        os.environ.pop("HZ_SOLVE_KERNEL", None)
+    else:
+        os.environ["HZ_SOLVE_KERNEL"] = saved
     print(json.dumps({"kernel_only": True, "copy_bytes": 2 * (1 << 30),
-                      "plane_solve_bytes": (2 * ny - 1) * nx * nx * 8 * nz + 4 * 8 * d.N}), flush=True)
+                      "plane_solve_bytes": {str(f): plane_solve_bytes(d.n, f) for f in fams}}), flush=True)
 
 
 def cpu_baseline(d, c, iters, m, threads):

@@ -1239,9 +1239,10 @@ struct SymShape {
   static constexpr int R = (units + dma::NT - 1) / dma::NT;
   static constexpr int PU = n * (int)sizeof(C) / 16;        // units of one y-line
   static constexpr size_t lds = (size_t)S * R * dma::NT * 16 + (size_t)NCG * n * sizeof(C) +
-                                4 * (size_t)n * sizeof(C) + 3 * 64 * 16;
+                                6 * (size_t)n * sizeof(C) + 3 * 64 * 16;
   static_assert(nru <= RUP && RUP % 64 == 0 && TB % NCG == 0 && TB % RPL == 0, "thread split");
-  static_assert(2 * TB * 32 == dma::NT, "transposed jobs: 32 lanes per column of a tile");
+  static_assert(TB * 64 == dma::NT, "transposed products: one wave per column of a tile");
+  static_assert(RUP % 64 == 0 && (dma::NT / 64) % NCG == 0, "row units: whole waves per column group");
   static_assert(PU <= 128, "line DMA of two waves");
   static_assert((S - 1) * R <= 63, "vmcnt range");
   static_assert(L::TPS >= S - 1, "the epilogue wait assumes a step of at least S-1 tiles");
@@ -1261,11 +1262,11 @@ __global__ __launch_bounds__(dma::NT) void k_solve_sym(const typename Cx<T>::t* 
   typedef typename Unit<T>::t U;
   constexpr int n = S_::n, nru = S_::nru, RUP = S_::RUP, CPB = S_::CPB, R = S_::R, PU = S_::PU;
   constexpr int TPS = L::TPS, NPAIR = L::NPAIR, PAIR = L::PAIR, BS = L::BS, T_ = L::T;
-  constexpr int TIT = ((T_ - 1) * TB / RPL + 31) / 32;  // transposed-job iterations (max)
+  constexpr int TIT = ((T_ - 1) * TB / RPL + 63) / 64;  // transposed units per lane (max)
   __shared__ U s0[R * dma::NT], s1[R * dma::NT], s2[R * dma::NT], s3[R * dma::NT];
   __shared__ U s4[R * dma::NT], s5[R * dma::NT], s6[R * dma::NT], s7[R * dma::NT];
   __shared__ C part[NCG * n];
-  __shared__ C tdot[2 * n];  // transposed products y_c of the current step, two half sums
+  __shared__ C tdot[4 * n];  // transposed products y_c of the current step, 4 partial sums
   __shared__ C v[n];
   // next-line operand (waves 0-1) and the step's coefficients (wave 2: cm_y, cp_y,
   // 1/dy), all fetched by DMA at the start of the step so that the epilogue
@@ -1283,7 +1284,11 @@ __global__ __launch_bounds__(dma::NT) void k_solve_sym(const typename Cx<T>::t* 
   const double2* dyi = symd + 2 * n + ny;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
-  const int cg = tid / RUP, ru = tid - cg * RUP;
+  // wave w: column group w % NCG, row units (w / NCG) * 64 + lane, so that the
+  // waves sharing a SIMD (w, w + 4) hold low and high rows of the same group:
+  // the direct product's work (rows 0..(J+1) TB of block column J) stays even
+  // across SIMDs
+  const int cg = wv % NCG, ru = (wv / NCG) * 64 + lane;
   const bool worker = ru < nru;
   const int nsteps = 2 * ny - 1;
   const int TT = nsteps * TPS;
@@ -1345,9 +1350,9 @@ __global__ __launch_bounds__(dma::NT) void k_solve_sym(const typename Cx<T>::t* 
         const C u = part[q * n + tid];                                                             \
         y.x += u.x; y.y += u.y;                                                                    \
       }                                                                                            \
-      {                                                                                            \
-        const C u0 = tdot[tid], u1 = tdot[n + tid];                                                \
-        y.x += u0.x + u1.x; y.y += u0.y + u1.y;                                                    \
+      _Pragma("unroll") for (int q = 0; q < 4; ++q) {                                              \
+        const C u = tdot[q * n + tid];                                                             \
+        y.x += u.x; y.y += u.y;                                                                    \
       }                                                                                            \
       const C pre = pline[tid];                                                                    \
       C nv;                                                                                        \
@@ -1391,45 +1396,55 @@ __global__ __launch_bounds__(dma::NT) void k_solve_sym(const typename Cx<T>::t* 
       }                                                                                            \
     }                                                                                              \
   }
-  // transposed products of the tile's off-diagonal rows: job (column, chunk),
-  // 32 chunks per column (all 512 threads for the tile's 16 columns); each
-  // 16-lane DPP row sums its chunks and its first lane stores one half of y_c
-  // (every column of a block gets exactly one value per half per step)
+  // transposed products of the tile's off-diagonal rows: wave c takes column c
+  // of both block columns of the pair, its lanes running over their
+  // concatenated off-diagonal units (J TB / RPL per column, (T-1) TB / RPL for
+  // the pair), so the work is the same for every tile and every wave; each
+  // 16-lane DPP row sums its lanes and its first lane stores one of 4 partial
+  // sums per output (every column of a block gets its 4 partials each step)
 #define HZ_SY_TRANSPOSED(SLOT, J1_, J2_)                                                           \
   {                                                                                                \
-    const int job_ = tid >> 5, ch_ = tid & 31;                                                     \
-    const int bsel_ = job_ / TB, cl_ = job_ - bsel_ * TB;                                          \
-    if (bsel_ == 0 || (J2_) != (J1_)) {                                                            \
-      const int Jb_ = bsel_ == 0 ? (J1_) : (J2_);                                                  \
-      const int Hu_ = (Jb_ + 1) * TB / RPL;                                                        \
-      const int ub_ = bsel_ == 0 ? 0 : ((J1_) + 1) * TB * TB / RPL;                                \
-      const int nu_ = Jb_ * TB / RPL; /* off-diagonal units of the column */                      \
-      T d0_ = 0, d1_ = 0;                                                                          \
-      U a_[TIT];                                                                                   \
-      U vv_[TIT];                                                                                  \
-      _Pragma("unroll") for (int it_ = 0; it_ < TIT; ++it_) {                                      \
-        const int u_ = ch_ + 32 * it_;                                                             \
-        const int uc_ = u_ < nu_ ? u_ : 0;                                                         \
-        a_[it_] = SLOT[ub_ + cl_ * Hu_ + uc_];                                                     \
-        vv_[it_] = reinterpret_cast<const U*>(v)[uc_];                                             \
-      }                                                                                            \
-      _Pragma("unroll") for (int it_ = 0; it_ < TIT; ++it_) {                                      \
-        if (ch_ + 32 * it_ < nu_) {                                                                \
-          const C* ag_ = reinterpret_cast<const C*>(&a_[it_]);                                     \
-          const C* vg_ = reinterpret_cast<const C*>(&vv_[it_]);                                    \
-          _Pragma("unroll") for (int i_ = 0; i_ < RPL; ++i_) {                                     \
-            d0_ += ag_[i_].x * vg_[i_].x - ag_[i_].y * vg_[i_].y;                                  \
-            d1_ += ag_[i_].x * vg_[i_].y + ag_[i_].y * vg_[i_].x;                                  \
-          }                                                                                        \
+    const int cl_ = wv;                                                                            \
+    const int Hu1_ = ((J1_) + 1) * TB / RPL, Hu2_ = ((J2_) + 1) * TB / RPL;                        \
+    const int nu1_ = (J1_) * TB / RPL;                                                             \
+    const int nu2_ = (J2_) != (J1_) ? (J2_) * TB / RPL : 0;                                        \
+    const int ub2_ = ((J1_) + 1) * TB * TB / RPL;                                                  \
+    T a0_ = 0, a1_ = 0, b0_ = 0, b1_ = 0;                                                          \
+    _Pragma("unroll") for (int it_ = 0; it_ < TIT; ++it_) {                                        \
+      const int u_ = lane + 64 * it_;                                                              \
+      if (u_ < nu1_ + nu2_) {                                                                      \
+        const bool first_ = u_ < nu1_;                                                             \
+        const int ur_ = first_ ? u_ : u_ - nu1_; /* row unit within its column */                 \
+        const U ga_ = SLOT[first_ ? cl_ * Hu1_ + ur_ : ub2_ + cl_ * Hu2_ + ur_];                    \
+        const U gv_ = reinterpret_cast<const U*>(v)[ur_];                                          \
+        const C* ag_ = reinterpret_cast<const C*>(&ga_);                                           \
+        const C* vg_ = reinterpret_cast<const C*>(&gv_);                                           \
+        T p0_ = 0, p1_ = 0;                                                                        \
+        _Pragma("unroll") for (int i_ = 0; i_ < RPL; ++i_) {                                       \
+          p0_ += ag_[i_].x * vg_[i_].x - ag_[i_].y * vg_[i_].y;                                    \
+          p1_ += ag_[i_].x * vg_[i_].y + ag_[i_].y * vg_[i_].x;                                    \
+        }                                                                                          \
+        if (first_) {                                                                              \
+          a0_ += p0_; a1_ += p1_;                                                                  \
+        } else {                                                                                   \
+          b0_ += p0_; b1_ += p1_;                                                                  \
         }                                                                                          \
       }                                                                                            \
-      d0_ = row16_sum(d0_);                                                                        \
-      d1_ = row16_sum(d1_);                                                                        \
-      if ((ch_ & 15) == 0) {                                                                       \
-        C o_;                                                                                      \
-        o_.x = d0_;                                                                                \
-        o_.y = d1_;                                                                                \
-        tdot[(ch_ >> 4) * n + Jb_ * TB + cl_] = o_;                                                \
+    }                                                                                              \
+    a0_ = row16_sum(a0_);                                                                          \
+    a1_ = row16_sum(a1_);                                                                          \
+    b0_ = row16_sum(b0_);                                                                          \
+    b1_ = row16_sum(b1_);                                                                          \
+    if ((lane & 15) == 0) {                                                                        \
+      const int q_ = lane >> 4;                                                                    \
+      C o_;                                                                                        \
+      o_.x = a0_;                                                                                  \
+      o_.y = a1_;                                                                                  \
+      tdot[q_ * n + (J1_) * TB + cl_] = o_;                                                        \
+      if ((J2_) != (J1_)) {                                                                        \
+        o_.x = b0_;                                                                                \
+        o_.y = b1_;                                                                                \
+        tdot[q_ * n + (J2_) * TB + cl_] = o_;                                                      \
       }                                                                                            \
     }                                                                                              \
   }
@@ -1457,7 +1472,7 @@ __global__ __launch_bounds__(dma::NT) void k_solve_sym(const typename Cx<T>::t* 
     if (Tc < TT && t == TPS - 1) HZ_SY_EPILOGUE(s)                                        \
   }
 
-  for (int i = tid; i < 2 * n; i += dma::NT) {
+  for (int i = tid; i < 4 * n; i += dma::NT) {
     C z;
     z.x = 0; z.y = 0;
     tdot[i] = z;

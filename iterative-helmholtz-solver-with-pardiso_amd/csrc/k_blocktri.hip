@@ -444,6 +444,22 @@ __device__ __forceinline__ void unit_fma(const typename Unit<T>::t& a, typename 
   }
 }
 
+// acc (RPL rows, re/im interleaved) += unit a (RPL complex entries) * vc, as
+// chains of fused multiply-adds into the accumulators (4 FMAs per complex
+// product instead of a product followed by an add)
+template <typename T, int RPL>
+__device__ __forceinline__ void unit_fma_fused(const typename Unit<T>::t& a, typename Cx<T>::t vc, T* acc) {
+  typedef typename Cx<T>::t C;
+  const C* ag = reinterpret_cast<const C*>(&a);
+#pragma unroll
+  for (int r = 0; r < RPL; ++r) {
+    acc[2 * r] = fma(ag[r].x, vc.x, acc[2 * r]);
+    acc[2 * r] = fma(-ag[r].y, vc.y, acc[2 * r]);
+    acc[2 * r + 1] = fma(ag[r].x, vc.y, acc[2 * r + 1]);
+    acc[2 * r + 1] = fma(ag[r].y, vc.x, acc[2 * r + 1]);
+  }
+}
+
 template <typename T, int RPL, int CH>
 __global__ __launch_bounds__(1024) void k_solve_fast(const typename Cx<T>::t* __restrict__ G,
                                                      typename Cx<T>::t* __restrict__ X,
@@ -1226,6 +1242,34 @@ __device__ __forceinline__ float row16_sum(float x) {
   x += dpp_f<0x140>(x);  // row_mirror
   return x;
 }
+// one transpose-reduce step: CTRL pairs each lane with a lane whose bit
+// (hi) differs; the lane with the bit set keeps b and adds the partner's b,
+// the other keeps a and adds the partner's a, so two values per lane become one
+template <int CTRL>
+__device__ __forceinline__ float pair_step(float a, float b, bool hi) {
+  const float keep = hi ? b : a, send = hi ? a : b;
+  return keep + dpp_f<CTRL>(send);
+}
+// 16-lane row sums of 8 values per lane in 22 operations: afterwards lane l
+// holds the row sum of x[l & 7] (lanes l and l + 8 alike)
+__device__ __forceinline__ float row16_sum8(const float* x, int lane) {
+  const bool b0 = lane & 1, b1 = lane & 2, b2 = lane & 4;
+  const float w0 = pair_step<0xB1>(x[0], x[1], b0), w1 = pair_step<0xB1>(x[2], x[3], b0);
+  const float w2 = pair_step<0xB1>(x[4], x[5], b0), w3 = pair_step<0xB1>(x[6], x[7], b0);
+  const float y0 = pair_step<0x4E>(w0, w1, b1), y1 = pair_step<0x4E>(w2, w3, b1);
+  float z = pair_step<0x124>(y0, y1, b2);  // row_ror:4
+  z += dpp_f<0x128>(z);                    // row_ror:8
+  return z;
+}
+// the same for 4 values: lane l holds the row sum of x[l & 3]
+__device__ __forceinline__ float row16_sum4(const float* x, int lane) {
+  const bool b0 = lane & 1, b1 = lane & 2;
+  const float w0 = pair_step<0xB1>(x[0], x[1], b0), w1 = pair_step<0xB1>(x[2], x[3], b0);
+  float z = pair_step<0x4E>(w0, w1, b1);
+  z += dpp_f<0x124>(z);  // row_ror:4
+  z += dpp_f<0x128>(z);  // row_ror:8
+  return z;
+}
 
 template <typename T, int RPL, int NCG, int TB, int NX>
 struct SymShape {
@@ -1238,10 +1282,11 @@ struct SymShape {
   static constexpr int units = L::PAIR / RPL;               // 16-byte units per pair tile
   static constexpr int R = (units + dma::NT - 1) / dma::NT;
   static constexpr int PU = n * (int)sizeof(C) / 16;        // units of one y-line
+  static constexpr int NQ = 4 * (RUP / 64);                 // partial sums of a transposed product
   static constexpr size_t lds = (size_t)S * R * dma::NT * 16 + (size_t)NCG * n * sizeof(C) +
-                                6 * (size_t)n * sizeof(C) + 3 * 64 * 16;
+                                (2 + NQ) * (size_t)n * sizeof(C) + 3 * 64 * 16 + S * 16;
   static_assert(nru <= RUP && RUP % 64 == 0 && TB % NCG == 0 && TB % RPL == 0, "thread split");
-  static_assert(TB * 64 == dma::NT, "transposed products: one wave per column of a tile");
+  static_assert(CPB == 2 && RPL == 2 && sizeof(T) == 4, "transposed values: 2 block columns x 2 columns x re/im");
   static_assert(RUP % 64 == 0 && (dma::NT / 64) % NCG == 0, "row units: whole waves per column group");
   static_assert(PU <= 128, "line DMA of two waves");
   static_assert((S - 1) * R <= 63, "vmcnt range");
@@ -1262,11 +1307,15 @@ __global__ __launch_bounds__(dma::NT) void k_solve_sym(const typename Cx<T>::t* 
   typedef typename Unit<T>::t U;
   constexpr int n = S_::n, nru = S_::nru, RUP = S_::RUP, CPB = S_::CPB, R = S_::R, PU = S_::PU;
   constexpr int TPS = L::TPS, NPAIR = L::NPAIR, PAIR = L::PAIR, BS = L::BS, T_ = L::T;
-  constexpr int TIT = ((T_ - 1) * TB / RPL + 63) / 64;  // transposed units per lane (max)
-  __shared__ U s0[R * dma::NT], s1[R * dma::NT], s2[R * dma::NT], s3[R * dma::NT];
-  __shared__ U s4[R * dma::NT], s5[R * dma::NT], s6[R * dma::NT], s7[R * dma::NT];
+  constexpr int NQ = S_::NQ;
+  // each slot ends with a zero unit (index ZU, never a DMA target): the operand
+  // of idle lanes, selected by index so that every read names its slot
+  constexpr int ZU = R * dma::NT;
+  __shared__ U s0[ZU + 1], s1[ZU + 1], s2[ZU + 1], s3[ZU + 1];
+  __shared__ U s4[ZU + 1], s5[ZU + 1], s6[ZU + 1], s7[ZU + 1];
   __shared__ C part[NCG * n];
-  __shared__ C tdot[4 * n];  // transposed products y_c of the current step, 4 partial sums
+  // transposed products y_c of the current step, NQ partial sums (re, im)
+  __shared__ __align__(16) T tdot[NQ * n * 2];
   __shared__ C v[n];
   // next-line operand (waves 0-1) and the step's coefficients (wave 2: cm_y, cp_y,
   // 1/dy), all fetched by DMA at the start of the step so that the epilogue
@@ -1283,12 +1332,15 @@ __global__ __launch_bounds__(dma::NT) void k_solve_sym(const typename Cx<T>::t* 
   const double2* dxi = symd + n + ny;
   const double2* dyi = symd + 2 * n + ny;
   const int tid = threadIdx.x;
-  const int lane = tid & 63, wv = tid >> 6;
+  // the wave index is uniform: held in a scalar register, so that the tile
+  // offsets built from it (column groups, transposed columns, DMA targets) are
+  // scalar arithmetic
+  const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   // wave w: column group w % NCG, row units (w / NCG) * 64 + lane, so that the
   // waves sharing a SIMD (w, w + 4) hold low and high rows of the same group:
   // the direct product's work (rows 0..(J+1) TB of block column J) stays even
   // across SIMDs
-  const int cg = wv % NCG, ru = (wv / NCG) * 64 + lane;
+  const int cg = wv % NCG, rb = (wv / NCG) * 64, ru = rb + lane;
   const bool worker = ru < nru;
   const int nsteps = 2 * ny - 1;
   const int TT = nsteps * TPS;
@@ -1329,6 +1381,7 @@ __global__ __launch_bounds__(dma::NT) void k_solve_sym(const typename Cx<T>::t* 
     }                                                                                                   \
   }
   T acc[2 * RPL];
+  U vr;  // the lane's row unit of v' (rows ru RPL ..), loaded at the start of each step
 #define HZ_SY_EPILOGUE(S_STEP)                                                                     \
   {                                                                                                \
     const int se_ = (S_STEP);                                                                      \
@@ -1350,8 +1403,8 @@ __global__ __launch_bounds__(dma::NT) void k_solve_sym(const typename Cx<T>::t* 
         const C u = part[q * n + tid];                                                             \
         y.x += u.x; y.y += u.y;                                                                    \
       }                                                                                            \
-      _Pragma("unroll") for (int q = 0; q < 4; ++q) {                                              \
-        const C u = tdot[q * n + tid];                                                             \
+      _Pragma("unroll") for (int q = 0; q < NQ; ++q) {                                             \
+        const C u = reinterpret_cast<const C*>(tdot)[q * n + tid];                                 \
         y.x += u.x; y.y += u.y;                                                                    \
       }                                                                                            \
       const C pre = pline[tid];                                                                    \
@@ -1385,66 +1438,89 @@ __global__ __launch_bounds__(dma::NT) void k_solve_sym(const typename Cx<T>::t* 
     }                                                                                              \
     __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);                                                   \
   }
-#define HZ_SY_BLOCKCOL(SLOT, J_, UBASE_)                                                           \
+  // both products of the tile in one pass over it: lane (cg, ru) reads row
+  // unit ru of columns cg CPB .. cg CPB + 1 of the pair's block columns and
+  // uses each unit twice, for the direct product (rows ru, accumulated in
+  // registers over the step) and for the transposed product (columns of the
+  // unit, from v' of the lane's own rows, zero on and below the diagonal
+  // block).  The transposed values are summed over each 16-lane row by
+  // transpose-reduce steps and stored as partial (rb / 64) 4 + row of their
+  // column; a wave whose rows lie below a block column's stored rows skips it
+  // (its partials for that column stay zero).  Lanes below a column's stored
+  // rows read the slot's zero unit.
+#define HZ_SY_UNIT_DOT(A_, W_, OUT_)                                                               \
   {                                                                                                \
-    const int Jb_ = (J_);                                                                          \
-    const int Hu_ = (Jb_ + 1) * TB / RPL; /* units per stored column */                           \
-    if (worker && ru < Hu_) {                                                                      \
-      _Pragma("unroll") for (int cc = 0; cc < CPB; ++cc) {                                         \
-        const int cl = cg * CPB + cc;                                                              \
-        unit_fma<T, RPL>(SLOT[(UBASE_) + cl * Hu_ + ru], v[Jb_ * TB + cl], acc);                   \
-      }                                                                                            \
+    const C* ag_ = reinterpret_cast<const C*>(&(A_));                                              \
+    _Pragma("unroll") for (int i_ = 0; i_ < RPL; ++i_) {                                           \
+      (OUT_)[0] = fma(ag_[i_].x, (W_)[i_].x, (OUT_)[0]);                                           \
+      (OUT_)[0] = fma(-ag_[i_].y, (W_)[i_].y, (OUT_)[0]);                                          \
+      (OUT_)[1] = fma(ag_[i_].x, (W_)[i_].y, (OUT_)[1]);                                           \
+      (OUT_)[1] = fma(ag_[i_].y, (W_)[i_].x, (OUT_)[1]);                                           \
     }                                                                                              \
   }
-  // transposed products of the tile's off-diagonal rows: wave c takes column c
-  // of both block columns of the pair, its lanes running over their
-  // concatenated off-diagonal units (J TB / RPL per column, (T-1) TB / RPL for
-  // the pair), so the work is the same for every tile and every wave; each
-  // 16-lane DPP row sums its lanes and its first lane stores one of 4 partial
-  // sums per output (every column of a block gets its 4 partials each step)
-#define HZ_SY_TRANSPOSED(SLOT, J1_, J2_)                                                           \
+#define HZ_SY_PRODUCTS(SLOT, J1_, J2_)                                                             \
   {                                                                                                \
-    const int cl_ = wv;                                                                            \
     const int Hu1_ = ((J1_) + 1) * TB / RPL, Hu2_ = ((J2_) + 1) * TB / RPL;                        \
-    const int nu1_ = (J1_) * TB / RPL;                                                             \
-    const int nu2_ = (J2_) != (J1_) ? (J2_) * TB / RPL : 0;                                        \
+    const int nu1_ = (J1_) * TB / RPL, nu2_ = (J2_) * TB / RPL;                                    \
     const int ub2_ = ((J1_) + 1) * TB * TB / RPL;                                                  \
-    T a0_ = 0, a1_ = 0, b0_ = 0, b1_ = 0;                                                          \
-    _Pragma("unroll") for (int it_ = 0; it_ < TIT; ++it_) {                                        \
-      const int u_ = lane + 64 * it_;                                                              \
-      if (u_ < nu1_ + nu2_) {                                                                      \
-        const bool first_ = u_ < nu1_;                                                             \
-        const int ur_ = first_ ? u_ : u_ - nu1_; /* row unit within its column */                 \
-        const U ga_ = SLOT[first_ ? cl_ * Hu1_ + ur_ : ub2_ + cl_ * Hu2_ + ur_];                    \
-        const U gv_ = reinterpret_cast<const U*>(v)[ur_];                                          \
-        const C* ag_ = reinterpret_cast<const C*>(&ga_);                                           \
-        const C* vg_ = reinterpret_cast<const C*>(&gv_);                                           \
-        T p0_ = 0, p1_ = 0;                                                                        \
-        _Pragma("unroll") for (int i_ = 0; i_ < RPL; ++i_) {                                       \
-          p0_ += ag_[i_].x * vg_[i_].x - ag_[i_].y * vg_[i_].y;                                    \
-          p1_ += ag_[i_].x * vg_[i_].y + ag_[i_].y * vg_[i_].x;                                    \
-        }                                                                                          \
-        if (first_) {                                                                              \
-          a0_ += p0_; a1_ += p1_;                                                                  \
-        } else {                                                                                   \
-          b0_ += p0_; b1_ += p1_;                                                                  \
-        }                                                                                          \
+    const bool two_ = (J2_) != (J1_);                                                              \
+    const C* vrow_ = reinterpret_cast<const C*>(&vr);                                              \
+    C zc_;                                                                                         \
+    zc_.x = 0; zc_.y = 0;                                                                          \
+    if (rb < Hu1_) {                                                                               \
+      const bool ok1_ = worker && ru < Hu1_;                                                       \
+      const bool ok2_ = worker && two_ && ru < Hu2_;                                               \
+      const int g1_ = ok1_ ? cg * CPB * Hu1_ + ru : ZU;                                            \
+      const int g2_ = ok2_ ? ub2_ + cg * CPB * Hu2_ + ru : ZU;                                     \
+      const int st1_ = ok1_ ? Hu1_ : 0, st2_ = ok2_ ? Hu2_ : 0;                                    \
+      U a1_[CPB], a2_[CPB];                                                                        \
+      _Pragma("unroll") for (int cc = 0; cc < CPB; ++cc) {                                         \
+        a1_[cc] = SLOT[g1_ + cc * st1_];                                                           \
+        a2_[cc] = SLOT[g2_ + cc * st2_];                                                           \
       }                                                                                            \
-    }                                                                                              \
-    a0_ = row16_sum(a0_);                                                                          \
-    a1_ = row16_sum(a1_);                                                                          \
-    b0_ = row16_sum(b0_);                                                                          \
-    b1_ = row16_sum(b1_);                                                                          \
-    if ((lane & 15) == 0) {                                                                        \
-      const int q_ = lane >> 4;                                                                    \
-      C o_;                                                                                        \
-      o_.x = a0_;                                                                                  \
-      o_.y = a1_;                                                                                  \
-      tdot[q_ * n + (J1_) * TB + cl_] = o_;                                                        \
-      if ((J2_) != (J1_)) {                                                                        \
-        o_.x = b0_;                                                                                \
-        o_.y = b1_;                                                                                \
-        tdot[q_ * n + (J2_) * TB + cl_] = o_;                                                      \
+      const C* v1_ = &v[(J1_) * TB + cg * CPB];                                                    \
+      const C* v2_ = &v[(J2_) * TB + cg * CPB];                                                    \
+      C w1_[RPL], w2_[RPL];                                                                        \
+      _Pragma("unroll") for (int i = 0; i < RPL; ++i) {                                            \
+        w1_[i] = ru < nu1_ ? vrow_[i] : zc_;                                                       \
+        w2_[i] = ru < nu2_ ? vrow_[i] : zc_;                                                       \
+      }                                                                                            \
+      T tv_[8];                                                                                    \
+      _Pragma("unroll") for (int i = 0; i < 8; ++i) tv_[i] = 0;                                    \
+      _Pragma("unroll") for (int cc = 0; cc < CPB; ++cc) {                                         \
+        unit_fma_fused<T, RPL>(a1_[cc], v1_[cc], acc);                                             \
+        unit_fma_fused<T, RPL>(a2_[cc], v2_[cc], acc);                                             \
+        HZ_SY_UNIT_DOT(a1_[cc], w1_, &tv_[2 * cc])                                                 \
+        HZ_SY_UNIT_DOT(a2_[cc], w2_, &tv_[4 + 2 * cc])                                             \
+      }                                                                                            \
+      const T s_ = row16_sum8(tv_, lane);                                                          \
+      const int k_ = lane & 7;                                                                     \
+      if ((lane & 8) == 0 && (two_ || k_ < 4)) {                                                   \
+        const int col_ = ((k_ & 4) ? (J2_) : (J1_)) * TB + cg * CPB + ((k_ >> 1) & 1);             \
+        const int q_ = (rb >> 6) * 4 + (lane >> 4);                                                \
+        tdot[(q_ * n + col_) * 2 + (k_ & 1)] = s_;                                                 \
+      }                                                                                            \
+    } else if (rb < Hu2_ && two_) {                                                                \
+      const bool ok2_ = worker && ru < Hu2_;                                                       \
+      const int g2_ = ok2_ ? ub2_ + cg * CPB * Hu2_ + ru : ZU;                                     \
+      const int st2_ = ok2_ ? Hu2_ : 0;                                                            \
+      U a2_[CPB];                                                                                  \
+      _Pragma("unroll") for (int cc = 0; cc < CPB; ++cc) a2_[cc] = SLOT[g2_ + cc * st2_];          \
+      const C* v2_ = &v[(J2_) * TB + cg * CPB];                                                    \
+      C w2_[RPL];                                                                                  \
+      _Pragma("unroll") for (int i = 0; i < RPL; ++i) w2_[i] = ru < nu2_ ? vrow_[i] : zc_;         \
+      T tv_[4];                                                                                    \
+      _Pragma("unroll") for (int i = 0; i < 4; ++i) tv_[i] = 0;                                    \
+      _Pragma("unroll") for (int cc = 0; cc < CPB; ++cc) {                                         \
+        unit_fma_fused<T, RPL>(a2_[cc], v2_[cc], acc);                                             \
+        HZ_SY_UNIT_DOT(a2_[cc], w2_, &tv_[2 * cc])                                                 \
+      }                                                                                            \
+      const T s_ = row16_sum4(tv_, lane);                                                          \
+      const int k_ = lane & 3;                                                                     \
+      if ((lane & 12) == 0) {                                                                      \
+        const int col_ = (J2_) * TB + cg * CPB + (k_ >> 1);                                        \
+        const int q_ = (rb >> 6) * 4 + (lane >> 4);                                                \
+        tdot[(q_ * n + col_) * 2 + (k_ & 1)] = s_;                                                 \
       }                                                                                            \
     }                                                                                              \
   }
@@ -1457,6 +1533,8 @@ __global__ __launch_bounds__(dma::NT) void k_solve_sym(const typename Cx<T>::t* 
     dma::raw_barrier();                                                                   \
     if (Tc < TT && t == 0) {                                                              \
       _Pragma("unroll") for (int i = 0; i < 2 * RPL; ++i) acc[i] = 0;                     \
+      vr = reinterpret_cast<const U*>(v)[worker ? ru : 0];                                \
+      if (!worker) vr.x = vr.y = vr.z = vr.w = 0;                                         \
       int slot_line = 0;                                                                  \
       if (s < ny - 1) slot_line = s + 1;                                                  \
       else if (2 * ny - 2 - s > 0) slot_line = 2 * ny - 3 - s;                            \
@@ -1464,19 +1542,19 @@ __global__ __launch_bounds__(dma::NT) void k_solve_sym(const typename Cx<T>::t* 
     }                                                                                     \
     if (Tc < TT) {                                                                        \
       const int J1 = t, J2 = T_ - 1 - t;                                                  \
-      HZ_SY_BLOCKCOL(SLOT, J1, 0)                                                         \
-      if (J2 != J1) HZ_SY_BLOCKCOL(SLOT, J2, (J1 + 1) * TB * TB / RPL)                    \
-      HZ_SY_TRANSPOSED(SLOT, J1, J2)                                                      \
+      HZ_SY_PRODUCTS(SLOT, J1, J2)                                                        \
     }                                                                                     \
     HZ_SY_TILE(Tc + S_::S - 1, NEXT)                                                      \
     if (Tc < TT && t == TPS - 1) HZ_SY_EPILOGUE(s)                                        \
   }
 
-  for (int i = tid; i < 4 * n; i += dma::NT) {
-    C z;
-    z.x = 0; z.y = 0;
-    tdot[i] = z;
+  if (tid == 0) {
+    U z;
+    z.x = 0; z.y = 0; z.z = 0; z.w = 0;
+    s0[ZU] = z; s1[ZU] = z; s2[ZU] = z; s3[ZU] = z;
+    s4[ZU] = z; s5[ZU] = z; s6[ZU] = z; s7[ZU] = z;
   }
+  for (int i = tid; i < NQ * n * 2; i += dma::NT) tdot[i] = 0;
   for (int i = tid; i < n; i += dma::NT) {
     const double2 a = dxi[i];
     C c;
@@ -1512,8 +1590,8 @@ __global__ __launch_bounds__(dma::NT) void k_solve_sym(const typename Cx<T>::t* 
     HZ_SY_STAGE(T0 + 7, s7, s6)
   }
 #undef HZ_SY_STAGE
-#undef HZ_SY_TRANSPOSED
-#undef HZ_SY_BLOCKCOL
+#undef HZ_SY_PRODUCTS
+#undef HZ_SY_UNIT_DOT
 #undef HZ_SY_EPILOGUE
 #undef HZ_SY_LINE
 #undef HZ_SY_TILE

@@ -1591,10 +1591,278 @@ __global__ __launch_bounds__(dma::NT) void k_solve_sym(const typename Cx<T>::t* 
   }
 #undef HZ_SY_STAGE
 #undef HZ_SY_PRODUCTS
-#undef HZ_SY_UNIT_DOT
 #undef HZ_SY_EPILOGUE
 #undef HZ_SY_LINE
 #undef HZ_SY_TILE
+  __builtin_amdgcn_s_waitcnt(dma::wait_vm(0));
+}
+
+// LDS DMA of 16 bytes per lane issued by inline assembly (M0 = the wave's
+// LDS destination, lanes at 16-byte steps).  The compiler does not track it:
+// reads of the ring below address it through a runtime slot offset and get no
+// automatic waits; every wait on these transfers is an explicit s_waitcnt.
+// (M0 is named as clobbered; the compiler sets M0 itself before each of its
+// own uses, and this code has no dynamically indexed register arrays.)
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+__device__ __forceinline__ void dma16_untracked(const void* g, unsigned lds_wave) {
+  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(g), "s"(lds_wave)
+               : "memory", "m0");
+}
+#pragma clang diagnostic pop
+
+// Symmetric packed plane solve with the step unrolled: the TPS tiles of a
+// block are a compile-time loop, so every tile parameter (block columns,
+// stored and off-diagonal rows, offsets, tile size) is a constant and the
+// per-tile scalar work is the ring slot, one DMA address and the waits.  The
+// 8-slot ring is one LDS array addressed by a runtime slot; its transfers are
+// untracked DMAs with explicit waits.  Products as in k_solve_sym: each unit
+// serves its rows (direct product, registers) and its columns against the
+// lane's own v' rows (transposed product, transpose-reduce row sums into NQ
+// partials).
+template <int NX>
+__global__ __launch_bounds__(dma::NT) void k_solve_sym2(const float2* __restrict__ P, float2* __restrict__ X,
+                                                        const double2* __restrict__ cmy,
+                                                        const double2* __restrict__ cpy,
+                                                        const double2* __restrict__ symd, int ny, int64_t n2,
+                                                        int64_t N) {
+  typedef float T;
+  typedef float2 C;
+  typedef float4 U;
+  constexpr int RPL = 2, NCG = 4, TB = 8;
+  typedef SymShape<T, RPL, NCG, TB, NX> S_;
+  typedef SymLayout<NX, TB> L;
+  constexpr int n = S_::n, nru = S_::nru, CPB = S_::CPB, R = S_::R, PU = S_::PU, NQ = S_::NQ;
+  constexpr int TPS = L::TPS, NPAIR = L::NPAIR, PAIR = L::PAIR, BS = L::BS, T_ = L::T;
+  constexpr int NS = S_::S;           // ring slots (a power of two)
+  constexpr int ZU = R * dma::NT;     // zero unit of a slot
+  constexpr int SU = ZU + 1;          // units per slot
+  static_assert((NS & (NS - 1)) == 0, "ring slots: power of two");
+  __shared__ U ring[NS * SU];
+  __shared__ C part[NCG * n];
+  __shared__ __align__(16) T tdot[NQ * n * 2];
+  __shared__ C v[n];
+  __shared__ U pbuf[3 * 64];
+  __shared__ C sdxi[n];
+  const C* pline = reinterpret_cast<const C*>(pbuf);
+  const double2* pscal = reinterpret_cast<const double2*>(&pbuf[128]);
+
+  const int k = blockIdx.x;
+  const int64_t rhs = blockIdx.y;
+  C* x = X + rhs * N + (int64_t)k * n2;
+  const C* Pk = P + (size_t)k * ny * BS;
+  const double2* dxi = symd + n + ny;
+  const double2* dyi = symd + 2 * n + ny;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int cg = wv % NCG, rb = (wv / NCG) * 64, ru = rb + lane;
+  const bool worker = ru < nru;
+  const int nsteps = 2 * ny - 1;
+  auto blk_of = [&](int st) { return st <= ny - 1 ? st : 2 * ny - 2 - st; };
+  const unsigned ring_lds = (unsigned)(size_t)(&ring[0]);
+  const unsigned pbuf_lds = (unsigned)(size_t)(&pbuf[0]);
+
+  for (int i = tid; i < NQ * n * 2; i += dma::NT) tdot[i] = 0;
+  if (tid < NS) ring[tid * SU + ZU] = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int i = tid; i < n; i += dma::NT) {
+    const double2 a = dxi[i];
+    sdxi[i] = make_float2((T)a.x, (T)a.y);
+  }
+  __syncthreads();
+  for (int i = tid; i < n; i += dma::NT) {
+    const C x0 = x[i];
+    const C a = sdxi[i];
+    const double2 b = dyi[blk_of(0)];
+    const double2 di = make_double2((double)a.x * b.x - (double)a.y * b.y, (double)a.x * b.y + (double)a.y * b.x);
+    v[i] = make_float2((T)((double)x0.x * di.x - (double)x0.y * di.y), (T)((double)x0.x * di.y + (double)x0.y * di.x));
+  }
+  __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
+
+  // tile tt of the block at byte address blk into ring slot sl (tiles past the
+  // end of the sweep re-read tile 0 so that every stage issues R transfers)
+  auto tile_dma = [&](const char* blk, int tt, int sl) {
+    const int tu = (tt < NPAIR ? PAIR : (NPAIR + 1) * TB * TB) / RPL;
+    const char* base = blk + (size_t)tt * PAIR * sizeof(C);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int u = min(r * dma::NT + tid, tu - 1);
+      dma16_untracked(base + (size_t)u * 16, ring_lds + (unsigned)((sl * SU + r * dma::NT + wv * 64) * 16));
+    }
+  };
+  auto blk_addr = [&](int st) {
+    return reinterpret_cast<const char*>(Pk + (size_t)blk_of(st < nsteps ? st : 0) * BS);
+  };
+#pragma unroll
+  for (int g = 0; g < NS - 1; ++g) tile_dma(blk_addr(0), g, g);
+
+  T acc[2 * RPL];
+  U vr;
+  int slot = 0;
+  for (int st = 0; st < nsteps; ++st) {
+    const char* b0 = blk_addr(st);
+    const char* b1 = blk_addr(st + 1);
+#pragma unroll
+    for (int t = 0; t < TPS; ++t) {
+      __builtin_amdgcn_s_waitcnt(dma::wait_vm((NS - 2) * R));
+      __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
+      dma::raw_barrier();
+      const U* SL = ring + slot * SU;
+      if (t == 0) {
+#pragma unroll
+        for (int i = 0; i < 2 * RPL; ++i) acc[i] = 0;
+        vr = reinterpret_cast<const U*>(v)[worker ? ru : 0];
+        if (!worker) vr = make_float4(0.f, 0.f, 0.f, 0.f);
+        // next y-line (waves 0-1) and the step's coefficients (wave 2)
+        int line = 0;
+        if (st < ny - 1) line = st + 1;
+        else if (2 * ny - 2 - st > 0) line = 2 * ny - 3 - st;
+        if (wv < 2) {
+          const char* lb = reinterpret_cast<const char*>(x + (size_t)n * line);
+          dma16_untracked(lb + (size_t)min(tid, PU - 1) * 16, pbuf_lds + (unsigned)(wv * 64 * 16));
+        } else if (wv == 2) {
+          const int sn = min(st + 1, nsteps - 1);
+          const int jb = min(max(2 * ny - 3 - st, 0), ny - 1);
+          const double2* src = lane == 1 ? cpy + jb : (lane == 2 ? dyi + blk_of(sn) : cmy + min(st + 1, ny - 1));
+          dma16_untracked(src, pbuf_lds + 128 * 16);
+        }
+      }
+      {
+        constexpr int dummy = 0;
+        (void)dummy;
+        const int J1 = t, J2 = T_ - 1 - t;
+        const int Hu1 = (J1 + 1) * TB / RPL, Hu2 = (J2 + 1) * TB / RPL;
+        const int nu1 = J1 * TB / RPL, nu2 = J2 * TB / RPL;
+        const int ub2 = (J1 + 1) * TB * TB / RPL;
+        const bool two = J2 != J1;
+        const C* vrow = reinterpret_cast<const C*>(&vr);
+        const C zc = make_float2(0.f, 0.f);
+        if (rb < Hu1) {
+          const bool ok1 = worker && ru < Hu1;
+          const bool ok2 = worker && two && ru < Hu2;
+          const int g1 = ok1 ? cg * CPB * Hu1 + ru : ZU;
+          const int g2 = ok2 ? ub2 + cg * CPB * Hu2 + ru : ZU;
+          const int st1 = ok1 ? Hu1 : 0, st2 = ok2 ? Hu2 : 0;
+          U a1[CPB], a2[CPB];
+#pragma unroll
+          for (int cc = 0; cc < CPB; ++cc) {
+            a1[cc] = SL[g1 + cc * st1];
+            a2[cc] = SL[g2 + cc * st2];
+          }
+          const C* v1 = &v[J1 * TB + cg * CPB];
+          const C* v2 = &v[J2 * TB + cg * CPB];
+          C w1[RPL], w2[RPL];
+#pragma unroll
+          for (int i = 0; i < RPL; ++i) {
+            w1[i] = ru < nu1 ? vrow[i] : zc;
+            w2[i] = ru < nu2 ? vrow[i] : zc;
+          }
+          T tv[8];
+#pragma unroll
+          for (int i = 0; i < 8; ++i) tv[i] = 0;
+#pragma unroll
+          for (int cc = 0; cc < CPB; ++cc) {
+            unit_fma_fused<T, RPL>(a1[cc], v1[cc], acc);
+            unit_fma_fused<T, RPL>(a2[cc], v2[cc], acc);
+            HZ_SY_UNIT_DOT(a1[cc], w1, &tv[2 * cc])
+            HZ_SY_UNIT_DOT(a2[cc], w2, &tv[4 + 2 * cc])
+          }
+          const T sm = row16_sum8(tv, lane);
+          const int kk = lane & 7;
+          if ((lane & 8) == 0 && (two || kk < 4)) {
+            const int col = ((kk & 4) ? J2 : J1) * TB + cg * CPB + ((kk >> 1) & 1);
+            const int q = (rb >> 6) * 4 + (lane >> 4);
+            tdot[(q * n + col) * 2 + (kk & 1)] = sm;
+          }
+        } else if (rb < Hu2 && two) {
+          const bool ok2 = worker && ru < Hu2;
+          const int g2 = ok2 ? ub2 + cg * CPB * Hu2 + ru : ZU;
+          const int st2 = ok2 ? Hu2 : 0;
+          U a2[CPB];
+#pragma unroll
+          for (int cc = 0; cc < CPB; ++cc) a2[cc] = SL[g2 + cc * st2];
+          const C* v2 = &v[J2 * TB + cg * CPB];
+          C w2[RPL];
+#pragma unroll
+          for (int i = 0; i < RPL; ++i) w2[i] = ru < nu2 ? vrow[i] : zc;
+          T tv[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) tv[i] = 0;
+#pragma unroll
+          for (int cc = 0; cc < CPB; ++cc) {
+            unit_fma_fused<T, RPL>(a2[cc], v2[cc], acc);
+            HZ_SY_UNIT_DOT(a2[cc], w2, &tv[2 * cc])
+          }
+          const T sm = row16_sum4(tv, lane);
+          const int kk = lane & 3;
+          if ((lane & 12) == 0) {
+            const int col = J2 * TB + cg * CPB + (kk >> 1);
+            const int q = (rb >> 6) * 4 + (lane >> 4);
+            tdot[(q * n + col) * 2 + (kk & 1)] = sm;
+          }
+        }
+      }
+      {
+        // the tile NS - 1 stages ahead into the slot the previous stage read
+        constexpr int dummy2 = 0;
+        (void)dummy2;
+        const int ta = t + NS - 1;
+        const int tt = ta % TPS;
+        const char* blk = (ta / TPS) == 0 ? b0 : b1;
+        const bool past = (ta / TPS) != 0 && st + 1 >= nsteps;
+        tile_dma(past ? reinterpret_cast<const char*>(Pk) : blk, past ? 0 : tt, (slot + NS - 1) & (NS - 1));
+      }
+      if (t == TPS - 1) {
+        if (worker) {
+#pragma unroll
+          for (int r = 0; r < RPL; ++r) part[cg * n + ru * RPL + r] = make_float2(acc[2 * r], acc[2 * r + 1]);
+        }
+        __builtin_amdgcn_s_waitcnt(dma::wait_vm((NS - 1) * R));
+        __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
+        dma::raw_barrier();
+        if (tid < n) {
+          C y = make_float2(0.f, 0.f);
+#pragma unroll
+          for (int q = 0; q < NCG; ++q) {
+            const C u = part[q * n + tid];
+            y.x += u.x; y.y += u.y;
+          }
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) {
+            const C u = reinterpret_cast<const C*>(tdot)[q * n + tid];
+            y.x += u.x; y.y += u.y;
+          }
+          const C pre = pline[tid];
+          C nv;
+          if (st < ny - 1) {
+            const double2 cm = pscal[0];
+            nv.x = pre.x - (T)(cm.x * y.x - cm.y * y.y);
+            nv.y = pre.y - (T)(cm.x * y.y + cm.y * y.x);
+            x[(size_t)n * (st + 1) + tid] = nv;
+          } else {
+            const int j = 2 * ny - 2 - st;
+            x[(size_t)n * j + tid] = y;
+            if (j > 0) {
+              const double2 cp = pscal[1];
+              nv.x = pre.x - (T)(cp.x * y.x - cp.y * y.y);
+              nv.y = pre.y - (T)(cp.x * y.y + cp.y * y.x);
+            } else {
+              nv = y;
+            }
+          }
+          if (st + 1 < nsteps) {
+            const C a = sdxi[tid];
+            const double2 b = pscal[2];
+            const double2 di =
+                make_double2((double)a.x * b.x - (double)a.y * b.y, (double)a.x * b.y + (double)a.y * b.x);
+            v[tid] = make_float2((T)((double)nv.x * di.x - (double)nv.y * di.y),
+                                 (T)((double)nv.x * di.y + (double)nv.y * di.x));
+          }
+        }
+        __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
+      }
+      slot = (slot + 1) & (NS - 1);
+    }
+  }
   __builtin_amdgcn_s_waitcnt(dma::wait_vm(0));
 }
 
@@ -1629,6 +1897,16 @@ bool try_solve_sym(hz_problem* p, int prec, int nrhs, void* x, hipStream_t s) {
   return true;
 }
 
+template <int NX>
+bool try_solve_sym2(hz_problem* p, int prec, int nrhs, void* x, hipStream_t s) {
+  if (p->nx != NX || prec != HZ_PREC_C64) return false;
+  hipLaunchKernelGGL((k_solve_sym2<NX>), dim3((unsigned)p->nz, (unsigned)nrhs), dim3(dma::NT), 0, s,
+                     p->fac[prec].Gsym.as<float2>(), (float2*)x, p->tab(1, 0), p->tab(1, 1), p->d_symd.as<double2>(),
+                     (int)p->ny, p->n2, p->N);
+  HZ_HIP(hipGetLastError());
+  return true;
+}
+
 #define HZ_SYM_WIDTHS(X)                 \
   X(HZ_PREC_C64, float, 2, 4, 8, 200)    \
   X(HZ_PREC_C64, float, 2, 4, 8, 192)    \
@@ -1644,7 +1922,10 @@ bool pack_sym_any(hz_problem* p, int prec, hipStream_t s) {
 }
 
 bool solve_sym_any(hz_problem* p, int prec, int nrhs, void* x, hipStream_t s) {
-#define HZ_SYM_SOLVE(P, T, RPL, NCG, TB, NX) \
+  const char* v1 = getenv("HZ_SYM_V1");
+  const bool unrolled = !(v1 && v1[0] == '1');
+#define HZ_SYM_SOLVE(P, T, RPL, NCG, TB, NX)                                  \
+  if (prec == P && unrolled && try_solve_sym2<NX>(p, prec, nrhs, x, s)) return true; \
   if (prec == P && try_solve_sym<T, RPL, NCG, TB, NX>(p, prec, nrhs, x, s)) return true;
   HZ_SYM_WIDTHS(HZ_SYM_SOLVE)
 #undef HZ_SYM_SOLVE

@@ -206,7 +206,7 @@ def plane_solve_kernel(nx, tuning):
     """Name of the complex64 single-RHS plane-solve kernel the tuned choice launches."""
     fam = tuning.get("solve_kernel", 0)
     if fam == 5:
-        return "k_solve_sym<float,2,4,8,%d>" % nx
+        return "k_solve_sym2<%d>" % nx
     if fam == 3:
         return "k_solve_dma<float,2,5,20,200>"
     if fam == 1:
@@ -403,7 +403,7 @@ def plane_solve_bytes(dims, fam):
 def measured_traffic(n, fam):
     """HBM bytes per launch of the plane solve from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE passes (scaled by a 1 GiB copy), for the 200^3 grid."""
-    names = {3: "k_solve_dma_c64_200cube_pmc.md", 5: "k_solve_sym_c64_200cube_pmc.md"}
+    names = {3: "k_solve_dma_c64_200cube_pmc.md", 5: "k_solve_sym2_c64_200cube_pmc.md"}
     if n != 200 or fam not in names:
         return None
     path = os.path.join(ROOT, "profiles", names[fam])

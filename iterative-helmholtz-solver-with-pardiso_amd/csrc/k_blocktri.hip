@@ -1620,7 +1620,7 @@ __device__ __forceinline__ void dma16_untracked(const void* g, unsigned lds_wave
 // serves its rows (direct product, registers) and its columns against the
 // lane's own v' rows (transposed product, transpose-reduce row sums into NQ
 // partials).
-template <int NX>
+template <int NX, bool ADJ>
 __global__ __launch_bounds__(dma::NT) void k_solve_sym2(const float2* __restrict__ P, float2* __restrict__ X,
                                                         const double2* __restrict__ cmy,
                                                         const double2* __restrict__ cpy,
@@ -1671,10 +1671,15 @@ __global__ __launch_bounds__(dma::NT) void k_solve_sym2(const float2* __restrict
   __syncthreads();
   for (int i = tid; i < n; i += dma::NT) {
     const C x0 = x[i];
-    const C a = sdxi[i];
-    const double2 b = dyi[blk_of(0)];
-    const double2 di = make_double2((double)a.x * b.x - (double)a.y * b.y, (double)a.x * b.y + (double)a.y * b.x);
-    v[i] = make_float2((T)((double)x0.x * di.x - (double)x0.y * di.y), (T)((double)x0.x * di.y + (double)x0.y * di.x));
+    if (ADJ) {
+      v[i] = make_float2(x0.x, -x0.y);  // u_0 = conj(b_0)
+    } else {
+      const C a = sdxi[i];
+      const double2 b = dyi[blk_of(0)];
+      const double2 di = make_double2((double)a.x * b.x - (double)a.y * b.y, (double)a.x * b.y + (double)a.y * b.x);
+      v[i] = make_float2((T)((double)x0.x * di.x - (double)x0.y * di.y),
+                         (T)((double)x0.x * di.y + (double)x0.y * di.x));
+    }
   }
   __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
 
@@ -1715,14 +1720,22 @@ __global__ __launch_bounds__(dma::NT) void k_solve_sym2(const float2* __restrict
         // next y-line (waves 0-1) and the step's coefficients (wave 2)
         int line = 0;
         if (st < ny - 1) line = st + 1;
-        else if (2 * ny - 2 - st > 0) line = 2 * ny - 3 - st;
+        else if (ADJ && st >= ny) line = 2 * ny - 2 - st;  // Y_j of the backward block
+        else if (!ADJ && 2 * ny - 2 - st > 0) line = 2 * ny - 3 - st;
         if (wv < 2) {
           const char* lb = reinterpret_cast<const char*>(x + (size_t)n * line);
           dma16_untracked(lb + (size_t)min(tid, PU - 1) * 16, pbuf_lds + (unsigned)(wv * 64 * 16));
         } else if (wv == 2) {
-          const int sn = min(st + 1, nsteps - 1);
-          const int jb = min(max(2 * ny - 3 - st, 0), ny - 1);
-          const double2* src = lane == 1 ? cpy + jb : (lane == 2 ? dyi + blk_of(sn) : cmy + min(st + 1, ny - 1));
+          const double2* src;
+          if (ADJ) {
+            // cm_y and cp_y of the current block, 1/d_y of the current block
+            const int jc = blk_of(st);
+            src = lane == 1 ? cpy + min(st, ny - 1) : (lane == 2 ? dyi + jc : cmy + jc);
+          } else {
+            const int sn = min(st + 1, nsteps - 1);
+            const int jb = min(max(2 * ny - 3 - st, 0), ny - 1);
+            src = lane == 1 ? cpy + jb : (lane == 2 ? dyi + blk_of(sn) : cmy + min(st + 1, ny - 1));
+          }
           dma16_untracked(src, pbuf_lds + 128 * 16);
         }
       }
@@ -1832,6 +1845,36 @@ __global__ __launch_bounds__(dma::NT) void k_solve_sym2(const float2* __restrict
             y.x += u.x; y.y += u.y;
           }
           const C pre = pline[tid];
+          if (ADJ) {
+            // conjugated space: z = P_j u, Z = D_j^-1 z; forward Y_j = Z (stored),
+            // u = conj(b_{j+1}) - cp_j Y_j; backward X_j = Y_j - Z, x_j = conj(X_j),
+            // u = cm_j X_j
+            const C a = sdxi[tid];
+            const double2 b = pscal[2];
+            const double2 di =
+                make_double2((double)a.x * b.x - (double)a.y * b.y, (double)a.x * b.y + (double)a.y * b.x);
+            const double zx = (double)y.x * di.x - (double)y.y * di.y, zy = (double)y.x * di.y + (double)y.y * di.x;
+            C u;
+            if (st < ny - 1) {
+              const double2 cp = pscal[1];
+              x[(size_t)n * st + tid] = make_float2((T)zx, (T)zy);
+              u.x = (T)((double)pre.x - (cp.x * zx - cp.y * zy));
+              u.y = (T)(-(double)pre.y - (cp.x * zy + cp.y * zx));
+            } else {
+              double X_x = zx, X_y = zy;
+              int j = ny - 1;
+              if (st >= ny) {
+                j = 2 * ny - 2 - st;
+                X_x = (double)pre.x - zx;
+                X_y = (double)pre.y - zy;
+              }
+              x[(size_t)n * j + tid] = make_float2((T)X_x, (T)(-X_y));
+              const double2 cm = pscal[0];
+              u.x = (T)(cm.x * X_x - cm.y * X_y);
+              u.y = (T)(cm.x * X_y + cm.y * X_x);
+            }
+            if (st + 1 < nsteps) v[tid] = u;
+          } else {
           C nv;
           if (st < ny - 1) {
             const double2 cm = pscal[0];
@@ -1856,6 +1899,7 @@ __global__ __launch_bounds__(dma::NT) void k_solve_sym2(const float2* __restrict
                 make_double2((double)a.x * b.x - (double)a.y * b.y, (double)a.x * b.y + (double)a.y * b.x);
             v[tid] = make_float2((T)((double)nv.x * di.x - (double)nv.y * di.y),
                                  (T)((double)nv.x * di.y + (double)nv.y * di.x));
+          }
           }
         }
         __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
@@ -1898,11 +1942,16 @@ bool try_solve_sym(hz_problem* p, int prec, int nrhs, void* x, hipStream_t s) {
 }
 
 template <int NX>
-bool try_solve_sym2(hz_problem* p, int prec, int nrhs, void* x, hipStream_t s) {
+bool try_solve_sym2(hz_problem* p, int prec, int nrhs, void* x, bool adjoint, hipStream_t s) {
   if (p->nx != NX || prec != HZ_PREC_C64) return false;
-  hipLaunchKernelGGL((k_solve_sym2<NX>), dim3((unsigned)p->nz, (unsigned)nrhs), dim3(dma::NT), 0, s,
-                     p->fac[prec].Gsym.as<float2>(), (float2*)x, p->tab(1, 0), p->tab(1, 1), p->d_symd.as<double2>(),
-                     (int)p->ny, p->n2, p->N);
+  const dim3 grid((unsigned)p->nz, (unsigned)nrhs);
+  const float2* P = p->fac[prec].Gsym.as<float2>();
+  if (adjoint)
+    hipLaunchKernelGGL((k_solve_sym2<NX, true>), grid, dim3(dma::NT), 0, s, P, (float2*)x, p->tab(1, 0), p->tab(1, 1),
+                       p->d_symd.as<double2>(), (int)p->ny, p->n2, p->N);
+  else
+    hipLaunchKernelGGL((k_solve_sym2<NX, false>), grid, dim3(dma::NT), 0, s, P, (float2*)x, p->tab(1, 0),
+                       p->tab(1, 1), p->d_symd.as<double2>(), (int)p->ny, p->n2, p->N);
   HZ_HIP(hipGetLastError());
   return true;
 }
@@ -1921,12 +1970,12 @@ bool pack_sym_any(hz_problem* p, int prec, hipStream_t s) {
   return false;
 }
 
-bool solve_sym_any(hz_problem* p, int prec, int nrhs, void* x, hipStream_t s) {
+bool solve_sym_any(hz_problem* p, int prec, int nrhs, void* x, bool adjoint, hipStream_t s) {
   const char* v1 = getenv("HZ_SYM_V1");
-  const bool unrolled = !(v1 && v1[0] == '1');
-#define HZ_SYM_SOLVE(P, T, RPL, NCG, TB, NX)                                  \
-  if (prec == P && unrolled && try_solve_sym2<NX>(p, prec, nrhs, x, s)) return true; \
-  if (prec == P && try_solve_sym<T, RPL, NCG, TB, NX>(p, prec, nrhs, x, s)) return true;
+  const bool unrolled = adjoint || !(v1 && v1[0] == '1');
+#define HZ_SYM_SOLVE(P, T, RPL, NCG, TB, NX)                                                     \
+  if (prec == P && unrolled && try_solve_sym2<NX>(p, prec, nrhs, x, adjoint, s)) return true; \
+  if (prec == P && !adjoint && try_solve_sym<T, RPL, NCG, TB, NX>(p, prec, nrhs, x, s)) return true;
   HZ_SYM_WIDTHS(HZ_SYM_SOLVE)
 #undef HZ_SYM_SOLVE
   return false;
@@ -2113,6 +2162,12 @@ void launch_plane_solve_with(hz_problem* p, int prec, int nrhs, void* x, bool ad
   const double2* cpy = p->tab(1, 1);
   dim3 grid((unsigned)p->nz, (unsigned)nrhs);
   SolveKernel sk = solve_kernel_choice();
+  // adjoint sweeps with G_j^H = conj(D_j^-1 P_j conj(.)) on the packed factors
+  // (the tuned families are forward kernels), unless the environment asks for
+  // a full-block kernel
+  if (adjoint && ny >= 1 && (sk == SK_DEFAULT || sk == SK_SYM) && f.sym_ready &&
+      reinterpret_cast<uintptr_t>(x) % 16 == 0 && solve_sym_any(p, prec, nrhs, x, true, s))
+    return;
   if (sk == SK_DEFAULT && t.solve_family == HZ_SOLVE_STREAM) sk = SK_MR;
   if (sk == SK_DEFAULT && t.solve_family == HZ_SOLVE_CHUNKED) sk = SK_FAST;
   if (sk == SK_DEFAULT && t.solve_family == HZ_SOLVE_STREAM_1) sk = SK_DMA;
@@ -2121,7 +2176,7 @@ void launch_plane_solve_with(hz_problem* p, int prec, int nrhs, void* x, bool ad
   // untuned default for one right-hand side: the symmetric packed stream where it is built
   if (sk == SK_DEFAULT && t.solve_family == HZ_SOLVE_DEFAULT && nrhs == 1 && f.sym_ready) sk = SK_SYM;
   if (!adjoint && ny >= 1 && sk == SK_SYM && f.sym_ready && reinterpret_cast<uintptr_t>(x) % 16 == 0 &&
-      solve_sym_any(p, prec, nrhs, x, s))
+      solve_sym_any(p, prec, nrhs, x, false, s))
     return;
   if (sk == SK_SYM) sk = SK_DEFAULT;
 

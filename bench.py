@@ -32,7 +32,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=200, help="grid nodes per axis (200: the 200^3 configuration)")
     ap.add_argument("--npml", type=int, default=None)
-    ap.add_argument("--m", type=int, default=45, help="FGMRES restart length")
+    ap.add_argument("--m", type=int, default=35, help="FGMRES restart length")
     ap.add_argument("--beta", type=float, default=1.0, help="complex shift of the preconditioner")
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--tol", type=float, default=1e-8)

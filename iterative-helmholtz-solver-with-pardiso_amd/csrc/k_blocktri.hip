@@ -994,10 +994,16 @@ struct MfShape {
   static_assert(lds <= 160 * 1024, "LDS budget");
 };
 
-template <int NX, int TCOL>
+// ADJ: the adjoint sweep in conjugated variables with G_j^H = conj(D_j^-1 G_j D_j)
+// (G_j D_j symmetric, D_j = diag(dx(i) dy(j)) from symd = dx | dy | 1/dx | 1/dy):
+// the vector v holds D_j u_j, the product z = G_j v gives Y_j = D_j^-1 z; forward
+// u_{j+1} = conj(b_{j+1}) - cp_j Y_j (Y_j stored), backward X_j = Y_j - D_j^-1 z,
+// x_j = conj(X_j), u = cm_j X_j.  Same block sequence as the forward solve.
+template <int NX, int TCOL, bool ADJ>
 __global__ __launch_bounds__(dma::NT) void k_solve_mfma8(const float2* __restrict__ G, float2* __restrict__ X,
                                                          const double2* __restrict__ cmy,
-                                                         const double2* __restrict__ cpy, int ny, int64_t n2,
+                                                         const double2* __restrict__ cpy,
+                                                         const double2* __restrict__ symd, int ny, int64_t n2,
                                                          int64_t N) {
   typedef MfShape<NX, TCOL> S;
   typedef float4 U;
@@ -1006,6 +1012,7 @@ __global__ __launch_bounds__(dma::NT) void k_solve_mfma8(const float2* __restric
   __shared__ U s0[R * dma::NT], s1[R * dma::NT], s2[R * dma::NT], s3[R * dma::NT];
   __shared__ U pbuf[8 * TPW * 64];
   __shared__ float2 v[n * NR];
+  __shared__ float2 sdx[ADJ ? n : 1], sdxi[ADJ ? n : 1];  // dx, 1/dx (adjoint only)
 
   const int k = blockIdx.x;
   const int64_t rhs0 = (int64_t)blockIdx.y * NR;
@@ -1070,7 +1077,36 @@ __global__ __launch_bounds__(dma::NT) void k_solve_mfma8(const float2* __restric
           const int row_ = 16 * rt_ + 4 * lg + r_;                                                        \
           const float2 y_ = make_float2(acc[a_][r_], im_[r_]);                                            \
           float2 nv_;                                                                                     \
-          if (se_ < ny - 1) {                                                                             \
+          if (ADJ) {                                                                                      \
+            const int jc_ = blk_of(se_);                                                                  \
+            const double2 dyi_ = symd[2 * n + ny + jc_];                                                  \
+            const float2 sxi_ = sdxi[row_ < n ? row_ : 0];                                                \
+            const double sx_ = (double)sxi_.x * dyi_.x - (double)sxi_.y * dyi_.y;                         \
+            const double sy_ = (double)sxi_.x * dyi_.y + (double)sxi_.y * dyi_.x;                         \
+            const double zx_ = y_.x * sx_ - y_.y * sy_, zy_ = y_.x * sy_ + y_.y * sx_;                    \
+            double ux_, uy_;                                                                              \
+            int jn_;                                                                                      \
+            if (se_ < ny - 1) {                                                                           \
+              const double2 cp = cpy[se_];                                                                \
+              out_[r_] = make_float2((float)zx_, (float)zy_);                                             \
+              ux_ = (double)pre_[r_].x - (cp.x * zx_ - cp.y * zy_);                                       \
+              uy_ = -(double)pre_[r_].y - (cp.x * zy_ + cp.y * zx_);                                      \
+              jn_ = se_ + 1;                                                                              \
+            } else {                                                                                      \
+              const double Xx_ = se_ >= ny ? (double)pre_[r_].x - zx_ : zx_;                              \
+              const double Xy_ = se_ >= ny ? (double)pre_[r_].y - zy_ : zy_;                              \
+              out_[r_] = make_float2((float)Xx_, (float)(-Xy_));                                          \
+              const double2 cm = cmy[jc_];                                                                \
+              ux_ = cm.x * Xx_ - cm.y * Xy_;                                                              \
+              uy_ = cm.x * Xy_ + cm.y * Xx_;                                                              \
+              jn_ = jc_ > 0 ? jc_ - 1 : 0;                                                                \
+            }                                                                                             \
+            const double2 dyn_ = symd[n + jn_];                                                           \
+            const float2 sxn_ = sdx[row_ < n ? row_ : 0];                                                 \
+            const double tx_ = (double)sxn_.x * dyn_.x - (double)sxn_.y * dyn_.y;                         \
+            const double ty_ = (double)sxn_.x * dyn_.y + (double)sxn_.y * dyn_.x;                         \
+            nv_ = make_float2((float)(ux_ * tx_ - uy_ * ty_), (float)(ux_ * ty_ + uy_ * tx_));            \
+          } else if (se_ < ny - 1) {                                                                      \
             const double2 cm = cmy[se_ + 1];                                                              \
             nv_.x = pre_[r_].x - (float)(cm.x * y_.x - cm.y * y_.y);                                      \
             nv_.y = pre_[r_].y - (float)(cm.x * y_.y + cm.y * y_.x);                                      \
@@ -1088,7 +1124,7 @@ __global__ __launch_bounds__(dma::NT) void k_solve_mfma8(const float2* __restric
           }                                                                                               \
           if (row_ < n) v[row_ * NR + q_] = nv_;                                                          \
         }                                                                                                 \
-        const int line_ = se_ < ny - 1 ? se_ + 1 : 2 * ny - 2 - se_;                                      \
+        const int line_ = ADJ ? blk_of(se_) : (se_ < ny - 1 ? se_ + 1 : 2 * ny - 2 - se_);                \
         float2* xq_ = xk + (int64_t)q_ * N + (size_t)n * line_ + 16 * rt_ + 4 * lg;                       \
         if (16 * rt_ + 4 * lg + 1 < n)                                                                    \
           *reinterpret_cast<U*>(xq_) = make_float4(out_[0].x, out_[0].y, out_[1].x, out_[1].y);           \
@@ -1109,7 +1145,8 @@ __global__ __launch_bounds__(dma::NT) void k_solve_mfma8(const float2* __restric
       _Pragma("unroll") for (int a = 0; a < TPW; ++a) acc[a] = hz_v4f{0.f, 0.f, 0.f, 0.f};    \
       int slot_line = 0;                                                                      \
       if (s < ny - 1) slot_line = s + 1;                                                      \
-      else if (2 * ny - 2 - s > 0) slot_line = 2 * ny - 3 - s;                                \
+      else if (ADJ && s >= ny) slot_line = 2 * ny - 2 - s;                                    \
+      else if (!ADJ && 2 * ny - 2 - s > 0) slot_line = 2 * ny - 3 - s;                        \
       HZ_MF_LINE(slot_line)                                                                   \
     }                                                                                         \
     if (Tc < TT) {                                                                            \
@@ -1132,9 +1169,26 @@ __global__ __launch_bounds__(dma::NT) void k_solve_mfma8(const float2* __restric
     if (Tc < TT && t == tps - 1) HZ_MF_EPILOGUE(s)                                            \
   }
 
+  if (ADJ) {
+    for (int i = tid; i < n; i += dma::NT) {
+      const double2 a = symd[i], b = symd[n + ny + i];
+      sdx[i] = make_float2((float)a.x, (float)a.y);
+      sdxi[i] = make_float2((float)b.x, (float)b.y);
+    }
+    __syncthreads();
+  }
   for (int i = tid; i < n * NR; i += dma::NT) {
     const int q = i % NR, row = i / NR;
-    v[i] = xk[(int64_t)q * N + row];
+    const float2 b = xk[(int64_t)q * N + row];
+    if (ADJ) {
+      // v = D_0 conj(b_0)
+      const float2 a = sdx[row];
+      const double2 d0 = symd[n];
+      const double tx = (double)a.x * d0.x - (double)a.y * d0.y, ty = (double)a.x * d0.y + (double)a.y * d0.x;
+      v[i] = make_float2((float)(b.x * tx + b.y * ty), (float)(b.x * ty - b.y * tx));
+    } else {
+      v[i] = b;
+    }
   }
   __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
   HZ_MF_TILE(0, s0)
@@ -1155,22 +1209,27 @@ __global__ __launch_bounds__(dma::NT) void k_solve_mfma8(const float2* __restric
 
 template <int NX, int TCOL>
 bool try_launch_mfma8(int n, int ny, int nz, int groups, hipStream_t s, const void* G, void* x, const double2* cmy,
-                      const double2* cpy, int64_t n2, int64_t N) {
+                      const double2* cpy, const double2* symd, bool adjoint, int64_t n2, int64_t N) {
   if (n != NX) return false;
-  hipLaunchKernelGGL((k_solve_mfma8<NX, TCOL>), dim3((unsigned)nz, (unsigned)groups), dim3(dma::NT), 0, s,
-                     (const float2*)G, (float2*)x, cmy, cpy, ny, n2, N);
+  const dim3 grid((unsigned)nz, (unsigned)groups);
+  if (adjoint)
+    hipLaunchKernelGGL((k_solve_mfma8<NX, TCOL, true>), grid, dim3(dma::NT), 0, s, (const float2*)G, (float2*)x, cmy,
+                       cpy, symd, ny, n2, N);
+  else
+    hipLaunchKernelGGL((k_solve_mfma8<NX, TCOL, false>), grid, dim3(dma::NT), 0, s, (const float2*)G, (float2*)x,
+                       cmy, cpy, symd, ny, n2, N);
   HZ_HIP(hipGetLastError());
   return true;
 }
 
 bool launch_mfma8_any(int n, int ny, int nz, int groups, hipStream_t s, const void* G, void* x, const double2* cmy,
-                      const double2* cpy, int64_t n2, int64_t N) {
-  return try_launch_mfma8<200, 20>(n, ny, nz, groups, s, G, x, cmy, cpy, n2, N) ||
-         try_launch_mfma8<192, 16>(n, ny, nz, groups, s, G, x, cmy, cpy, n2, N) ||
-         try_launch_mfma8<160, 20>(n, ny, nz, groups, s, G, x, cmy, cpy, n2, N) ||
-         try_launch_mfma8<128, 32>(n, ny, nz, groups, s, G, x, cmy, cpy, n2, N) ||
-         try_launch_mfma8<96, 32>(n, ny, nz, groups, s, G, x, cmy, cpy, n2, N) ||
-         try_launch_mfma8<64, 32>(n, ny, nz, groups, s, G, x, cmy, cpy, n2, N);
+                      const double2* cpy, const double2* symd, bool adjoint, int64_t n2, int64_t N) {
+  return try_launch_mfma8<200, 20>(n, ny, nz, groups, s, G, x, cmy, cpy, symd, adjoint, n2, N) ||
+         try_launch_mfma8<192, 16>(n, ny, nz, groups, s, G, x, cmy, cpy, symd, adjoint, n2, N) ||
+         try_launch_mfma8<160, 20>(n, ny, nz, groups, s, G, x, cmy, cpy, symd, adjoint, n2, N) ||
+         try_launch_mfma8<128, 32>(n, ny, nz, groups, s, G, x, cmy, cpy, symd, adjoint, n2, N) ||
+         try_launch_mfma8<96, 32>(n, ny, nz, groups, s, G, x, cmy, cpy, symd, adjoint, n2, N) ||
+         try_launch_mfma8<64, 32>(n, ny, nz, groups, s, G, x, cmy, cpy, symd, adjoint, n2, N);
 }
 
 // ------------------------------------------ symmetric packed plane solve
@@ -2162,12 +2221,22 @@ void launch_plane_solve_with(hz_problem* p, int prec, int nrhs, void* x, bool ad
   const double2* cpy = p->tab(1, 1);
   dim3 grid((unsigned)p->nz, (unsigned)nrhs);
   SolveKernel sk = solve_kernel_choice();
-  // adjoint sweeps with G_j^H = conj(D_j^-1 P_j conj(.)) on the packed factors
-  // (the tuned families are forward kernels), unless the environment asks for
-  // a full-block kernel
-  if (adjoint && ny >= 1 && (sk == SK_DEFAULT || sk == SK_SYM) && f.sym_ready &&
-      reinterpret_cast<uintptr_t>(x) % 16 == 0 && solve_sym_any(p, prec, nrhs, x, true, s))
-    return;
+  // adjoint sweeps (the tuned families are forward kernels): complex64 groups
+  // of 8 on the matrix cores with G_j^H = conj(D_j^-1 G_j D_j), the rest on the
+  // packed symmetric factors with G_j^H = conj(D_j^-1 P_j conj(.)), unless the
+  // environment asks for another kernel family
+  if (adjoint && ny >= 1 && prec == HZ_PREC_C64 && reinterpret_cast<uintptr_t>(x) % 16 == 0 &&
+      (sk == SK_DEFAULT || sk == SK_SYM || sk == SK_MFMA)) {
+    if (sk != SK_SYM && nrhs >= 8 && mfma_solve_available(prec, n)) {
+      const int g = nrhs / 8;
+      launch_mfma8_any(n, ny, (int)p->nz, g, s, f.G.ptr, x, cmy, cpy, p->d_symd.as<double2>(), true, p->n2, p->N);
+      nrhs -= 8 * g;
+      if (nrhs == 0) return;
+      x = reinterpret_cast<float2*>(x) + (size_t)8 * g * p->N;
+      grid = dim3((unsigned)p->nz, (unsigned)nrhs);
+    }
+    if (sk != SK_MFMA && f.sym_ready && solve_sym_any(p, prec, nrhs, x, true, s)) return;
+  }
   if (sk == SK_DEFAULT && t.solve_family == HZ_SOLVE_STREAM) sk = SK_MR;
   if (sk == SK_DEFAULT && t.solve_family == HZ_SOLVE_CHUNKED) sk = SK_FAST;
   if (sk == SK_DEFAULT && t.solve_family == HZ_SOLVE_STREAM_1) sk = SK_DMA;
@@ -2190,7 +2259,7 @@ void launch_plane_solve_with(hz_problem* p, int prec, int nrhs, void* x, bool ad
       reinterpret_cast<uintptr_t>(x) % 16 == 0) {
     // groups of 8 on the matrix cores, the remainder on the VALU streaming kernel
     const int g = nrhs / 8;
-    launch_mfma8_any(n, ny, (int)p->nz, g, s, f.G.ptr, x, cmy, cpy, p->n2, p->N);
+    launch_mfma8_any(n, ny, (int)p->nz, g, s, f.G.ptr, x, cmy, cpy, p->d_symd.as<double2>(), false, p->n2, p->N);
     const int rest = nrhs - 8 * g;
     if (rest == 0) return;
     x = reinterpret_cast<float2*>(x) + (size_t)8 * g * p->N;

@@ -68,7 +68,9 @@ struct PlaneTabs {
   const double2* sig;
 };
 
+typedef float hz_v4f __attribute__((ext_vector_type(4)));
 constexpr int FACT_THREADS = 512;
+constexpr int TCM = 64;  // columns per trailing-update chunk on the matrix cores (complex64)
 constexpr int TC = 32;   // columns per trailing-update tile
 constexpr int CB = 4;    // columns per thread in the trailing update
 
@@ -78,8 +80,8 @@ __global__ __launch_bounds__(FACT_THREADS) void k_factor(typename Cx<T>::t* __re
   typedef typename Cx<T>::t C;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C* P = reinterpret_cast<C*>(smem_raw);     // nb * n, column-major panel
-  C* AK = P + (size_t)nb * n;                // TC * nb (AK[cl * nb + kk])
-  C* prow = AK + (size_t)TC * nb;            // nb
+  C* AK = P + (size_t)nb * n;                // (TCM or TC) * nb (AK[cl * nb + kk])
+  C* prow = AK + (size_t)(sizeof(T) == 4 ? TCM : TC) * nb;  // nb
   C* pcol = prow + nb;                       // n
   int* piv = reinterpret_cast<int*>(pcol + n);  // n
   int* sh = piv + n;                         // scratch ints
@@ -199,8 +201,69 @@ __global__ __launch_bounds__(FACT_THREADS) void k_factor(typename Cx<T>::t* __re
         const int r = idx % n, c = idx / n;
         Gj[r + (size_t)n * (p0 + c)] = P[c * n + r];
       }
-      // trailing update of the columns outside the panel
+      // trailing update of the columns outside the panel: G[:, rest] = P AK
+      // (+ the old value outside the panel rows), AK = the panel rows of those
+      // columns; complex64 on the matrix cores, complex128 on the vector units
       const int nrest = n - kb;
+      if constexpr (sizeof(T) == 4) {
+        // 16-row x 8-column tasks, C = [Re Y | Im Y] from two FP32 MFMAs per
+        // k-slice: Re(P) [Re AK | Im AK] + Im(P) [-Im AK | Re AK]
+        const int li = lane & 15, lg = lane >> 4;
+        const int RTn = (n + 15) / 16;
+        for (int c0 = 0; c0 < nrest; c0 += TCM) {
+          const int tc = min(TCM, nrest - c0);
+          __syncthreads();
+          for (int idx = tid; idx < kb * tc; idx += nt) {
+            const int kk = idx % kb, cl = idx / kb;
+            const int cg = c0 + cl;
+            const int cc = cg < p0 ? cg : cg + kb;
+            AK[cl * nb + kk] = Gj[(p0 + kk) + (size_t)n * cc];
+          }
+          __syncthreads();
+          const int CGn = (tc + 7) / 8;
+          for (int task = wave; task < RTn * CGn; task += nt / 64) {
+            const int rt = task % RTn, cgp = task / RTn;
+            hz_v4f acc = {0.f, 0.f, 0.f, 0.f};
+            for (int ks = 0; 4 * ks < kb; ++ks) {
+              const int kq = 4 * ks + lg;
+              C ga, bv;
+              ga.x = ga.y = bv.x = bv.y = 0;
+              if (kq < kb) {
+                ga = P[kq * n + 16 * rt + li];  // rows past n read the next column (discarded)
+                bv = AK[(cgp * 8 + (li & 7)) * nb + kq];
+              }
+              const float b1 = li < 8 ? bv.x : bv.y;
+              const float b2 = li < 8 ? -bv.y : bv.x;
+              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ga.x, b1, acc, 0, 0, 0);
+              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ga.y, b2, acc, 0, 0, 0);
+            }
+            float im[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) im[r] = __shfl_xor(acc[r], 8);
+            const int col = cgp * 8 + li;
+            if (li < 8 && col < tc) {
+              const int cg = c0 + col;
+              const int cc = cg < p0 ? cg : cg + kb;
+#pragma unroll
+              for (int r = 0; r < 4; ++r) {
+                const int row = 16 * rt + 4 * lg + r;
+                if (row < n) {
+                  C* dst = &Gj[row + (size_t)n * cc];
+                  C y;
+                  y.x = acc[r];
+                  y.y = im[r];
+                  if (row < p0 || row >= p0 + kb) {
+                    const C old = *dst;
+                    y.x += old.x;
+                    y.y += old.y;
+                  }
+                  *dst = y;
+                }
+              }
+            }
+          }
+        }
+      } else {
       for (int c0 = 0; c0 < nrest; c0 += TC) {
         const int tc = min(TC, nrest - c0);
         __syncthreads();
@@ -234,6 +297,7 @@ __global__ __launch_bounds__(FACT_THREADS) void k_factor(typename Cx<T>::t* __re
           for (int b = 0; b < CB; ++b)
             if (q * CB + b < tc) Gj[r + (size_t)n * ccs[b]] = acc[b];
         }
+      }
       }
       __syncthreads();
     }
@@ -972,7 +1036,6 @@ __global__ __launch_bounds__(dma::NT) void k_solve_mr(const typename Cx<T>::t* _
 // 4 (l / 16) + 0..3 of column l % 16.  The B operands of a k-slice come from
 // the vectors in LDS once per wave and serve every row tile of the wave.
 // Wave w owns row tiles w and w + 8.
-typedef float hz_v4f __attribute__((ext_vector_type(4)));
 
 template <int NX, int TCOL>
 struct MfShape {
@@ -2158,7 +2221,8 @@ PlaneTabs plane_tabs(hz_problem* p) {
 template <typename T>
 size_t factor_lds(int n, int nb) {
   typedef typename Cx<T>::t C;
-  return sizeof(C) * ((size_t)nb * n + (size_t)TC * nb + nb + n) + sizeof(int) * (n + 4);
+  const size_t tc = sizeof(T) == 4 ? TCM : TC;  // trailing-update chunk (matrix cores for complex64)
+  return sizeof(C) * ((size_t)nb * n + tc * nb + nb + n) + sizeof(int) * (n + 4);
 }
 
 }  // namespace

@@ -2032,6 +2032,277 @@ __global__ __launch_bounds__(dma::NT) void k_solve_sym2(const float2* __restrict
   __builtin_amdgcn_s_waitcnt(dma::wait_vm(0));
 }
 
+// ---- complex128 form of the step-unrolled packed solve
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double x) {
+  const long long b = __builtin_bit_cast(long long, x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+template <int CTRL>
+__device__ __forceinline__ double pair_step_d(double a, double b, bool hi) {
+  const double keep = hi ? b : a, send = hi ? a : b;
+  return keep + dpp_d<CTRL>(send);
+}
+// wave sums of NV (8 or 16) values per lane: afterwards lane l holds the sum
+// over all 64 lanes of x[l & (NV - 1)] (transpose-reduce over each 16-lane
+// row, then the four rows by lane exchanges)
+template <int NV>
+__device__ __forceinline__ double wave_sum_values_d(const double* x, int lane) {
+  static_assert(NV == 8 || NV == 16, "values per lane");
+  const bool b0 = lane & 1, b1 = lane & 2, b2 = lane & 4, b3 = lane & 8;
+  double w[NV / 2];
+#pragma unroll
+  for (int i = 0; i < NV / 2; ++i) w[i] = pair_step_d<0xB1>(x[2 * i], x[2 * i + 1], b0);  // xor 1
+  double y[NV / 4];
+#pragma unroll
+  for (int i = 0; i < NV / 4; ++i) y[i] = pair_step_d<0x4E>(w[2 * i], w[2 * i + 1], b1);  // xor 2
+  double u;
+  if constexpr (NV == 16) {
+    const double z0 = pair_step_d<0x124>(y[0], y[1], b2);  // row_ror:4
+    const double z1 = pair_step_d<0x124>(y[2], y[3], b2);
+    u = pair_step_d<0x128>(z0, z1, b3);                   // row_ror:8
+  } else {
+    u = pair_step_d<0x124>(y[0], y[1], b2);
+    u += dpp_d<0x128>(u);
+  }
+  u += __shfl_xor(u, 16);
+  u += __shfl_xor(u, 32);
+  return u;
+}
+
+// Complex128 packed plane solve (forward), same structure as k_solve_sym2: a
+// 16-byte unit is one complex128 row entry; two column groups of 4 columns
+// per block column; lane (cg, ru) takes row ru = rb + lane; the transposed
+// values (2 block columns x 4 columns x re/im) are summed over the wave and
+// stored as one partial per row wave (4 per column).
+template <int NX>
+__global__ __launch_bounds__(dma::NT) void k_solve_sym2d(const double2* __restrict__ P, double2* __restrict__ X,
+                                                         const double2* __restrict__ cmy,
+                                                         const double2* __restrict__ cpy,
+                                                         const double2* __restrict__ symd, int ny, int64_t n2,
+                                                         int64_t N) {
+  typedef double T;
+  typedef double2 C;
+  typedef double2 U;
+  constexpr int RPL = 1, NCG = 2, TB = 8;
+  typedef SymLayout<NX, TB> L;
+  constexpr int n = NX, nru = n, CPB = TB / NCG, RUP = dma::NT / NCG, NQ = RUP / 64;
+  constexpr int TPS = L::TPS, NPAIR = L::NPAIR, PAIR = L::PAIR, BS = L::BS, T_ = L::T;
+  constexpr int R = (PAIR + dma::NT - 1) / dma::NT;  // DMA rounds per tile (PAIR units)
+  constexpr int ZU = R * dma::NT, SU = ZU + 1;
+  constexpr int PU = n;                              // units of one y-line
+  constexpr int LW = (PU + 63) / 64;                 // waves fetching the next line
+  constexpr int OTHER = NCG * n * 16 + NQ * n * 16 + 2 * n * 16 + (LW + 1) * 64 * 16;
+  constexpr int NS_LDS = (160 * 1024 - OTHER) / (SU * 16);
+  constexpr int NS = NS_LDS < TPS + 1 ? NS_LDS : TPS + 1;
+  static_assert(NS >= 3 && (NS - 1) * R <= 63 && nru <= RUP && LW < 8, "ring and thread split");
+  static_assert(CPB == 4, "transposed values: 2 block columns x 4 columns x re/im");
+  __shared__ U ring[NS * SU];
+  __shared__ C part[NCG * n];
+  __shared__ __align__(16) T tdot[NQ * n * 2];
+  __shared__ C v[n];
+  __shared__ U pbuf[(LW + 1) * 64];
+  __shared__ C sdxi[n];
+  const C* pline = pbuf;
+  const double2* pscal = &pbuf[LW * 64];
+
+  const int k = blockIdx.x;
+  const int64_t rhs = blockIdx.y;
+  C* x = X + rhs * N + (int64_t)k * n2;
+  const C* Pk = P + (size_t)k * ny * BS;
+  const double2* dxi = symd + n + ny;
+  const double2* dyi = symd + 2 * n + ny;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int cg = wv % NCG, rb = (wv / NCG) * 64, ru = rb + lane;
+  const bool worker = ru < nru;
+  const int nsteps = 2 * ny - 1;
+  auto blk_of = [&](int st) { return st <= ny - 1 ? st : 2 * ny - 2 - st; };
+  const unsigned ring_lds = (unsigned)(size_t)(&ring[0]);
+  const unsigned pbuf_lds = (unsigned)(size_t)(&pbuf[0]);
+  auto cmulc = [](C a, C b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); };
+
+  for (int i = tid; i < NQ * n * 2; i += dma::NT) tdot[i] = 0;
+  if (tid < NS) ring[tid * SU + ZU] = make_double2(0.0, 0.0);
+  for (int i = tid; i < n; i += dma::NT) sdxi[i] = dxi[i];
+  __syncthreads();
+  for (int i = tid; i < n; i += dma::NT) v[i] = cmulc(x[i], cmulc(sdxi[i], dyi[blk_of(0)]));
+  __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
+
+  auto tile_dma = [&](const char* blk, int tt, int sl) {
+    const int tu = tt < NPAIR ? PAIR : (NPAIR + 1) * TB * TB;
+    const char* base = blk + (size_t)tt * PAIR * sizeof(C);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int u = min(r * dma::NT + tid, tu - 1);
+      dma16_untracked(base + (size_t)u * 16, ring_lds + (unsigned)((sl * SU + r * dma::NT + wv * 64) * 16));
+    }
+  };
+  auto blk_addr = [&](int st) {
+    return reinterpret_cast<const char*>(Pk + (size_t)blk_of(st < nsteps ? st : 0) * BS);
+  };
+#pragma unroll
+  for (int g = 0; g < NS - 1; ++g) tile_dma(blk_addr(0), g, g);
+
+  C acc = make_double2(0.0, 0.0);
+  C vr = make_double2(0.0, 0.0);
+  int slot = 0;
+  for (int st = 0; st < nsteps; ++st) {
+    const char* b0 = blk_addr(st);
+    const char* b1 = blk_addr(st + 1);
+#pragma unroll
+    for (int t = 0; t < TPS; ++t) {
+      __builtin_amdgcn_s_waitcnt(dma::wait_vm((NS - 2) * R));
+      __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
+      dma::raw_barrier();
+      const U* SL = ring + slot * SU;
+      if (t == 0) {
+        acc = make_double2(0.0, 0.0);
+        vr = v[worker ? ru : 0];
+        if (!worker) vr = make_double2(0.0, 0.0);
+        int line = 0;
+        if (st < ny - 1) line = st + 1;
+        else if (2 * ny - 2 - st > 0) line = 2 * ny - 3 - st;
+        if (wv < LW) {
+          const char* lb = reinterpret_cast<const char*>(x + (size_t)n * line);
+          dma16_untracked(lb + (size_t)min(tid, PU - 1) * 16, pbuf_lds + (unsigned)(wv * 64 * 16));
+        } else if (wv == LW) {
+          const int sn = min(st + 1, nsteps - 1);
+          const int jb = min(max(2 * ny - 3 - st, 0), ny - 1);
+          const double2* src = lane == 1 ? cpy + jb : (lane == 2 ? dyi + blk_of(sn) : cmy + min(st + 1, ny - 1));
+          dma16_untracked(src, pbuf_lds + (unsigned)(LW * 64 * 16));
+        }
+      }
+      {
+        const int J1 = t, J2 = T_ - 1 - t;
+        const int Hu1 = (J1 + 1) * TB, Hu2 = (J2 + 1) * TB;
+        const int nu1 = J1 * TB, nu2 = J2 * TB;
+        const int ub2 = (J1 + 1) * TB * TB;
+        const bool two = J2 != J1;
+        const C zc = make_double2(0.0, 0.0);
+        if (rb < Hu1) {
+          const bool ok1 = worker && ru < Hu1;
+          const bool ok2 = worker && two && ru < Hu2;
+          const int g1 = ok1 ? cg * CPB * Hu1 + ru : ZU;
+          const int g2 = ok2 ? ub2 + cg * CPB * Hu2 + ru : ZU;
+          const int st1 = ok1 ? Hu1 : 0, st2 = ok2 ? Hu2 : 0;
+          C a1[CPB], a2[CPB];
+#pragma unroll
+          for (int cc = 0; cc < CPB; ++cc) {
+            a1[cc] = SL[g1 + cc * st1];
+            a2[cc] = SL[g2 + cc * st2];
+          }
+          const C w1 = ru < nu1 ? vr : zc, w2 = ru < nu2 ? vr : zc;
+          T tv[16];
+#pragma unroll
+          for (int cc = 0; cc < CPB; ++cc) {
+            const C v1 = v[J1 * TB + cg * CPB + cc], v2 = v[J2 * TB + cg * CPB + cc];
+            acc.x = fma(a1[cc].x, v1.x, acc.x);
+            acc.x = fma(-a1[cc].y, v1.y, acc.x);
+            acc.y = fma(a1[cc].x, v1.y, acc.y);
+            acc.y = fma(a1[cc].y, v1.x, acc.y);
+            acc.x = fma(a2[cc].x, v2.x, acc.x);
+            acc.x = fma(-a2[cc].y, v2.y, acc.x);
+            acc.y = fma(a2[cc].x, v2.y, acc.y);
+            acc.y = fma(a2[cc].y, v2.x, acc.y);
+            tv[2 * cc] = fma(-a1[cc].y, w1.y, a1[cc].x * w1.x);
+            tv[2 * cc + 1] = fma(a1[cc].y, w1.x, a1[cc].x * w1.y);
+            tv[8 + 2 * cc] = fma(-a2[cc].y, w2.y, a2[cc].x * w2.x);
+            tv[8 + 2 * cc + 1] = fma(a2[cc].y, w2.x, a2[cc].x * w2.y);
+          }
+          const T sm = wave_sum_values_d<16>(tv, lane);
+          const int kk = lane & 15;
+          if (lane < 16 && (two || kk < 8)) {
+            const int col = ((kk & 8) ? J2 : J1) * TB + cg * CPB + ((kk >> 1) & 3);
+            tdot[((rb >> 6) * n + col) * 2 + (kk & 1)] = sm;
+          }
+        } else if (rb < Hu2 && two) {
+          const bool ok2 = worker && ru < Hu2;
+          const int g2 = ok2 ? ub2 + cg * CPB * Hu2 + ru : ZU;
+          const int st2 = ok2 ? Hu2 : 0;
+          const C w2 = ru < nu2 ? vr : zc;
+          T tv[8];
+#pragma unroll
+          for (int cc = 0; cc < CPB; ++cc) {
+            const C a2 = SL[g2 + cc * st2];
+            const C v2 = v[J2 * TB + cg * CPB + cc];
+            acc.x = fma(a2.x, v2.x, acc.x);
+            acc.x = fma(-a2.y, v2.y, acc.x);
+            acc.y = fma(a2.x, v2.y, acc.y);
+            acc.y = fma(a2.y, v2.x, acc.y);
+            tv[2 * cc] = fma(-a2.y, w2.y, a2.x * w2.x);
+            tv[2 * cc + 1] = fma(a2.y, w2.x, a2.x * w2.y);
+          }
+          const T sm = wave_sum_values_d<8>(tv, lane);
+          const int kk = lane & 7;
+          if (lane < 8) {
+            const int col = J2 * TB + cg * CPB + (kk >> 1);
+            tdot[((rb >> 6) * n + col) * 2 + (kk & 1)] = sm;
+          }
+        }
+      }
+      {
+        const int ta = t + NS - 1;
+        const int tt = ta % TPS;
+        const char* blk = (ta / TPS) == 0 ? b0 : b1;
+        const bool past = (ta / TPS) != 0 && st + 1 >= nsteps;
+        tile_dma(past ? reinterpret_cast<const char*>(Pk) : blk, past ? 0 : tt, slot == 0 ? NS - 1 : slot - 1);
+      }
+      if (t == TPS - 1) {
+        if (worker) part[cg * n + ru] = acc;
+        __builtin_amdgcn_s_waitcnt(dma::wait_vm((NS - 1) * R));
+        __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
+        dma::raw_barrier();
+        if (tid < n) {
+          C y = make_double2(0.0, 0.0);
+#pragma unroll
+          for (int q = 0; q < NCG; ++q) {
+            const C u = part[q * n + tid];
+            y.x += u.x; y.y += u.y;
+          }
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) {
+            y.x += tdot[(q * n + tid) * 2];
+            y.y += tdot[(q * n + tid) * 2 + 1];
+          }
+          const C pre = pline[tid];
+          C nv;
+          if (st < ny - 1) {
+            const double2 cm = pscal[0];
+            nv = make_double2(pre.x - (cm.x * y.x - cm.y * y.y), pre.y - (cm.x * y.y + cm.y * y.x));
+            x[(size_t)n * (st + 1) + tid] = nv;
+          } else {
+            const int j = 2 * ny - 2 - st;
+            x[(size_t)n * j + tid] = y;
+            if (j > 0) {
+              const double2 cp = pscal[1];
+              nv = make_double2(pre.x - (cp.x * y.x - cp.y * y.y), pre.y - (cp.x * y.y + cp.y * y.x));
+            } else {
+              nv = y;
+            }
+          }
+          if (st + 1 < nsteps) v[tid] = cmulc(nv, cmulc(sdxi[tid], pscal[2]));
+        }
+        __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
+      }
+      slot = slot + 1 == NS ? 0 : slot + 1;
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(dma::wait_vm(0));
+}
+
+template <int NX>
+bool try_solve_sym2d(hz_problem* p, int prec, int nrhs, void* x, hipStream_t s) {
+  if (p->nx != NX || prec != HZ_PREC_FP64) return false;
+  hipLaunchKernelGGL((k_solve_sym2d<NX>), dim3((unsigned)p->nz, (unsigned)nrhs), dim3(dma::NT), 0, s,
+                     p->fac[prec].Gsym.as<double2>(), (double2*)x, p->tab(1, 0), p->tab(1, 1),
+                     p->d_symd.as<double2>(), (int)p->ny, p->n2, p->N);
+  HZ_HIP(hipGetLastError());
+  return true;
+}
+
 template <typename T, int RPL, int NCG, int TB, int NX>
 bool try_pack_sym(hz_problem* p, int prec, hipStream_t s) {
   if (p->nx != NX) return false;
@@ -2089,6 +2360,9 @@ bool pack_sym_any(hz_problem* p, int prec, hipStream_t s) {
   if (prec == P && try_pack_sym<T, RPL, NCG, TB, NX>(p, prec, s)) return true;
   HZ_SYM_WIDTHS(HZ_SYM_PACK)
 #undef HZ_SYM_PACK
+  if (prec == HZ_PREC_FP64)
+    return try_pack_sym<double, 1, 2, 8, 200>(p, prec, s) || try_pack_sym<double, 1, 2, 8, 192>(p, prec, s) ||
+           try_pack_sym<double, 1, 2, 8, 160>(p, prec, s) || try_pack_sym<double, 1, 2, 8, 128>(p, prec, s);
   return false;
 }
 
@@ -2100,6 +2374,9 @@ bool solve_sym_any(hz_problem* p, int prec, int nrhs, void* x, bool adjoint, hip
   if (prec == P && !adjoint && try_solve_sym<T, RPL, NCG, TB, NX>(p, prec, nrhs, x, s)) return true;
   HZ_SYM_WIDTHS(HZ_SYM_SOLVE)
 #undef HZ_SYM_SOLVE
+  if (prec == HZ_PREC_FP64 && !adjoint)
+    return try_solve_sym2d<200>(p, prec, nrhs, x, s) || try_solve_sym2d<192>(p, prec, nrhs, x, s) ||
+           try_solve_sym2d<160>(p, prec, nrhs, x, s) || try_solve_sym2d<128>(p, prec, nrhs, x, s);
   return false;
 }
 
@@ -2184,7 +2461,7 @@ bool launch_mr_any(int prec, int n, int ny, int nz, int nrhs, int cap, hipStream
 }  // namespace
 
 bool sym_solve_available(int prec, int nx) {
-  return prec == HZ_PREC_C64 && (nx == 200 || nx == 192 || nx == 160 || nx == 128);
+  return (prec == HZ_PREC_C64 || prec == HZ_PREC_FP64) && (nx == 200 || nx == 192 || nx == 160 || nx == 128);
 }
 
 bool mfma_solve_available(int prec, int nx) {

@@ -272,6 +272,37 @@ def test_plane_solve_symmetric_packed_matches_full(hz, oracle, monkeypatch, nx, 
 
 
 @pytest.mark.parametrize("mode", ["s7", "ref"])
+@pytest.mark.parametrize("nx", [128, 160, 192, 200])
+def test_plane_solve_symmetric_packed_fp64(hz, oracle, monkeypatch, nx, mode):
+    """Complex128 packed symmetric solve: equals the full-block complex128
+    solve to 1e-11 and keeps the FP64 preconditioner's 1e-10 parity with the
+    oracle, in the 7-point and the reference operator modes."""
+    print("case sym fp64 nx=%d %s" % (nx, mode), flush=True)
+    pr = hz.problem
+    if mode == "s7":
+        d = pr.cube_geometry((nx, 12, 6), (8, 3, 2), pr.omega_for_ppw(0.8, 1.0 / nx, 8), beta=0.5, pml_c=30.0)
+        c = 0.8 + 0.4 * np.random.default_rng(nx + 5).random(d.N)
+    else:
+        d = pr.ref_geometry(n_nopml=nx - 16, pml=8, spg=2, nz_nopml=4, length=(240.0,) * 3, homo=False)
+        d = d.replace(n=(nx, 20, d.n[2]), l=(d.l[0], d.l[0] * 20 / nx, d.l[2]), h=(d.h[0], d.h[0], d.h[2]))
+        c = pr.ref_sound_speed(d)
+    S = hz.Solver(d, c)
+    S.factor(hz.abi.PREC_FP64)
+    x = _dev(rand_vec(d.N, 380 + nx, 3))
+    monkeypatch.setenv("HZ_SOLVE_KERNEL", "mr")
+    ref = S.plane_solve(x.clone(), hz.abi.PREC_FP64)
+    monkeypatch.setenv("HZ_SOLVE_KERNEL", "sym")
+    got = S.plane_solve(x.clone(), hz.abi.PREC_FP64)
+    for q in range(3):
+        e = (torch.linalg.vector_norm(got[q] - ref[q]) / torch.linalg.vector_norm(ref[q])).item()
+        assert e < 1e-11, (q, e)
+    xb = rand_vec(d.N, 390 + nx)
+    e = _rel(_host(S.precond_apply(_dev(xb), hz.abi.PREC_FP64)), oracle.Problem(d, c).precond_apply(xb))
+    print("nx=%d %s: FP64 packed preconditioner error %.2e" % (nx, mode, e))
+    assert e < 1e-10
+
+
+@pytest.mark.parametrize("mode", ["s7", "ref"])
 @pytest.mark.parametrize("nx", [64, 96, 128, 160, 192, 200])
 def test_adjoint_plane_solve_mfma_groups_match_full(hz, monkeypatch, nx, mode):
     """Adjoint sweeps of 11 right-hand sides: one group of 8 on the matrix

@@ -260,6 +260,36 @@ int hz_fft_z(hz_problem* p, int prec, int sign, int nrhs, const hz_c128* in, hz_
  * functions.cpp:5941-6028. */
 int hz_plane_solve(hz_problem* p, int prec, int adjoint, int nrhs, void* x);
 
+/* ---- velocity models, solution and convergence files (host memory) ---- */
+
+/* Reads a velocity model of the physical box (the nodes inside the pml + spg
+ * layers of every axis) stored as float32 in [y][x][z] order, z fastest, into
+ * c (host, N doubles, x fastest) and extends it into the layers: x and y layer
+ * nodes copy the nearest physical node of their row / column
+ * (HeteroSoundSpeed3DExtensionToPML, functions.cpp:2675-2716), z layer nodes
+ * the nearest physical z node (the reference leaves them at zero).  Rejects a
+ * short file and non-positive or non-finite speeds.  Replaces
+ * SetSoundSpeed3DFromFile (functions.cpp:1458-1531). */
+int hz_velocity_from_file(const char* path, const hz_desc* desc, double* c);
+/* The reference's model file name for spacing h: "OT_small_at_%dm.bin" with (int) h
+ * (functions.cpp:1471). */
+int hz_velocity_file_name(double h, char* buf, int len);
+/* Writes the physical box of u (host, N complex128) as one "%18.16lf %18.16lf" line per node,
+ * x fastest, then y, then z (reducePML3D order, functions.cpp:238-263): the sol3D_N%d.dat
+ * format of main.cpp:827-834. */
+int hz_write_solution(const char* path, const hz_desc* desc, const hz_c128* u);
+/* Reads that format into the physical box of u (host, N complex128; layer nodes set to zero);
+ * rejects a file with fewer or more values than physical nodes. */
+int hz_read_solution(const char* path, const hz_desc* desc, hz_c128* u);
+/* Writes a convergence log, one "%d %e %e %lf" line per entry: iteration, residual, relative
+ * residual, relative change of the physical-box solution (FGMRES.cpp:646). */
+int hz_write_convergence(const char* path, int count, const int* iter, const double* res, const double* relres,
+                         const double* diff);
+/* The reference's convergence file name
+ * "convergence_N%d_PML%d_Lx%d_FREQ%d_SPG%6.lf_BETA%5.3lf_FGMRES.dat" (FGMRES.cpp:50): physical
+ * x nodes, x PML nodes, (int) x length, (int) frequency nu, z sponge width 2 spg h_z, beta. */
+int hz_convergence_file_name(const hz_desc* desc, double length_x, double nu, double beta, char* buf, int len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -3,6 +3,7 @@ preconditioner with block-tridiagonal plane factors, FGMRES/BiCGStab drivers and
 the receiver-misfit adjoint, behind the C ABI of include/hz.h (libhz.so)."""
 from . import problem  # noqa: F401
 from . import abi  # noqa: F401
+from . import files  # noqa: F401
 
 
 def Solver(*args, **kw):

@@ -112,6 +112,13 @@ SIGNATURES = {
     "hz_misfit_backward": (_i, [_vp, _vp]),
     "hz_fft_z": (_i, [_vp, _i, _i, _i, _vp, _vp]),
     "hz_plane_solve": (_i, [_vp, _i, _i, _i, _vp]),
+    "hz_velocity_from_file": (_i, [C.c_char_p, C.POINTER(HzDesc), C.POINTER(C.c_double)]),
+    "hz_velocity_file_name": (_i, [_d, C.c_char_p, _i]),
+    "hz_write_solution": (_i, [C.c_char_p, C.POINTER(HzDesc), _vp]),
+    "hz_read_solution": (_i, [C.c_char_p, C.POINTER(HzDesc), _vp]),
+    "hz_write_convergence": (_i, [C.c_char_p, _i, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double)]),
+    "hz_convergence_file_name": (_i, [C.POINTER(HzDesc), _d, _d, _d, C.c_char_p, _i]),
 }
 
 _lib = None

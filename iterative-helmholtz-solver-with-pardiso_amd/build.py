@@ -18,6 +18,7 @@ SOURCES = [
     "hz_abi.cpp",
     "hz_ref_shim.cpp",
     "hz_tune.cpp",
+    "hz_io.cpp",
     "k_fft.hip",
     "k_stencil.hip",
     "k_blocktri.hip",

@@ -206,7 +206,7 @@ def plane_solve_kernel(nx, tuning):
     """Name of the complex64 single-RHS plane-solve kernel the tuned choice launches."""
     fam = tuning.get("solve_kernel", 0)
     if fam == 5:
-        return "k_solve_sym2<%d>" % nx
+        return "k_solve_sym2<%d, false>" % nx
     if fam == 3:
         return "k_solve_dma<float,2,5,20,200>"
     if fam == 1:

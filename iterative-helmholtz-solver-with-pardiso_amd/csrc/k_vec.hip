@@ -111,7 +111,7 @@ __global__ __launch_bounds__(RT) void k_axpy_dot_partial(int64_t n, const double
 // ------------------------------------------------ one-pass Gram-Schmidt blocks
 // Multi-dot over a group of MD vectors: r_q = V_q^H w and (vj != nullptr)
 // l_q = V_q^H vj for q in the group; partial[(group * RB + block) * 2 MD + ...].
-constexpr int MD = 8;
+constexpr int MD = 12;
 
 __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);

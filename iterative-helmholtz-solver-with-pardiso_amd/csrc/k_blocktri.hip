@@ -2303,6 +2303,282 @@ bool try_solve_sym2d(hz_problem* p, int prec, int nrhs, void* x, hipStream_t s) 
   return true;
 }
 
+// ---- 8 complex64 right-hand sides on the packed factors, FP32 MFMA
+// Same packed tiles, untracked DMA ring and step-unrolled tile loop as
+// k_solve_sym2.  V' = D_j^-1 V (n x 8 complex) sits in LDS.  Direct product of
+// block column J: 16-row tiles of its stored rows times the 8 columns (two
+// k-slices of 4), C = [Re Y | Im Y] from Re(P) [Re V | Im V] + Im(P) [-Im V | Re V],
+// accumulated in registers over the step by the wave owning the row tile.
+// Transposed product of the pair: the 16 columns of both block columns as the
+// MFMA rows, their off-diagonal rows as k (rows past a column's off-diagonal
+// part read the slot's zero unit), k-steps spread over the 8 waves and summed
+// into yT by LDS float atomics.
+template <int NX>
+__global__ __launch_bounds__(dma::NT) void k_solve_sym_mfma8(const float2* __restrict__ P, float2* __restrict__ X,
+                                                             const double2* __restrict__ cmy,
+                                                             const double2* __restrict__ cpy,
+                                                             const double2* __restrict__ symd, int ny, int64_t n2,
+                                                             int64_t N) {
+  typedef float4 U;
+  constexpr int TB = 8, NR = 8;
+  typedef SymLayout<NX, TB> L;
+  constexpr int n = NX;
+  constexpr int TPS = L::TPS, NPAIR = L::NPAIR, PAIR = L::PAIR, BS = L::BS, T_ = L::T;
+  constexpr int R = (PAIR / 2 + dma::NT - 1) / dma::NT;  // DMA rounds per tile (PAIR / 2 units)
+  constexpr int ZU = R * dma::NT, SU = ZU + 1;
+  constexpr int ZC = 2 * ZU;                             // complex index of the slot's zero unit
+  constexpr int RT = (n + 15) / 16, TPW = (RT + 7) / 8;  // row tiles, per wave
+  constexpr int NS = 6;                                  // ring slots
+  static_assert(TPW <= 2 && (NS - 1) * R <= 63 && TPS >= NS - 1, "shape");
+  __shared__ U ring[NS * SU];
+  __shared__ float2 v[n * NR];            // V' (row-major, 8 right-hand sides)
+  __shared__ __align__(16) float yT[n * NR * 2];  // transposed products of the step
+  // each wave's partial transposed tile (16 columns x [Re | Im] of 8 RHS),
+  // double-buffered by tile parity and folded into yT in the next stage
+  __shared__ float tscr[2][8][256];
+  __shared__ U pbuf[8 * TPW * 64];
+  __shared__ float2 sdxi[n];
+
+  const int k = blockIdx.x;
+  const int64_t rhs0 = (int64_t)blockIdx.y * NR;
+  float2* xk = X + rhs0 * N + (int64_t)k * n2;
+  const float2* Pk = P + (size_t)k * ny * BS;
+  const double2* dxi = symd + n + ny;
+  const double2* dyi = symd + 2 * n + ny;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int li = lane & 15, lg = lane >> 4;
+  const int nsteps = 2 * ny - 1;
+  auto blk_of = [&](int st) { return st <= ny - 1 ? st : 2 * ny - 2 - st; };
+  const unsigned ring_lds = (unsigned)(size_t)(&ring[0]);
+  const unsigned pbuf_lds = (unsigned)(size_t)(&pbuf[0]);
+
+  for (int i = tid; i < n * NR * 2; i += dma::NT) yT[i] = 0.f;
+  if (tid < NS) ring[tid * SU + ZU] = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int i = tid; i < n; i += dma::NT) {
+    const double2 a = dxi[i];
+    sdxi[i] = make_float2((float)a.x, (float)a.y);
+  }
+  __syncthreads();
+  {
+    const double2 b = dyi[blk_of(0)];
+    for (int i = tid; i < n * NR; i += dma::NT) {
+      const int q = i % NR, row = i / NR;
+      const float2 x0 = xk[(int64_t)q * N + row];
+      const float2 a = sdxi[row];
+      const double dx = (double)a.x * b.x - (double)a.y * b.y, dy = (double)a.x * b.y + (double)a.y * b.x;
+      v[i] = make_float2((float)(x0.x * dx - x0.y * dy), (float)(x0.x * dy + x0.y * dx));
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
+
+  auto tile_dma = [&](const char* blk, int tt, int sl) {
+    const int tu = (tt < NPAIR ? PAIR : (NPAIR + 1) * TB * TB) / 2;
+    const char* base = blk + (size_t)tt * PAIR * sizeof(float2);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int u = min(r * dma::NT + tid, tu - 1);
+      dma16_untracked(base + (size_t)u * 16, ring_lds + (unsigned)((sl * SU + r * dma::NT + wv * 64) * 16));
+    }
+  };
+  auto blk_addr = [&](int st) {
+    return reinterpret_cast<const char*>(Pk + (size_t)blk_of(st < nsteps ? st : 0) * BS);
+  };
+#pragma unroll
+  for (int g = 0; g < NS - 1; ++g) tile_dma(blk_addr(0), g, g);
+
+  hz_v4f acc[TPW];
+  int slot = 0;
+  for (int st = 0; st < nsteps; ++st) {
+    const char* b0 = blk_addr(st);
+    const char* b1 = blk_addr(st + 1);
+#pragma unroll
+    for (int t = 0; t < TPS; ++t) {
+      __builtin_amdgcn_s_waitcnt(dma::wait_vm((NS - 2) * R));
+      __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
+      dma::raw_barrier();
+      const float2* SLc = reinterpret_cast<const float2*>(ring + slot * SU);
+      // fold the previous tile's partial transposed products into yT (the last
+      // tile of a step is folded by the step's epilogue)
+      auto fold = [&](int tp) {
+        const int Jp1 = tp, Jp2 = T_ - 1 - tp;
+        if (tid < 256) {
+          const int m = tid >> 4, nn = tid & 15;
+          if (m < 8 || Jp2 != Jp1) {
+            float sum = 0.f;
+#pragma unroll
+            for (int w = 0; w < 8; ++w) sum += tscr[tp & 1][w][tid];
+            const int col = (m < 8 ? Jp1 : Jp2) * TB + (m & 7);
+            yT[(col * NR + (nn & 7)) * 2 + (nn >> 3)] += sum;
+          }
+        }
+      };
+      if (t > 0) fold(t - 1);
+      if (t == 0) {
+#pragma unroll
+        for (int a = 0; a < TPW; ++a) acc[a] = hz_v4f{0.f, 0.f, 0.f, 0.f};
+        int line = 0;
+        if (st < ny - 1) line = st + 1;
+        else if (2 * ny - 2 - st > 0) line = 2 * ny - 3 - st;
+        // the epilogue operand of this wave's row tiles: lane l loads unit
+        // 8 rt + 2 g + (l & 1) of right-hand side q, (g, q) = ((l >> 1) / 8, (l >> 1) % 8)
+#pragma unroll
+        for (int a = 0; a < TPW; ++a) {
+          const int rt = wv + 8 * a;
+          if (rt < RT) {
+            const int c = lane >> 1, g = c >> 3, q = c & 7;
+            int u = 8 * rt + 2 * g + (lane & 1);
+            u = u < n / 2 ? u : n / 2 - 1;
+            const float2* src = xk + (int64_t)q * N + (size_t)n * line + 2 * (size_t)u;
+            dma16_untracked(src, pbuf_lds + (unsigned)((wv * TPW + a) * 64 * 16));
+          }
+        }
+      }
+      {
+        const int J1 = t, J2 = T_ - 1 - t;
+        const bool two = J2 != J1;
+        const int Hc1 = (J1 + 1) * TB, Hc2 = (J2 + 1) * TB;  // stored rows of a column
+        const int ub2 = (J1 + 1) * TB * TB;                   // complex offset of block column J2
+        // direct products of this wave's row tiles
+#pragma unroll
+        for (int a = 0; a < TPW; ++a) {
+          const int rt = wv + 8 * a;
+          if (rt < RT) {
+#pragma unroll
+            for (int part = 0; part < 2; ++part) {
+              const int J = part ? J2 : J1, Hc = part ? Hc2 : Hc1, ub = part ? ub2 : 0;
+              if ((part == 0 || two) && 16 * rt < Hc) {
+#pragma unroll
+                for (int ks = 0; ks < 2; ++ks) {
+                  const int row = 16 * rt + li, cc = 4 * ks + lg;
+                  const float2 ga = SLc[row < Hc ? ub + cc * Hc + row : ZC];
+                  const float2 vv = v[(J * TB + cc) * NR + (li & 7)];
+                  const float b1 = li < 8 ? vv.x : vv.y;
+                  const float b2 = li < 8 ? -vv.y : vv.x;
+                  acc[a] = __builtin_amdgcn_mfma_f32_16x16x4f32(ga.x, b1, acc[a], 0, 0, 0);
+                  acc[a] = __builtin_amdgcn_mfma_f32_16x16x4f32(ga.y, b2, acc[a], 0, 0, 0);
+                }
+              }
+            }
+          }
+        }
+        // transposed products: C[m][n], m = column of the pair (0-7 J1, 8-15 J2)
+        const int nk = 2 * (two ? J2 : J1);  // k-steps of 4 off-diagonal rows
+        {
+          hz_v4f ct = hz_v4f{0.f, 0.f, 0.f, 0.f};
+          for (int kk = wv; kk < nk; kk += 8) {
+            const int r = 4 * kk + lg;
+            int idx = ZC;
+            if (li < 8) {
+              if (r < TB * J1) idx = li * Hc1 + r;
+            } else if (two) {
+              idx = ub2 + (li - 8) * Hc2 + r;  // r < TB J2 here
+            }
+            const float2 ga = SLc[idx];
+            const float2 vv = v[r * NR + (li & 7)];
+            const float b1 = li < 8 ? vv.x : vv.y;
+            const float b2 = li < 8 ? -vv.y : vv.x;
+            ct = __builtin_amdgcn_mfma_f32_16x16x4f32(ga.x, b1, ct, 0, 0, 0);
+            ct = __builtin_amdgcn_mfma_f32_16x16x4f32(ga.y, b2, ct, 0, 0, 0);
+          }
+#pragma unroll
+          for (int i = 0; i < 4; ++i) tscr[t & 1][wv][(4 * lg + i) * 16 + li] = ct[i];
+        }
+      }
+      {
+        const int ta = t + NS - 1;
+        const int tt = ta % TPS;
+        const char* blk = (ta / TPS) == 0 ? b0 : b1;
+        const bool past = (ta / TPS) != 0 && st + 1 >= nsteps;
+        tile_dma(past ? reinterpret_cast<const char*>(Pk) : blk, past ? 0 : tt, slot == 0 ? NS - 1 : slot - 1);
+      }
+      if (t == TPS - 1) {
+        __builtin_amdgcn_s_waitcnt(dma::wait_vm((NS - 1) * R));
+        __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
+        dma::raw_barrier();  // every wave's products of this step are done
+        fold(t);
+        __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
+        dma::raw_barrier();
+        const int jn = blk_of(min(st + 1, nsteps - 1));
+        const double2 dyn = dyi[jn];
+#pragma unroll
+        for (int a = 0; a < TPW; ++a) {
+          const int rt = wv + 8 * a;
+          float im[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) im[i] = __shfl_xor(acc[a][i], 8);
+          if (rt < RT && li < 8) {
+            const int q = li;
+            const U p0 = pbuf[(wv * TPW + a) * 64 + 2 * (8 * lg + q)];
+            const U p1 = pbuf[(wv * TPW + a) * 64 + 2 * (8 * lg + q) + 1];
+            const float2 pre[4] = {make_float2(p0.x, p0.y), make_float2(p0.z, p0.w), make_float2(p1.x, p1.y),
+                                   make_float2(p1.z, p1.w)};
+            float2 out[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const int row = 16 * rt + 4 * lg + i;
+              const int rr = row < n ? row : 0;
+              float* yt = &yT[(rr * NR + q) * 2];
+              float2 y = make_float2(acc[a][i] + yt[0], im[i] + yt[1]);
+              if (row < n) {
+                yt[0] = 0.f;
+                yt[1] = 0.f;
+              }
+              float2 nv;
+              if (st < ny - 1) {
+                const double2 cm = cmy[st + 1];
+                nv.x = pre[i].x - (float)(cm.x * y.x - cm.y * y.y);
+                nv.y = pre[i].y - (float)(cm.x * y.y + cm.y * y.x);
+                out[i] = nv;
+              } else {
+                const int j = 2 * ny - 2 - st;
+                out[i] = y;
+                if (j > 0) {
+                  const double2 cp = cpy[j - 1];
+                  nv.x = pre[i].x - (float)(cp.x * y.x - cp.y * y.y);
+                  nv.y = pre[i].y - (float)(cp.x * y.y + cp.y * y.x);
+                } else {
+                  nv = y;
+                }
+              }
+              if (row < n && st + 1 < nsteps) {
+                const float2 sa = sdxi[row];
+                const double dx = (double)sa.x * dyn.x - (double)sa.y * dyn.y;
+                const double dy = (double)sa.x * dyn.y + (double)sa.y * dyn.x;
+                v[row * NR + q] = make_float2((float)(nv.x * dx - nv.y * dy), (float)(nv.x * dy + nv.y * dx));
+              }
+            }
+            const int line = st < ny - 1 ? st + 1 : 2 * ny - 2 - st;
+            float2* xq = xk + (int64_t)q * N + (size_t)n * line + 16 * rt + 4 * lg;
+            if (16 * rt + 4 * lg + 1 < n)
+              *reinterpret_cast<U*>(xq) = make_float4(out[0].x, out[0].y, out[1].x, out[1].y);
+            if (16 * rt + 4 * lg + 3 < n)
+              *reinterpret_cast<U*>(xq + 2) = make_float4(out[2].x, out[2].y, out[3].x, out[3].y);
+          }
+        }
+        __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
+      }
+      slot = slot + 1 == NS ? 0 : slot + 1;
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(dma::wait_vm(0));
+}
+
+template <int NX>
+bool try_solve_sym_mfma8(hz_problem* p, int groups, void* x, hipStream_t s) {
+  if (p->nx != NX) return false;
+  hipLaunchKernelGGL((k_solve_sym_mfma8<NX>), dim3((unsigned)p->nz, (unsigned)groups), dim3(dma::NT), 0, s,
+                     p->fac[HZ_PREC_C64].Gsym.as<float2>(), (float2*)x, p->tab(1, 0), p->tab(1, 1),
+                     p->d_symd.as<double2>(), (int)p->ny, p->n2, p->N);
+  HZ_HIP(hipGetLastError());
+  return true;
+}
+
+bool solve_sym_mfma8_any(hz_problem* p, int groups, void* x, hipStream_t s) {
+  return try_solve_sym_mfma8<200>(p, groups, x, s) || try_solve_sym_mfma8<192>(p, groups, x, s) ||
+         try_solve_sym_mfma8<160>(p, groups, x, s) || try_solve_sym_mfma8<128>(p, groups, x, s);
+}
+
 template <typename T, int RPL, int NCG, int TB, int NX>
 bool try_pack_sym(hz_problem* p, int prec, hipStream_t s) {
   if (p->nx != NX) return false;
@@ -2420,7 +2696,7 @@ bool try_launch_mr(int n, int ny, int nz, int nrhs, int cap, hipStream_t s, cons
 
 // Plane-solve kernel family: HZ_SOLVE_KERNEL=mr|dma|fast|mfma|sym forces one for
 // timing; otherwise the tuned (or default) choice applies.
-enum SolveKernel { SK_DEFAULT, SK_MR, SK_DMA, SK_FAST, SK_MFMA, SK_SYM };
+enum SolveKernel { SK_DEFAULT, SK_MR, SK_DMA, SK_FAST, SK_MFMA, SK_SYM, SK_MFMA_SYM };
 SolveKernel solve_kernel_choice() {
   const char* v = getenv("HZ_SOLVE_KERNEL");
   if (!v || !v[0]) return SK_DEFAULT;
@@ -2428,6 +2704,7 @@ SolveKernel solve_kernel_choice() {
   if (!strcmp(v, "dma")) return SK_DMA;
   if (!strcmp(v, "fast")) return SK_FAST;
   if (!strcmp(v, "mfma")) return SK_MFMA;
+  if (!strcmp(v, "mfma_sym")) return SK_MFMA_SYM;
   if (!strcmp(v, "sym")) return SK_SYM;
   return SK_DEFAULT;
 }
@@ -2596,11 +2873,15 @@ void launch_plane_solve_with(hz_problem* p, int prec, int nrhs, void* x, bool ad
       mfma_solve_available(prec, n))
     sk = SK_MFMA;
   if (sk == SK_DEFAULT && nrhs == 1 && single_ring_available(prec, n)) sk = SK_DMA;
-  if (!adjoint && ny >= 1 && sk == SK_MFMA && prec == HZ_PREC_C64 && nrhs >= 8 && mfma_solve_available(prec, n) &&
-      reinterpret_cast<uintptr_t>(x) % 16 == 0) {
-    // groups of 8 on the matrix cores, the remainder on the VALU streaming kernel
+  if (!adjoint && ny >= 1 && (sk == SK_MFMA || sk == SK_MFMA_SYM) && prec == HZ_PREC_C64 && nrhs >= 8 &&
+      mfma_solve_available(prec, n) && reinterpret_cast<uintptr_t>(x) % 16 == 0) {
+    // groups of 8 on the matrix cores, the remainder on the VALU streaming
+    // kernel; HZ_SOLVE_KERNEL=mfma_sym takes the packed symmetric factors
+    // instead of the full blocks (half the bytes, the same MFMA work plus the
+    // transposed tiles' idle rows: slower at 8 right-hand sides)
     const int g = nrhs / 8;
-    launch_mfma8_any(n, ny, (int)p->nz, g, s, f.G.ptr, x, cmy, cpy, p->d_symd.as<double2>(), false, p->n2, p->N);
+    if (!(sk == SK_MFMA_SYM && f.sym_ready && solve_sym_mfma8_any(p, g, x, s)))
+      launch_mfma8_any(n, ny, (int)p->nz, g, s, f.G.ptr, x, cmy, cpy, p->d_symd.as<double2>(), false, p->n2, p->N);
     const int rest = nrhs - 8 * g;
     if (rest == 0) return;
     x = reinterpret_cast<float2*>(x) + (size_t)8 * g * p->N;
@@ -2609,7 +2890,7 @@ void launch_plane_solve_with(hz_problem* p, int prec, int nrhs, void* x, bool ad
     sk = SK_MR;
   }
   if (!adjoint && ny >= 1 && (sk == SK_DEFAULT || sk == SK_MR || sk == SK_MFMA) &&
-      launch_mr_any(prec, n, ny, (int)p->nz, nrhs, sk == SK_MFMA ? 4 : t.group_cap, s, f.G.ptr, x, cmy, cpy, p->n2,
+      launch_mr_any(prec, n, ny, (int)p->nz, nrhs, (sk == SK_MFMA || sk == SK_MFMA_SYM) ? 4 : t.group_cap, s, f.G.ptr, x, cmy, cpy, p->n2,
                     p->N))
     return;
   if (!adjoint && ny >= 1 && sk == SK_DMA && reinterpret_cast<uintptr_t>(x) % 16 == 0) {

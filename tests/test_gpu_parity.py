@@ -410,6 +410,55 @@ def test_plane_solve_mfma_matches_vector_kernel(hz, oracle, monkeypatch, nx):
         assert _rel(y[q], P.precond_apply(xb[q])) < 1e-4
 
 
+@pytest.mark.parametrize("mode", ["s7", "ref"])
+@pytest.mark.parametrize("nx", [128, 160, 192, 200])
+def test_plane_solve_packed_mfma_groups(hz, oracle, monkeypatch, nx, mode):
+    """8 right-hand sides on the matrix cores over the packed symmetric factors
+    (direct row tiles + transposed column MFMAs summed by LDS atomics): of 19
+    RHS the two groups of 8 equal the packed single-RHS kernel's solves and the
+    3 left over the full-block vector kernel's, to FP32 rounding (1e-5); the
+    preconditioner of a group
+    of 8 keeps the complex64 accuracy against the oracle (within 2x of the
+    full-block MFMA kernel's error, below 1e-5 in the 7-point mode)."""
+    print("case packed mfma nx=%d %s" % (nx, mode), flush=True)
+    pr = hz.problem
+    if mode == "s7":
+        d = pr.cube_geometry((nx, 10, 6), (8, 2, 2), pr.omega_for_ppw(0.8, 1.0 / nx, 8), beta=0.5, pml_c=30.0)
+        c = 0.8 + 0.4 * np.random.default_rng(nx + 6).random(d.N)
+    else:
+        d = pr.ref_geometry(n_nopml=nx - 16, pml=8, spg=2, nz_nopml=4, length=(240.0,) * 3, homo=False)
+        d = d.replace(n=(nx, 20, d.n[2]), l=(d.l[0], d.l[0] * 20 / nx, d.l[2]), h=(d.h[0], d.h[0], d.h[2]))
+        c = pr.ref_sound_speed(d)
+    S = hz.Solver(d, c)
+    S.factor(hz.abi.PREC_C64)
+    x = torch.from_numpy(rand_vec(d.N, 460 + nx, 19)).to(torch.complex64).cuda()
+    monkeypatch.setenv("HZ_SOLVE_KERNEL", "sym")
+    ref = S.plane_solve(x.clone(), hz.abi.PREC_C64)
+    monkeypatch.setenv("HZ_SOLVE_KERNEL", "mr")
+    ref_full = S.plane_solve(x.clone(), hz.abi.PREC_C64)
+    monkeypatch.setenv("HZ_SOLVE_KERNEL", "mfma_sym")
+    got = S.plane_solve(x.clone(), hz.abi.PREC_C64)
+    worst = 0.0
+    for q in range(19):
+        # groups of 8 on the packed factors; the 3 left over on the full-block vector kernel
+        r = ref[q] if q < 16 else ref_full[q]
+        e = (torch.linalg.vector_norm(got[q] - r) / torch.linalg.vector_norm(r)).item()
+        worst = max(worst, e) if q < 16 else worst
+        assert e < 1e-5, (q, e)
+    xb = rand_vec(d.N, 470 + nx, 8)
+    P = oracle.Problem(d, c)
+    want = [P.precond_apply(xb[q]) for q in (0, 7)]
+    y = _host(S.precond_apply(_dev(xb), hz.abi.PREC_C64))
+    monkeypatch.setenv("HZ_SOLVE_KERNEL", "mfma")
+    yf = _host(S.precond_apply(_dev(xb), hz.abi.PREC_C64))
+    for i, q in enumerate((0, 7)):
+        e_p, e_f = _rel(y[q], want[i]), _rel(yf[q], want[i])
+        print("nx=%d %s q=%d: packed MFMA %.2e, full-block MFMA %.2e (vs packed 1-RHS %.1e)" % (nx, mode, q, e_p, e_f, worst))
+        assert e_p < 2.0 * e_f + 1e-7
+        if mode == "s7":
+            assert e_p < 1e-5
+
+
 def test_autotune_choices_and_cache(hz, tmp_path, monkeypatch):
     """The autotuner picks a valid launch configuration per (precision, RHS
     count), the tuned preconditioner gives the untuned one's result, and the

@@ -650,7 +650,6 @@ __global__ __launch_bounds__(1024) void k_solve_fast(const typename Cx<T>::t* __
 namespace dma {
 #define HZ_LDS __attribute__((address_space(3)))
 constexpr int NT = 512;
-constexpr int NMAX = 256;
 
 __device__ __forceinline__ constexpr int wait_vm(int vm) {
   // s_waitcnt: vmcnt = vm, expcnt and lgkmcnt at their maxima (no wait)
@@ -1357,13 +1356,6 @@ template <int CTRL>
 __device__ __forceinline__ float dpp_f(float x) {
   return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xF, 0xF, false));
 }
-__device__ __forceinline__ float row16_sum(float x) {
-  x += dpp_f<0xB1>(x);   // quad_perm [1,0,3,2]
-  x += dpp_f<0x4E>(x);   // quad_perm [2,3,0,1]
-  x += dpp_f<0x141>(x);  // row_half_mirror
-  x += dpp_f<0x140>(x);  // row_mirror
-  return x;
-}
 // one transpose-reduce step: CTRL pairs each lane with a lane whose bit
 // (hi) differs; the lane with the bit set keeps b and adds the partner's b,
 // the other keeps a and adds the partner's a, so two values per lane become one
@@ -1416,160 +1408,8 @@ struct SymShape {
   static_assert(lds <= 160 * 1024, "LDS budget");
 };
 
-template <typename T, int RPL, int NCG, int TB, int NX>
-__global__ __launch_bounds__(dma::NT) void k_solve_sym(const typename Cx<T>::t* __restrict__ P,
-                                                       typename Cx<T>::t* __restrict__ X,
-                                                       const double2* __restrict__ cmy,
-                                                       const double2* __restrict__ cpy,
-                                                       const double2* __restrict__ symd, int ny, int64_t n2,
-                                                       int64_t N) {
-  typedef SymShape<T, RPL, NCG, TB, NX> S_;
-  typedef SymLayout<NX, TB> L;
-  typedef typename Cx<T>::t C;
-  typedef typename Unit<T>::t U;
-  constexpr int n = S_::n, nru = S_::nru, RUP = S_::RUP, CPB = S_::CPB, R = S_::R, PU = S_::PU;
-  constexpr int TPS = L::TPS, NPAIR = L::NPAIR, PAIR = L::PAIR, BS = L::BS, T_ = L::T;
-  constexpr int NQ = S_::NQ;
-  // each slot ends with a zero unit (index ZU, never a DMA target): the operand
-  // of idle lanes, selected by index so that every read names its slot
-  constexpr int ZU = R * dma::NT;
-  __shared__ U s0[ZU + 1], s1[ZU + 1], s2[ZU + 1], s3[ZU + 1];
-  __shared__ U s4[ZU + 1], s5[ZU + 1], s6[ZU + 1], s7[ZU + 1];
-  __shared__ C part[NCG * n];
-  // transposed products y_c of the current step, NQ partial sums (re, im)
-  __shared__ __align__(16) T tdot[NQ * n * 2];
-  __shared__ C v[n];
-  // next-line operand (waves 0-1) and the step's coefficients (wave 2: cm_y, cp_y,
-  // 1/dy), all fetched by DMA at the start of the step so that the epilogue
-  // issues no vector loads behind the tiles in flight
-  __shared__ U pbuf[3 * 64];
-  __shared__ C sdxi[n];  // 1/dx
-  const C* pline = reinterpret_cast<const C*>(pbuf);
-  const double2* pscal = reinterpret_cast<const double2*>(&pbuf[128]);
-
-  const int k = blockIdx.x;
-  const int64_t rhs = blockIdx.y;
-  C* x = X + rhs * N + (int64_t)k * n2;
-  const C* Pk = P + (size_t)k * ny * BS;
-  const double2* dxi = symd + n + ny;
-  const double2* dyi = symd + 2 * n + ny;
-  const int tid = threadIdx.x;
-  // the wave index is uniform: held in a scalar register, so that the tile
-  // offsets built from it (column groups, transposed columns, DMA targets) are
-  // scalar arithmetic
-  const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-  // wave w: column group w % NCG, row units (w / NCG) * 64 + lane, so that the
-  // waves sharing a SIMD (w, w + 4) hold low and high rows of the same group:
-  // the direct product's work (rows 0..(J+1) TB of block column J) stays even
-  // across SIMDs
-  const int cg = wv % NCG, rb = (wv / NCG) * 64, ru = rb + lane;
-  const bool worker = ru < nru;
-  const int nsteps = 2 * ny - 1;
-  const int TT = nsteps * TPS;
-  auto blk_of = [&](int s) { return s <= ny - 1 ? s : 2 * ny - 2 - s; };
-  // v' = v D_j^{-1}: the scaling of row r for block j (dyi[j] is wave-uniform)
-  auto dinv = [&](int r, int j) {
-    const C a = sdxi[r];
-    const double2 b = dyi[j];
-    return make_double2((double)a.x * b.x - (double)a.y * b.y, (double)a.x * b.y + (double)a.y * b.x);
-  };
-
-#define HZ_SY_TILE(TC_, SLOT)                                                                         \
-  {                                                                                                   \
-    const int tc_ = (TC_) >= TT ? 0 : (TC_); /* past the end: re-read tile 0 */                      \
-    const int s_ = tc_ / TPS, t_ = tc_ - s_ * TPS;                                                    \
-    const int tu_ = (t_ < NPAIR ? PAIR : (NPAIR + 1) * TB * TB) / RPL;                                \
-    const unsigned char* base_ =                                                                      \
-        reinterpret_cast<const unsigned char*>(Pk + (size_t)blk_of(s_) * BS + (size_t)t_ * PAIR);     \
-    _Pragma("unroll") for (int r_ = 0; r_ < R; ++r_) {                                                \
-      const int p_ = r_ * dma::NT + tid;                                                              \
-      const int u_ = p_ < tu_ ? p_ : tu_ - 1;                                                         \
-      __builtin_amdgcn_global_load_lds((const void*)(base_ + (size_t)u_ * 16),                        \
-                                       (HZ_LDS void*)(&SLOT[r_ * dma::NT + wv * 64]), 16, 0, 0);      \
-    }                                                                                                 \
-  }
-#define HZ_SY_LINE(J_, SE_)                                                                             \
-  {                                                                                                     \
-    if (wv < 2) {                                                                                       \
-      const unsigned char* base_ = reinterpret_cast<const unsigned char*>(x + (size_t)n * (J_));        \
-      const int u_ = min(tid, PU - 1);                                                                  \
-      __builtin_amdgcn_global_load_lds((const void*)(base_ + (size_t)u_ * 16),                          \
-                                       (HZ_LDS void*)(&pbuf[wv * 64]), 16, 0, 0);                       \
-    } else if (wv == 2) {                                                                               \
-      const int sn_ = min((SE_) + 1, nsteps - 1);                                                       \
-      const int jb_ = min(max(2 * ny - 3 - (SE_), 0), ny - 1);                                          \
-      const double2* src_ = lane == 1 ? cpy + jb_ : (lane == 2 ? dyi + blk_of(sn_) : cmy + min((SE_) + 1, ny - 1)); \
-      __builtin_amdgcn_global_load_lds((const void*)src_, (HZ_LDS void*)(&pbuf[128]), 16, 0, 0);        \
-    }                                                                                                   \
-  }
-  T acc[2 * RPL];
-  U vr;  // the lane's row unit of v' (rows ru RPL ..), loaded at the start of each step
-#define HZ_SY_EPILOGUE(S_STEP)                                                                     \
-  {                                                                                                \
-    const int se_ = (S_STEP);                                                                      \
-    if (worker) {                                                                                  \
-      _Pragma("unroll") for (int r = 0; r < RPL; ++r) {                                            \
-        C o;                                                                                       \
-        o.x = acc[2 * r];                                                                          \
-        o.y = acc[2 * r + 1];                                                                      \
-        part[cg * n + ru * RPL + r] = o;                                                           \
-      }                                                                                            \
-    }                                                                                              \
-    __builtin_amdgcn_s_waitcnt(dma::wait_vm((S_::S - 1) * R));                                     \
-    __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);                                                   \
-    dma::raw_barrier();                                                                            \
-    if (tid < n) {                                                                                 \
-      C y;                                                                                         \
-      y.x = 0; y.y = 0;                                                                            \
-      _Pragma("unroll") for (int q = 0; q < NCG; ++q) {                                            \
-        const C u = part[q * n + tid];                                                             \
-        y.x += u.x; y.y += u.y;                                                                    \
-      }                                                                                            \
-      _Pragma("unroll") for (int q = 0; q < NQ; ++q) {                                             \
-        const C u = reinterpret_cast<const C*>(tdot)[q * n + tid];                                 \
-        y.x += u.x; y.y += u.y;                                                                    \
-      }                                                                                            \
-      const C pre = pline[tid];                                                                    \
-      C nv;                                                                                        \
-      if (se_ < ny - 1) {                                                                          \
-        const double2 cm = pscal[0];                                                               \
-        nv.x = pre.x - (T)(cm.x * y.x - cm.y * y.y);                                               \
-        nv.y = pre.y - (T)(cm.x * y.y + cm.y * y.x);                                               \
-        x[(size_t)n * (se_ + 1) + tid] = nv;                                                       \
-      } else {                                                                                     \
-        const int j = 2 * ny - 2 - se_;                                                            \
-        x[(size_t)n * j + tid] = y;                                                                \
-        if (j > 0) {                                                                               \
-          const double2 cp = pscal[1];                                                             \
-          nv.x = pre.x - (T)(cp.x * y.x - cp.y * y.y);                                             \
-          nv.y = pre.y - (T)(cp.x * y.y + cp.y * y.x);                                             \
-        } else {                                                                                   \
-          nv = y;                                                                                  \
-        }                                                                                          \
-      }                                                                                            \
-      if (se_ + 1 < nsteps) {                                                                      \
-        const C a_ = sdxi[tid];                                                                    \
-        const double2 b_ = pscal[2];                                                               \
-        const double2 di = make_double2((double)a_.x * b_.x - (double)a_.y * b_.y,                 \
-                                        (double)a_.x * b_.y + (double)a_.y * b_.x);                \
-        C sv;                                                                                      \
-        sv.x = (T)((double)nv.x * di.x - (double)nv.y * di.y);                                     \
-        sv.y = (T)((double)nv.x * di.y + (double)nv.y * di.x);                                     \
-        v[tid] = sv;                                                                               \
-      }                                                                                            \
-    }                                                                                              \
-    __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);                                                   \
-  }
-  // both products of the tile in one pass over it: lane (cg, ru) reads row
-  // unit ru of columns cg CPB .. cg CPB + 1 of the pair's block columns and
-  // uses each unit twice, for the direct product (rows ru, accumulated in
-  // registers over the step) and for the transposed product (columns of the
-  // unit, from v' of the lane's own rows, zero on and below the diagonal
-  // block).  The transposed values are summed over each 16-lane row by
-  // transpose-reduce steps and stored as partial (rb / 64) 4 + row of their
-  // column; a wave whose rows lie below a block column's stored rows skips it
-  // (its partials for that column stay zero).  Lanes below a column's stored
-  // rows read the slot's zero unit.
+// OUT_[0..1] (re, im) += sum over the unit's RPL rows of A_(row) * W_(row):
+// one column's transposed contribution from a 16-byte unit
 #define HZ_SY_UNIT_DOT(A_, W_, OUT_)                                                               \
   {                                                                                                \
     const C* ag_ = reinterpret_cast<const C*>(&(A_));                                              \
@@ -1580,144 +1420,6 @@ __global__ __launch_bounds__(dma::NT) void k_solve_sym(const typename Cx<T>::t* 
       (OUT_)[1] = fma(ag_[i_].y, (W_)[i_].x, (OUT_)[1]);                                           \
     }                                                                                              \
   }
-#define HZ_SY_PRODUCTS(SLOT, J1_, J2_)                                                             \
-  {                                                                                                \
-    const int Hu1_ = ((J1_) + 1) * TB / RPL, Hu2_ = ((J2_) + 1) * TB / RPL;                        \
-    const int nu1_ = (J1_) * TB / RPL, nu2_ = (J2_) * TB / RPL;                                    \
-    const int ub2_ = ((J1_) + 1) * TB * TB / RPL;                                                  \
-    const bool two_ = (J2_) != (J1_);                                                              \
-    const C* vrow_ = reinterpret_cast<const C*>(&vr);                                              \
-    C zc_;                                                                                         \
-    zc_.x = 0; zc_.y = 0;                                                                          \
-    if (rb < Hu1_) {                                                                               \
-      const bool ok1_ = worker && ru < Hu1_;                                                       \
-      const bool ok2_ = worker && two_ && ru < Hu2_;                                               \
-      const int g1_ = ok1_ ? cg * CPB * Hu1_ + ru : ZU;                                            \
-      const int g2_ = ok2_ ? ub2_ + cg * CPB * Hu2_ + ru : ZU;                                     \
-      const int st1_ = ok1_ ? Hu1_ : 0, st2_ = ok2_ ? Hu2_ : 0;                                    \
-      U a1_[CPB], a2_[CPB];                                                                        \
-      _Pragma("unroll") for (int cc = 0; cc < CPB; ++cc) {                                         \
-        a1_[cc] = SLOT[g1_ + cc * st1_];                                                           \
-        a2_[cc] = SLOT[g2_ + cc * st2_];                                                           \
-      }                                                                                            \
-      const C* v1_ = &v[(J1_) * TB + cg * CPB];                                                    \
-      const C* v2_ = &v[(J2_) * TB + cg * CPB];                                                    \
-      C w1_[RPL], w2_[RPL];                                                                        \
-      _Pragma("unroll") for (int i = 0; i < RPL; ++i) {                                            \
-        w1_[i] = ru < nu1_ ? vrow_[i] : zc_;                                                       \
-        w2_[i] = ru < nu2_ ? vrow_[i] : zc_;                                                       \
-      }                                                                                            \
-      T tv_[8];                                                                                    \
-      _Pragma("unroll") for (int i = 0; i < 8; ++i) tv_[i] = 0;                                    \
-      _Pragma("unroll") for (int cc = 0; cc < CPB; ++cc) {                                         \
-        unit_fma_fused<T, RPL>(a1_[cc], v1_[cc], acc);                                             \
-        unit_fma_fused<T, RPL>(a2_[cc], v2_[cc], acc);                                             \
-        HZ_SY_UNIT_DOT(a1_[cc], w1_, &tv_[2 * cc])                                                 \
-        HZ_SY_UNIT_DOT(a2_[cc], w2_, &tv_[4 + 2 * cc])                                             \
-      }                                                                                            \
-      const T s_ = row16_sum8(tv_, lane);                                                          \
-      const int k_ = lane & 7;                                                                     \
-      if ((lane & 8) == 0 && (two_ || k_ < 4)) {                                                   \
-        const int col_ = ((k_ & 4) ? (J2_) : (J1_)) * TB + cg * CPB + ((k_ >> 1) & 1);             \
-        const int q_ = (rb >> 6) * 4 + (lane >> 4);                                                \
-        tdot[(q_ * n + col_) * 2 + (k_ & 1)] = s_;                                                 \
-      }                                                                                            \
-    } else if (rb < Hu2_ && two_) {                                                                \
-      const bool ok2_ = worker && ru < Hu2_;                                                       \
-      const int g2_ = ok2_ ? ub2_ + cg * CPB * Hu2_ + ru : ZU;                                     \
-      const int st2_ = ok2_ ? Hu2_ : 0;                                                            \
-      U a2_[CPB];                                                                                  \
-      _Pragma("unroll") for (int cc = 0; cc < CPB; ++cc) a2_[cc] = SLOT[g2_ + cc * st2_];          \
-      const C* v2_ = &v[(J2_) * TB + cg * CPB];                                                    \
-      C w2_[RPL];                                                                                  \
-      _Pragma("unroll") for (int i = 0; i < RPL; ++i) w2_[i] = ru < nu2_ ? vrow_[i] : zc_;         \
-      T tv_[4];                                                                                    \
-      _Pragma("unroll") for (int i = 0; i < 4; ++i) tv_[i] = 0;                                    \
-      _Pragma("unroll") for (int cc = 0; cc < CPB; ++cc) {                                         \
-        unit_fma_fused<T, RPL>(a2_[cc], v2_[cc], acc);                                             \
-        HZ_SY_UNIT_DOT(a2_[cc], w2_, &tv_[2 * cc])                                                 \
-      }                                                                                            \
-      const T s_ = row16_sum4(tv_, lane);                                                          \
-      const int k_ = lane & 3;                                                                     \
-      if ((lane & 12) == 0) {                                                                      \
-        const int col_ = (J2_) * TB + cg * CPB + (k_ >> 1);                                        \
-        const int q_ = (rb >> 6) * 4 + (lane >> 4);                                                \
-        tdot[(q_ * n + col_) * 2 + (k_ & 1)] = s_;                                                 \
-      }                                                                                            \
-    }                                                                                              \
-  }
-#define HZ_SY_STAGE(TC_, SLOT, NEXT)                                                      \
-  {                                                                                       \
-    const int Tc = (TC_);                                                                 \
-    const int s = Tc / TPS, t = Tc - s * TPS;                                             \
-    __builtin_amdgcn_s_waitcnt(dma::wait_vm((S_::S - 2) * R));                            \
-    __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);                                          \
-    dma::raw_barrier();                                                                   \
-    if (Tc < TT && t == 0) {                                                              \
-      _Pragma("unroll") for (int i = 0; i < 2 * RPL; ++i) acc[i] = 0;                     \
-      vr = reinterpret_cast<const U*>(v)[worker ? ru : 0];                                \
-      if (!worker) vr.x = vr.y = vr.z = vr.w = 0;                                         \
-      int slot_line = 0;                                                                  \
-      if (s < ny - 1) slot_line = s + 1;                                                  \
-      else if (2 * ny - 2 - s > 0) slot_line = 2 * ny - 3 - s;                            \
-      HZ_SY_LINE(slot_line, s)                                                            \
-    }                                                                                     \
-    if (Tc < TT) {                                                                        \
-      const int J1 = t, J2 = T_ - 1 - t;                                                  \
-      HZ_SY_PRODUCTS(SLOT, J1, J2)                                                        \
-    }                                                                                     \
-    HZ_SY_TILE(Tc + S_::S - 1, NEXT)                                                      \
-    if (Tc < TT && t == TPS - 1) HZ_SY_EPILOGUE(s)                                        \
-  }
-
-  if (tid == 0) {
-    U z;
-    z.x = 0; z.y = 0; z.z = 0; z.w = 0;
-    s0[ZU] = z; s1[ZU] = z; s2[ZU] = z; s3[ZU] = z;
-    s4[ZU] = z; s5[ZU] = z; s6[ZU] = z; s7[ZU] = z;
-  }
-  for (int i = tid; i < NQ * n * 2; i += dma::NT) tdot[i] = 0;
-  for (int i = tid; i < n; i += dma::NT) {
-    const double2 a = dxi[i];
-    C c;
-    c.x = (T)a.x;
-    c.y = (T)a.y;
-    sdxi[i] = c;
-  }
-  __syncthreads();
-  for (int i = tid; i < n; i += dma::NT) {
-    const C x0 = x[i];
-    const double2 di = dinv(i, blk_of(0));
-    C sv;
-    sv.x = (T)((double)x0.x * di.x - (double)x0.y * di.y);
-    sv.y = (T)((double)x0.x * di.y + (double)x0.y * di.x);
-    v[i] = sv;
-  }
-  __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
-  HZ_SY_TILE(0, s0)
-  HZ_SY_TILE(1, s1)
-  HZ_SY_TILE(2, s2)
-  HZ_SY_TILE(3, s3)
-  HZ_SY_TILE(4, s4)
-  HZ_SY_TILE(5, s5)
-  HZ_SY_TILE(6, s6)
-  for (int T0 = 0; T0 < TT; T0 += 8) {
-    HZ_SY_STAGE(T0, s0, s7)
-    HZ_SY_STAGE(T0 + 1, s1, s0)
-    HZ_SY_STAGE(T0 + 2, s2, s1)
-    HZ_SY_STAGE(T0 + 3, s3, s2)
-    HZ_SY_STAGE(T0 + 4, s4, s3)
-    HZ_SY_STAGE(T0 + 5, s5, s4)
-    HZ_SY_STAGE(T0 + 6, s6, s5)
-    HZ_SY_STAGE(T0 + 7, s7, s6)
-  }
-#undef HZ_SY_STAGE
-#undef HZ_SY_PRODUCTS
-#undef HZ_SY_EPILOGUE
-#undef HZ_SY_LINE
-#undef HZ_SY_TILE
-  __builtin_amdgcn_s_waitcnt(dma::wait_vm(0));
-}
 
 // LDS DMA of 16 bytes per lane issued by inline assembly (M0 = the wave's
 // LDS destination, lanes at 16-byte steps).  The compiler does not track it:
@@ -2086,7 +1788,7 @@ __global__ __launch_bounds__(dma::NT) void k_solve_sym2d(const double2* __restri
   typedef double T;
   typedef double2 C;
   typedef double2 U;
-  constexpr int RPL = 1, NCG = 2, TB = 8;
+  constexpr int NCG = 2, TB = 8;  // one complex128 row per 16-byte unit
   typedef SymLayout<NX, TB> L;
   constexpr int n = NX, nru = n, CPB = TB / NCG, RUP = dma::NT / NCG, NQ = RUP / 64;
   constexpr int TPS = L::TPS, NPAIR = L::NPAIR, PAIR = L::PAIR, BS = L::BS, T_ = L::T;
@@ -2598,18 +2300,6 @@ bool try_pack_sym(hz_problem* p, int prec, hipStream_t s) {
   return true;
 }
 
-template <typename T, int RPL, int NCG, int TB, int NX>
-bool try_solve_sym(hz_problem* p, int prec, int nrhs, void* x, hipStream_t s) {
-  if (p->nx != NX) return false;
-  typedef typename Cx<T>::t C;
-  const double2* cmy = p->tab(1, 0);
-  const double2* cpy = p->tab(1, 1);
-  hipLaunchKernelGGL((k_solve_sym<T, RPL, NCG, TB, NX>), dim3((unsigned)p->nz, (unsigned)nrhs), dim3(dma::NT), 0, s,
-                     p->fac[prec].Gsym.as<C>(), (C*)x, cmy, cpy, p->d_symd.as<double2>(), (int)p->ny, p->n2, p->N);
-  HZ_HIP(hipGetLastError());
-  return true;
-}
-
 template <int NX>
 bool try_solve_sym2(hz_problem* p, int prec, int nrhs, void* x, bool adjoint, hipStream_t s) {
   if (p->nx != NX || prec != HZ_PREC_C64) return false;
@@ -2643,11 +2333,8 @@ bool pack_sym_any(hz_problem* p, int prec, hipStream_t s) {
 }
 
 bool solve_sym_any(hz_problem* p, int prec, int nrhs, void* x, bool adjoint, hipStream_t s) {
-  const char* v1 = getenv("HZ_SYM_V1");
-  const bool unrolled = adjoint || !(v1 && v1[0] == '1');
-#define HZ_SYM_SOLVE(P, T, RPL, NCG, TB, NX)                                                     \
-  if (prec == P && unrolled && try_solve_sym2<NX>(p, prec, nrhs, x, adjoint, s)) return true; \
-  if (prec == P && !adjoint && try_solve_sym<T, RPL, NCG, TB, NX>(p, prec, nrhs, x, s)) return true;
+#define HZ_SYM_SOLVE(P, T, RPL, NCG, TB, NX) \
+  if (prec == P && try_solve_sym2<NX>(p, prec, nrhs, x, adjoint, s)) return true;
   HZ_SYM_WIDTHS(HZ_SYM_SOLVE)
 #undef HZ_SYM_SOLVE
   if (prec == HZ_PREC_FP64 && !adjoint)

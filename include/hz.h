@@ -290,6 +290,31 @@ int hz_write_convergence(const char* path, int count, const int* iter, const dou
  * x nodes, x PML nodes, (int) x length, (int) frequency nu, z sponge width 2 spg h_z, beta. */
 int hz_convergence_file_name(const hz_desc* desc, double length_x, double nu, double beta, char* buf, int len);
 
+/* ---- accuracy studies over solution files (host memory) ---- */
+
+/* Runge's ratio of three nested solutions (Runge, functions.cpp:932-1134, 3D branch): the files
+ * hold cubic physical boxes of n_coarse, n_mid, n_fine nodes per side in the sol3D_N%d.dat
+ * format, with (n_fine + 1) a multiple of (n_coarse + 1) and of (n_mid + 1) (the reference's
+ * 49, 99, 199).  out[0] = h * sum over the coarse box of |u_fine - u_coarse|, out[1] the same
+ * over the middle box (IntegralNorm3D, functions.cpp:1348-1360), out[2] = out[0] / out[1]; the
+ * reference prints log2(out[2]) as the order (main.cpp:375-376).  Rejects boxes that are not
+ * nested and files with fewer or more values than the box. */
+int hz_runge_ratio(const char* coarse, const char* mid, const char* fine, int n_coarse, int n_mid, int n_fine,
+                   double h, double* out);
+/* Beta3D's shell norms (Beta3D, functions.cpp:1136-1229; compute_and_print_circle_norm_nopml,
+ * :1290-1313; check_norm_circle_3D_nopml, :629-674): coarse holds a box of grid - 1 nodes per
+ * side, fine one of 2 grid - 1; node (i, j, k) of the coarse box sits at ((i+1) h, (j+1) h,
+ * (k+1) h).  For e = 2 .. grid - 2: radius[e-2] = e h and norm[e-2] = ||u_coarse - u_fine|| /
+ * ||u_fine|| (Frobenius) over the coarse nodes with 2h <= |x - source| <= e h; count must be
+ * grid - 3. */
+int hz_beta3d_norms(const char* coarse, const char* fine, int grid, double h, const double* source, int count,
+                    double* radius, double* norm);
+/* Writes those norms as "%lf %lf" lines (functions.cpp:1295-1302). */
+int hz_write_beta_norms(const char* path, int count, const double* radius, const double* norm);
+/* The norms file name "beta_3D_norm2_Nx%dNy%d_SPG%d.dat" for n physical nodes per side and
+ * spg z sponge nodes (functions.cpp:1295). */
+int hz_beta_norm_file_name(int n, int spg, char* buf, int len);
+
 #ifdef __cplusplus
 }
 #endif

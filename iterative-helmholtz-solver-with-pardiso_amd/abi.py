@@ -119,6 +119,11 @@ SIGNATURES = {
     "hz_write_convergence": (_i, [C.c_char_p, _i, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double),
                                   C.POINTER(C.c_double)]),
     "hz_convergence_file_name": (_i, [C.POINTER(HzDesc), _d, _d, _d, C.c_char_p, _i]),
+    "hz_runge_ratio": (_i, [C.c_char_p, C.c_char_p, C.c_char_p, _i, _i, _i, _d, C.POINTER(C.c_double)]),
+    "hz_beta3d_norms": (_i, [C.c_char_p, C.c_char_p, _i, _d, C.POINTER(C.c_double), _i, C.POINTER(C.c_double),
+                             C.POINTER(C.c_double)]),
+    "hz_write_beta_norms": (_i, [C.c_char_p, _i, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "hz_beta_norm_file_name": (_i, [_i, _i, C.c_char_p, _i]),
 }
 
 _lib = None

@@ -7,6 +7,11 @@
 //                           in reducePML3D order (functions.cpp:238-263)
 //   hz_read_solution        the same format read back (Beta3D / Runge inputs, functions.cpp:932-1239)
 //   hz_write_convergence    convergence_*.dat lines (FGMRES.cpp:50, 646)
+//   hz_runge_ratio          Runge's ratio of three nested solution files (functions.cpp:932-1134)
+//   hz_beta3d_norms         Beta3D's shell norms of two nested solution files
+//                           (functions.cpp:1136-1229, 629-674, 1290-1313)
+//   hz_write_beta_norms     beta_3D_norm2_*.dat lines (functions.cpp:1295-1302)
+#include <algorithm>
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
@@ -42,6 +47,35 @@ struct File {
     if (f) std::fclose(f);
   }
 };
+
+// a sol3D_N%d.dat file of an n x n x n physical box (x fastest), exactly n^3 values
+std::vector<hz_c128> read_box(const char* path, int64_t n) {
+  HZ_CHECK(n >= 1, HZ_ERR_ARG, "box of %lld nodes per side", (long long)n);
+  File f(path, "r");
+  std::vector<hz_c128> u((size_t)(n * n * n));
+  for (size_t l = 0; l < u.size(); ++l)
+    HZ_CHECK(std::fscanf(f.f, "%lf %lf", &u[l].re, &u[l].im) == 2, HZ_ERR_ARG,
+             "%s: %zu of %zu values (box of %lld nodes per side)", path, l, u.size(), (long long)n);
+  double extra;
+  HZ_CHECK(std::fscanf(f.f, "%lf", &extra) != 1, HZ_ERR_ARG, "%s: more values than a box of %lld nodes per side",
+           path, (long long)n);
+  return u;
+}
+
+// value at coarse node (i, j, k) of a box of nc nodes per side, taken from the
+// box of nf = s (nc + 1) - 1 nodes per side: fine node s (i + 1) - 1 is at the
+// same position (functions.cpp:1102-1112)
+inline const hz_c128& restrict_at(const std::vector<hz_c128>& uf, int64_t nf, int64_t s, int64_t i, int64_t j,
+                                  int64_t k) {
+  return uf[(size_t)((s * (i + 1) - 1) + nf * (s * (j + 1) - 1) + nf * nf * (s * (k + 1) - 1))];
+}
+
+int64_t step_between(int64_t nc, int64_t nf) {
+  HZ_CHECK(nc >= 1 && nf >= nc && (nf + 1) % (nc + 1) == 0, HZ_ERR_ARG,
+           "a box of %lld nodes per side is not nested in one of %lld ((n + 1) must divide)", (long long)nc,
+           (long long)nf);
+  return (nf + 1) / (nc + 1);
+}
 
 }  // namespace
 
@@ -174,6 +208,92 @@ int hz_convergence_file_name(const hz_desc* d, double length_x, double nu, doubl
   const int64_t nxp = d->n[0] - 2 * layers(d, 0);
   const int w = std::snprintf(buf, (size_t)len, "convergence_N%d_PML%d_Lx%d_FREQ%d_SPG%6.lf_BETA%5.3lf_FGMRES.dat",
                               (int)nxp, d->pml[0], (int)length_x, (int)nu, d->h[2] * 2 * d->spg[2], beta);
+  HZ_CHECK(w > 0 && w < len, HZ_ERR_ARG, "buffer of %d bytes too short", len);
+  HZ_API_END
+}
+
+int hz_runge_ratio(const char* coarse, const char* mid, const char* fine, int n_coarse, int n_mid, int n_fine,
+                   double h, double* out) {
+  HZ_API_BEGIN
+  HZ_CHECK(out, HZ_ERR_ARG, "null output");
+  HZ_CHECK(std::isfinite(h) && h > 0.0, HZ_ERR_ARG, "h = %g", h);
+  const int64_t s1 = step_between(n_coarse, n_fine), s2 = step_between(n_mid, n_fine);
+  const std::vector<hz_c128> u1 = read_box(coarse, n_coarse), u2 = read_box(mid, n_mid), u3 = read_box(fine, n_fine);
+  // h * sum over the coarser box of |u_fine - u| (IntegralNorm3D, functions.cpp:1348-1360)
+  auto dist = [&](const std::vector<hz_c128>& u, int64_t n, int64_t s) {
+    double sum = 0.0;
+    for (int64_t k = 0; k < n; ++k)
+      for (int64_t j = 0; j < n; ++j)
+        for (int64_t i = 0; i < n; ++i) {
+          const hz_c128& f = restrict_at(u3, n_fine, s, i, j, k);
+          const hz_c128& c = u[(size_t)(i + n * j + n * n * k)];
+          sum += std::hypot(f.re - c.re, f.im - c.im);
+        }
+    return sum * h;
+  };
+  out[0] = dist(u1, n_coarse, s1);
+  out[1] = dist(u2, n_mid, s2);
+  out[2] = out[0] / out[1];
+  HZ_API_END
+}
+
+int hz_beta3d_norms(const char* coarse, const char* fine, int grid, double h, const double* source, int count,
+                    double* radius, double* norm) {
+  HZ_API_BEGIN
+  HZ_CHECK(grid >= 4 && count == grid - 3, HZ_ERR_ARG, "grid %d: %d shells requested, the study has grid - 3", grid,
+           count);
+  HZ_CHECK(source && radius && norm, HZ_ERR_ARG, "null argument");
+  HZ_CHECK(std::isfinite(h) && h > 0.0, HZ_ERR_ARG, "h = %g", h);
+  const int64_t nc = grid - 1, nf = 2 * (int64_t)grid - 1;
+  const std::vector<hz_c128> uc = read_box(coarse, nc), uf = read_box(fine, nf);
+  // Shell e (2 <= e <= grid - 2) holds the coarse nodes with 2h <= r <= e h
+  // (check_norm_circle_3D_nopml, functions.cpp:654-666), so each node is binned
+  // at the first shell that holds it and the shells are prefix sums.  The
+  // comparisons are the reference's own double expressions.
+  const double r0 = 2 * h;
+  std::vector<double> dsum((size_t)grid, 0.0), osum((size_t)grid, 0.0);
+  for (int64_t k = 0; k < nc; ++k)
+    for (int64_t j = 0; j < nc; ++j)
+      for (int64_t i = 0; i < nc; ++i) {
+        const double x = (i + 1) * h - source[0];
+        const double y = (j + 1) * h - source[1];
+        const double z = (k + 1) * h - source[2];
+        const double r = std::sqrt(x * x + y * y + z * z);
+        if (!(r >= r0)) continue;
+        int64_t e = std::max<int64_t>(2, (int64_t)std::ceil(r / h));
+        while (e > 2 && r <= (e - 1) * h) --e;
+        while (e <= grid - 2 && !(r <= e * h)) ++e;
+        if (e > grid - 2) continue;
+        const hz_c128& f = restrict_at(uf, nf, 2, i, j, k);
+        const hz_c128& c = uc[(size_t)(i + nc * j + nc * nc * k)];
+        const double dr = c.re - f.re, di = c.im - f.im;
+        dsum[(size_t)e] += dr * dr + di * di;
+        osum[(size_t)e] += f.re * f.re + f.im * f.im;
+      }
+  // ||u_coarse - u_fine|| / ||u_fine|| over each shell (RelError, Frobenius)
+  double dc = 0.0, oc = 0.0;
+  for (int e = 2; e <= grid - 2; ++e) {
+    dc += dsum[(size_t)e];
+    oc += osum[(size_t)e];
+    radius[e - 2] = e * h;
+    norm[e - 2] = std::sqrt(dc) / std::sqrt(oc);
+  }
+  HZ_API_END
+}
+
+int hz_write_beta_norms(const char* path, int count, const double* radius, const double* norm) {
+  HZ_API_BEGIN
+  HZ_CHECK(count >= 0 && (count == 0 || (radius && norm)), HZ_ERR_ARG, "bad norm arrays");
+  File f(path, "w");
+  for (int q = 0; q < count; ++q)
+    HZ_CHECK(std::fprintf(f.f, "%lf %lf\n", radius[q], norm[q]) > 0, HZ_ERR_ARG, "%s: write failed", path);
+  HZ_API_END
+}
+
+int hz_beta_norm_file_name(int n, int spg, char* buf, int len) {
+  HZ_API_BEGIN
+  HZ_CHECK(buf && len > 0, HZ_ERR_ARG, "null buffer");
+  const int w = std::snprintf(buf, (size_t)len, "beta_3D_norm2_Nx%dNy%d_SPG%d.dat", n, n, spg);
   HZ_CHECK(w > 0 && w < len, HZ_ERR_ARG, "buffer of %d bytes too short", len);
   HZ_API_END
 }

@@ -120,3 +120,108 @@ def test_convergence_file_name_matches_reference(hz, ref):
     name = hz.files.convergence_file_name(d, 3200.0, 4.0, 0.5)
     assert name == os.path.basename(GOLDEN)
     assert name == ref.convergence_name(49, 20, 3200.0, 4.0, 64.0 * 2 * 15, 0.5)
+
+
+def _write_box(path, u):
+    path.write_text("".join("%18.16f %18.16f\n" % (v.real, v.imag) for v in np.asarray(u).reshape(-1)))
+
+
+def _box_index(n, s):
+    """Fine-grid index s (i + 1) - 1 of every node of an n-per-side box, [k][j][i] order."""
+    m = s * (np.arange(n) + 1) - 1
+    k, j, i = np.meshgrid(m, m, m, indexing="ij")
+    return i.reshape(-1), j.reshape(-1), k.reshape(-1)
+
+
+def test_runge_ratio_closed_form(hz, ref, tmp_path):
+    """Three nested boxes (4, 9, 19 per side: steps 4 and 2, as the reference's
+    49, 99, 199) that agree up to constants at coinciding nodes: every
+    difference is that constant, so f = h n^3 |c_fine - c| exactly.  A wrong
+    restriction index, axis order or step makes the differences vary."""
+    n1, n2, n3, h = 4, 9, 19, 0.5
+
+    def field(i, j, k):  # integer-valued, distinct along every axis
+        return (3 * i + 5 * j * j - 7 * k) + 1j * (i * k - 2 * j)
+
+    c1, c2, c3 = 0.25 - 0.5j, 0.625 + 0.0j, 1.0 + 0.5j  # |c3 - c1| = 1.25, |c3 - c2| = 0.625
+    u1 = field(*_box_index(n1, 4)) + c1
+    u2 = field(*_box_index(n2, 2)) + c2
+    u3 = field(*_box_index(n3, 1)) + c3
+    paths = [tmp_path / ("sol3D_N%d.dat" % (n + 1)) for n in (n1, n2, n3)]
+    for p, u in zip(paths, (u1, u2, u3)):
+        _write_box(p, u)
+    f1, f2, ratio = hz.files.runge_ratio(*paths, n1, n2, n3, h)
+    assert f1 == h * n1 ** 3 * abs(c3 - c1) == 40.0
+    assert f2 == h * n2 ** 3 * abs(c3 - c2) == 227.8125
+    assert ratio == f1 / f2
+    # random boxes: the restatement on the same file contents
+    rng = np.random.default_rng(5)
+    us = []
+    for p, n in zip(paths, (n1, n2, n3)):
+        _write_box(p, rng.standard_normal(n ** 3) + 1j * rng.standard_normal(n ** 3))
+        us.append(np.loadtxt(p).view(np.complex128).reshape(-1))
+    got = hz.files.runge_ratio(*paths, n1, n2, n3, h)
+    want = ref.runge_ratio(*us, n1, n2, n3, h)
+    assert np.allclose(got, want, rtol=1e-13, atol=0)
+
+
+def test_runge_ratio_errors(hz, tmp_path):
+    paths = [tmp_path / ("b%d.dat" % n) for n in (4, 9, 19)]
+    for p, n in zip(paths, (4, 9, 19)):
+        _write_box(p, np.ones(n ** 3, dtype=complex))
+    with pytest.raises(hz.abi.HzError, match="not nested"):
+        hz.files.runge_ratio(paths[0], paths[1], paths[2], 4, 8, 19, 1.0)
+    with pytest.raises(hz.abi.HzError, match="more values"):
+        hz.files.runge_ratio(paths[0], paths[1], paths[2], 3, 9, 19, 1.0)
+    with pytest.raises(hz.abi.HzError, match="64 of 125 values"):
+        hz.files.runge_ratio(paths[0], paths[1], paths[2], 5, 9, 29, 1.0)
+
+
+def test_beta3d_norms_lattice_counts(hz, ref, tmp_path):
+    """h = 2^-6 and the source on a node make every offset, radius and
+    comparison exact, so shell membership is the integer test 4 <= |v|^2 <= e^2.
+    The coarse solution is 1 + eps on the nodes with |v|^2 <= R2 and 1 elsewhere,
+    the fine one is 1 at the coarse nodes and 1e3 between them: the norm of
+    shell e is eps sqrt(#{4 <= |v|^2 <= min(e^2, R2)} / #{4 <= |v|^2 <= e^2})."""
+    grid, h, eps, R2 = 12, 2.0 ** -6, 0.5, 10
+    src = (grid // 2 * h,) * 3
+    n1, n2 = grid - 1, 2 * grid - 1
+    off = np.arange(1, grid) - grid // 2  # (i + 1) - grid / 2
+    vk, vj, vi = np.meshgrid(off, off, off, indexing="ij")
+    q = (vi * vi + vj * vj + vk * vk).reshape(-1)
+    uc = 1.0 + eps * (q <= R2) + 0j
+    fi, fj, fk = _box_index(n2, 1)
+    on = ((fi + 1) % 2 == 0) & ((fj + 1) % 2 == 0) & ((fk + 1) % 2 == 0)
+    uf = np.where(on, 1.0, 1e3) + 0j
+    pc, pf = tmp_path / "sol3D_N12.dat", tmp_path / "sol3D_N24.dat"
+    _write_box(pc, uc)
+    _write_box(pf, uf)
+    radius, norm = hz.files.beta3d_norms(pc, pf, grid, h, src)
+    assert len(norm) == grid - 3
+    for e in range(2, grid - 1):
+        inside = (q >= 4) & (q <= e * e)
+        want = eps * np.sqrt(np.count_nonzero(inside & (q <= R2)) / np.count_nonzero(inside))
+        assert radius[e - 2] == e * h
+        assert abs(norm[e - 2] - want) <= 1e-14 * want, (e, norm[e - 2], want)
+    # random fields and an off-node source: the shell-by-shell restatement
+    rng = np.random.default_rng(8)
+    grid, h, src = 10, 0.1, (0.47, 0.52, 0.5)
+    n1, n2 = grid - 1, 2 * grid - 1
+    _write_box(pc, rng.standard_normal(n1 ** 3) + 1j * rng.standard_normal(n1 ** 3))
+    _write_box(pf, rng.standard_normal(n2 ** 3) + 1j * rng.standard_normal(n2 ** 3))
+    radius, norm = hz.files.beta3d_norms(pc, pf, grid, h, src)
+    rr, rn = ref.beta3d_norms(np.loadtxt(pc).view(np.complex128).reshape(-1),
+                              np.loadtxt(pf).view(np.complex128).reshape(-1), grid, h, src)
+    assert np.array_equal(radius, rr)
+    assert np.isnan(norm[0]) and np.isnan(rn[0])  # no node in 2h <= r <= 2h: 0 / 0, as the reference
+    assert np.allclose(norm, rn, rtol=1e-12, atol=0, equal_nan=True)
+
+
+def test_beta_norms_file(hz, ref, tmp_path):
+    radius, norm = np.arange(2, 9) * 0.02, np.linspace(0.01, 0.3, 7)
+    name = hz.files.beta_norm_file_name(49, 5)
+    assert name == "beta_3D_norm2_Nx49Ny49_SPG5.dat" == ref.beta_norm_name(49, 5)
+    hz.files.write_beta_norms(tmp_path / name, radius, norm)
+    assert (tmp_path / name).read_text() == ref.beta_norm_text(radius, norm)
+    with pytest.raises(hz.abi.HzError, match="shells"):
+        hz.files.beta3d_norms(tmp_path / name, tmp_path / name, 3, 1.0, (0.0, 0.0, 0.0))

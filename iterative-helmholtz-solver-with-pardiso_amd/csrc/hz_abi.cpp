@@ -177,6 +177,10 @@ static void lsq(int m, int n, std::vector<zc>& H, int ld, std::vector<zc>& b) {
   }
 }
 
+// device scalars of one lockstep member: H (ldh x m), y (m), r = V^H w and
+// l = V^H V_j (m + 1 each), the ICWY factor L (ldh x ldh), ldh = m + 1
+static size_t krylov_scalars(int m) { return 2 * (size_t)(m + 2) * (m + 2) + 4 * (m + 2) + 64; }
+
 static void ensure_krylov(hz_problem* p, int m, int R = 1) {
   const size_t vb = (size_t)p->N * sizeof(double2) * R;
   const bool fresh = p->kV.bytes < vb * (m + 1) || p->kZ.bytes < vb * m;
@@ -186,7 +190,7 @@ static void ensure_krylov(hz_problem* p, int m, int R = 1) {
   grow(p->kw, vb);
   grow(p->kx, vb);
   grow(p->kaux, vb);
-  const size_t sb = (size_t)(m + 2) * (m + 2) + 4 * (m + 2) + 64;
+  const size_t sb = krylov_scalars(m);
   grow(p->scal, std::max<size_t>(4096, 64 + sb * R + 4 * R + 256) * sizeof(double2));
   if (fresh) {
     // basis slots a batch member never reaches still enter the batched
@@ -238,13 +242,30 @@ static void fgmres_lockstep(hz_problem* p, int R, const double2* f, double2* x, 
   double2* W = p->kw.as<double2>();
   double2* Rv = p->kr.as<double2>();
   const int ldh = m + 1;
-  // per-member device scalars: H (ldh x m), y (m), r = V^H w (m+1), l = V^H V_j (m+1)
-  const size_t SB = (size_t)(m + 2) * (m + 2) + 4 * (m + 2) + 64;
+  // per-member device scalars: H (ldh x m), y (m), r = V^H w (m+1), l = V^H V_j (m+1),
+  // the ICWY factor L (ldh x ldh)
+  const size_t SB = krylov_scalars(m);
   double2* dTop = dscal(p, 64);
   auto dH = [&](int q) { return dTop + SB * q; };
   auto dY = [&](int q) { return dH(q) + (size_t)ldh * m; };
   auto dR = [&](int q) { return dY(q) + m; };
   auto dL = [&](int q) { return dR(q) + m + 1; };
+  auto dLg = [&](int q) { return dL(q) + m + 1; };
+  // Each step's Hessenberg column comes back through pinned memory behind an
+  // event, so the host's Givens update overlaps the next step's device work.
+  struct HostCols {
+    double2* ptr = nullptr;
+    hipEvent_t ev = nullptr;
+    ~HostCols() {
+      if (ev) (void)hipEventDestroy(ev);
+      if (ptr) (void)hipHostFree(ptr);
+    }
+  } hcols;
+  HZ_HIP(hipHostMalloc((void**)&hcols.ptr, (size_t)R * (m + 2) * sizeof(double2), hipHostMallocDefault));
+  HZ_HIP(hipEventCreateWithFlags(&hcols.ev, hipEventDisableTiming));
+  auto hcol_host = [&](int q) { return hcols.ptr + (size_t)q * (m + 2); };
+  // the ICWY correction runs on the device for restarts up to ICWY_MAX
+  const bool icwy_dev = m <= ICWY_MAX;
   double2* dNrm = dTop + SB * R;  // 2 R scalars
   auto Vb = [&](int j, int q) { return V + ((int64_t)j * R + q) * N; };
   auto Zb = [&](int j, int q) { return Z + ((int64_t)j * R + q) * N; };
@@ -348,8 +369,10 @@ static void fgmres_lockstep(hz_problem* p, int R, const double2* f, double2* x, 
       a.in_cycle = true;
     }
     if (!any(&Member::in_cycle, true)) break;
+    bool pre = false;  // step j's preconditioner + operator already enqueued
     for (int j = 0; j < m; ++j) {
-      step_apply(j);
+      if (!pre) step_apply(j);
+      pre = false;
       total++;
       for (int q = 0; q < R; ++q)
         if (M[q].in_cycle) st[q].iters++;
@@ -368,10 +391,20 @@ static void fgmres_lockstep(hz_problem* p, int R, const double2* f, double2* x, 
         }
       } else {
         // modified Gram-Schmidt in inverse compact-WY form: r = V^H w and the new
-        // row l = V_j^H V_q of the Gram matrix in one pass, h = (I + L)^{-1} r on
-        // the host (L strict lower part of V^H V), then w -= V h and ||w|| in a
-        // second pass.  Same coefficients as the sequential form in exact
-        // arithmetic and the same O(eps) loss of orthogonality.
+        // row l = V_j^H V_q of the Gram matrix in one pass, h = (I + L)^{-1} r
+        // (L strict lower part of V^H V; on the device, or on the host past
+        // ICWY_MAX), then w -= V h and ||w|| in a second pass.  Same
+        // coefficients as the sequential form in exact arithmetic and the same
+        // O(eps) loss of orthogonality.
+        if (icwy_dev) {
+          for (int q = 0; q < R; ++q) {
+            if (!M[q].in_cycle) continue;
+            double2* hq = dH(q) + (size_t)ldh * j;
+            launch_multidot(p, N, j + 1, Vb(0, q), vstride, W + q * N, Vb(j, q), dR(q), dL(q), p->stream);
+            launch_icwy(j, ldh, dR(q), dL(q), dLg(q), hq, p->stream);
+            launch_multi_axpy_norm(p, N, j + 1, Vb(0, q), vstride, hq, W + q * N, hq + j + 1, p->stream);
+          }
+        } else {
         for (int q = 0; q < R; ++q) {
           if (!M[q].in_cycle) continue;
           launch_multidot(p, N, j + 1, Vb(0, q), vstride, W + q * N, Vb(j, q), dR(q), dL(q), p->stream);
@@ -395,15 +428,29 @@ static void fgmres_lockstep(hz_problem* p, int R, const double2* f, double2* x, 
           launch_multi_axpy_norm(p, N, j + 1, Vb(0, q), vstride, dH(q) + (size_t)ldh * j, W + q * N,
                                  dH(q) + (j + 1) + (size_t)ldh * j, p->stream);
         }
+        }
       }
       for (int q = 0; q < R; ++q)
         if (M[q].in_cycle)
-          HZ_HIP(hipMemcpyAsync(M[q].hcol.data(), dH(q) + (size_t)ldh * j, (j + 2) * sizeof(double2),
+          HZ_HIP(hipMemcpyAsync(hcol_host(q), dH(q) + (size_t)ldh * j, (j + 2) * sizeof(double2),
                                 hipMemcpyDeviceToHost, p->stream));
-      sync(p);
+      HZ_HIP(hipEventRecord(hcols.ev, p->stream));
+      // v_{j+1} = w / h_{j+1,j} and step j + 1 are enqueued before the host
+      // sees this step's column; a member that stops here (converged or broken
+      // down, where the guard writes v_{j+1} = 0) leaves both unused.
+      for (int q = 0; q < R; ++q)
+        if (M[q].in_cycle)
+          launch_scale_inv_guard(N, dH(q) + (j + 1) + (size_t)ldh * j, o.breakdown, W + q * N, Vb(j + 1, q),
+                                 p->stream);
+      if (j + 1 < m) {
+        step_apply(j + 1);
+        pre = true;
+      }
+      HZ_HIP(hipEventSynchronize(hcols.ev));
       for (int q = 0; q < R; ++q) {
         Member& a = M[q];
         if (!a.in_cycle) continue;
+        std::copy(hcol_host(q), hcol_host(q) + j + 2, a.hcol.begin());
         std::vector<zc>& H = a.H;
         for (int i = 0; i <= j + 1; ++i) H[i + (size_t)ldh * j] = zz(a.hcol[i]);
         const double hn = a.hcol[j + 1].x;
@@ -434,7 +481,6 @@ static void fgmres_lockstep(hz_problem* p, int R, const double2* f, double2* x, 
         }
         if (est < o.tol) { a.in_cycle = false; continue; }
         if (hn <= o.breakdown) { a.brk = true; a.in_cycle = false; continue; }
-        launch_scale_inv_dev(N, dH(q) + (j + 1) + (size_t)ldh * j, W + q * N, Vb(j + 1, q), p->stream);
       }
       if (!any(&Member::in_cycle, true)) break;
     }

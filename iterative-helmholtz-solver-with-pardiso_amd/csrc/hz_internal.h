@@ -222,6 +222,12 @@ void launch_multi_axpy_norm(hz_problem* p, int64_t n, int nv, const double2* V, 
                             const double2* coef, double2* w, double2* out_norm, hipStream_t s);
 // y = x / |coef| (coef holds a norm in .x)
 void launch_scale_inv_dev(int64_t n, const double2* coef, const double2* x, double2* y, hipStream_t s);
+// y = x / coef.x if coef.x > thr, else y = 0
+void launch_scale_inv_guard(int64_t n, const double2* coef, double thr, const double2* x, double2* y, hipStream_t s);
+// ICWY correction h = (I + L)^{-1} r of Arnoldi step j < ICWY_MAX on the device
+// (stores row j of L from l = V^H V_j first); one 64-lane workgroup
+constexpr int ICWY_MAX = 64;
+void launch_icwy(int j, int ldl, const double2* r, const double2* l, double2* L, double2* h, hipStream_t s);
 // y = sum_i c[i] * X_i  + (accumulate ? y : 0)
 void launch_lincomb(int64_t n, int nv, const double2* c, const double2* X, int64_t xstride, double2* y,
                     bool accumulate, hipStream_t s);

@@ -231,13 +231,50 @@ __global__ void k_axpy_dev(int64_t n, const double2* __restrict__ coef, double s
   }
 }
 
-__global__ void k_scale_inv_dev(int64_t n, const double2* __restrict__ coef, const double2* __restrict__ x,
+// y = x / coef.x; with GUARD, y = 0 unless coef.x > thr (a breakdown or a
+// non-finite norm leaves a finite vector)
+template <bool GUARD>
+__global__ void k_scale_inv_dev(int64_t n, const double2* __restrict__ coef, double thr, const double2* __restrict__ x,
                                 double2* __restrict__ y) {
-  const double s = 1.0 / coef[0].x;
+  const double c = coef[0].x;
+  const double s = GUARD ? (c > thr ? 1.0 / c : 0.0) : 1.0 / c;
   for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += (int64_t)gridDim.x * blockDim.x) {
     const double2 xv = x[l];
     y[l] = make_double2(xv.x * s, xv.y * s);
   }
+}
+
+// Inverse compact-WY correction of Arnoldi step j: h = (I + L)^{-1} r, where
+// L is the strict lower part of V^H V (row i holds V_i^H V_c, c < i).  Row j,
+// conj(l_c) with l_c = V_c^H V_j from this step's dot pass, is stored into L
+// (leading dimension ldl) for the later steps of the cycle.  One wave: the
+// rows 0..j go to LDS, lane i holds h_i, and the forward substitution runs
+// over the columns c = 0..j-1 (h_i -= L_ic h_c, c increasing), the finished
+// h_c broadcast from lane c.  Needs j < 64 and (j + 1)^2 double2 of LDS.
+__global__ __launch_bounds__(64) void k_icwy(int j, int ldl, const double2* __restrict__ r,
+                                             const double2* __restrict__ l, double2* __restrict__ L,
+                                             double2* __restrict__ h) {
+  extern __shared__ double2 sL[];
+  const int lane = threadIdx.x;
+  const int n = j + 1;
+  if (lane < j) L[(size_t)j * ldl + lane] = make_double2(l[lane].x, -l[lane].y);
+  for (int e = lane; e < n * n; e += 64) {
+    const int i = e / n, c = e - i * n;
+    double2 v = make_double2(0.0, 0.0);
+    if (c < i) v = i == j ? make_double2(l[c].x, -l[c].y) : L[(size_t)i * ldl + c];
+    sL[e] = v;
+  }
+  __syncthreads();
+  double2 hi = lane < n ? r[lane] : make_double2(0.0, 0.0);
+  for (int c = 0; c < j; ++c) {
+    const double hx = __shfl(hi.x, c), hy = __shfl(hi.y, c);
+    if (lane > c && lane < n) {
+      const double2 a = sL[lane * n + c];
+      hi.x -= a.x * hx - a.y * hy;
+      hi.y -= a.x * hy + a.y * hx;
+    }
+  }
+  if (lane < n) h[lane] = hi;
 }
 
 __global__ void k_lincomb(int64_t n, int nv, const double2* __restrict__ c, const double2* __restrict__ X,
@@ -417,7 +454,19 @@ void launch_axpy_dev(int64_t n, const double2* coef, bool neg, bool conj_coef, c
 }
 
 void launch_scale_inv_dev(int64_t n, const double2* coef, const double2* x, double2* y, hipStream_t s) {
-  hipLaunchKernelGGL(k_scale_inv_dev, dim3(grid_for(n)), dim3(256), 0, s, n, coef, x, y);
+  hipLaunchKernelGGL(k_scale_inv_dev<false>, dim3(grid_for(n)), dim3(256), 0, s, n, coef, 0.0, x, y);
+  HZ_HIP(hipGetLastError());
+}
+
+void launch_scale_inv_guard(int64_t n, const double2* coef, double thr, const double2* x, double2* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_scale_inv_dev<true>, dim3(grid_for(n)), dim3(256), 0, s, n, coef, thr, x, y);
+  HZ_HIP(hipGetLastError());
+}
+
+void launch_icwy(int j, int ldl, const double2* r, const double2* l, double2* L, double2* h, hipStream_t s) {
+  HZ_CHECK(j >= 0 && j < ICWY_MAX, HZ_ERR_ARG, "device ICWY step %d (at most %d)", j, ICWY_MAX - 1);
+  const size_t lds = (size_t)(j + 1) * (j + 1) * sizeof(double2);
+  hipLaunchKernelGGL(k_icwy, dim3(1), dim3(64), lds, s, j, ldl, r, l, L, h);
   HZ_HIP(hipGetLastError());
 }
 

@@ -608,6 +608,26 @@ def test_fgmres_fp64_and_mixed_vs_oracle(hz, oracle):
         assert np.linalg.norm(r) / np.linalg.norm(f) < 1.5e-10
 
 
+@pytest.mark.parametrize("m", [64, 70])
+def test_fgmres_icwy_device_and_host_corrections(hz, oracle, m):
+    """Restarts up to 64 correct the one-pass Gram-Schmidt coefficients on the
+    device, longer ones on the host: both give the oracle's FGMRES(m)
+    (iterations +-1, residual history, solution)."""
+    print("case icwy m=%d" % m, flush=True)
+    d, c = s7_case(hz, n=(20, 18, 22), npml=4, smooth=True)
+    S, P = hz.Solver(d, c), oracle.Problem(d, c)
+    S.factor(hz.abi.PREC_FP64)
+    f = hz.problem.point_source(d, (9, 8, 11), amplitude=1.0)
+    u_o, st_o = P.fgmres(f, m=m, rest=6, tol=1e-10)
+    u, st = S.fgmres(_dev(f), m=m, restarts=6, tol=1e-10, prec=hz.abi.PREC_FP64)
+    print("icwy m=%d iters oracle %d gpu %d" % (m, st_o.iters, st.iters))
+    assert st.converged and st_o.converged
+    assert abs(st.iters - st_o.iters) <= 1
+    k = min(len(st.hist), len(st_o.hist), 40)
+    assert np.allclose(st.hist[:k], st_o.hist[:k], rtol=1e-8)
+    assert _rel(_host(u), u_o) < 1e-8
+
+
 @pytest.mark.parametrize("shape", [(20, 18, 22), (64, 14, 12)])
 def test_fgmres_lockstep_batch_matches_single(hz, oracle, shape):
     """Five point sources advanced in lockstep (one batched preconditioner and

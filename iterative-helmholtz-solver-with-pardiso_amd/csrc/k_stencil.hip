@@ -69,6 +69,7 @@ __global__ __launch_bounds__(ST) void k_stencil7(const double2* __restrict__ u, 
   int64_t l = ij + n2 * k0;
   double2 um = k0 > 0 ? u[l - n2] : make_double2(0, 0);
   double2 uc = u[l];
+#pragma unroll 4  // more loads in flight per thread: 87 -> 82 us at 200^3
   for (int k = k0; k < k1; ++k, l += n2) {
     const double2 up = (k + 1 < nz) ? u[l + n2] : make_double2(0, 0);
     const double2 xm = hxm ? u[l - 1] : make_double2(0, 0);

@@ -11,6 +11,7 @@ namespace hz {
 namespace {
 
 constexpr int RB = 1024;  // stage-1 blocks per vector
+constexpr int RBA = 512;  // blocks of the Gram-Schmidt update pass (measured best at 200^3: 1024 -> 512)
 constexpr int RT = 256;   // threads per block
 
 struct Box {
@@ -26,6 +27,15 @@ __device__ __forceinline__ bool in_box(const Box& b, int64_t l) {
   const int64_t r = l - (int64_t)k * n2;
   const int j = (int)(r / b.nx), i = (int)(r - (int64_t)j * b.nx);
   return i >= b.x0 && i < b.x1 && j >= b.y0 && j < b.y1 && k >= b.z0 && k < b.z1;
+}
+
+// Basis vectors are streamed once per pass and never reused from cache: the
+// Gram-Schmidt passes load them nontemporally (the update pass runs ~10-13%
+// faster at 200^3 than with ordinary loads).
+__device__ __forceinline__ double2 ldv(const double2* p) {
+  typedef double dv2 __attribute__((ext_vector_type(2)));
+  const dv2 t = __builtin_nontemporal_load(reinterpret_cast<const dv2*>(p));
+  return make_double2(t.x, t.y);
 }
 
 __device__ __forceinline__ double2 block_sum(double2 v) {
@@ -68,13 +78,14 @@ __global__ __launch_bounds__(RT) void k_dot_partial(int64_t n, const double2* __
 }
 
 // out[v] = sum of RB partials (mode 1: sqrt of the real part, imaginary 0)
+// sums of nb partials per vector
 __global__ __launch_bounds__(RT) void k_dot_final(const double2* __restrict__ partial, int mode,
-                                                  double2* __restrict__ out) {
+                                                  double2* __restrict__ out, int nb) {
   const int v = blockIdx.x;
   double sx = 0.0, sy = 0.0;
-  for (int b = threadIdx.x; b < RB; b += RT) {
-    sx += partial[(int64_t)v * RB + b].x;
-    sy += partial[(int64_t)v * RB + b].y;
+  for (int b = threadIdx.x; b < nb; b += RT) {
+    sx += partial[(int64_t)v * nb + b].x;
+    sy += partial[(int64_t)v * nb + b].y;
   }
   const double2 s = block_sum(make_double2(sx, sy));
   if (threadIdx.x == 0) out[v] = mode == 1 ? make_double2(sqrt(s.x), 0.0) : s;
@@ -135,7 +146,7 @@ __global__ __launch_bounds__(RT) void k_multidot_partial(int64_t n, int nv, cons
 #pragma unroll
     for (int g = 0; g < MD; ++g) {
       if (g < nq) {
-        const double2 a = V[(int64_t)(q0 + g) * vs + l];
+        const double2 a = ldv(V + (int64_t)(q0 + g) * vs + l);
         ar[g][0] += a.x * wv.x + a.y * wv.y;
         ar[g][1] += a.x * wv.y - a.y * wv.x;
         al[g][0] += a.x * jv.x + a.y * jv.y;
@@ -192,13 +203,13 @@ __global__ __launch_bounds__(RT) void k_multi_axpy_norm_partial(int64_t n, int n
   for (int q = threadIdx.x; q < nv; q += RT) cf[q] = coef[q];
   __syncthreads();
   double acc = 0.0;
-  const int64_t stride = (int64_t)RB * RT;
+  const int64_t stride = (int64_t)gridDim.x * RT;
   for (int64_t l = (int64_t)blockIdx.x * RT + threadIdx.x; l < n; l += stride) {
     double2 s = w[l];
     int q = 0;
     for (; q + 3 < nv; q += 4) {
-      const double2 a0 = V[(int64_t)q * vs + l], a1 = V[(int64_t)(q + 1) * vs + l];
-      const double2 a2 = V[(int64_t)(q + 2) * vs + l], a3 = V[(int64_t)(q + 3) * vs + l];
+      const double2 a0 = ldv(V + (int64_t)q * vs + l), a1 = ldv(V + (int64_t)(q + 1) * vs + l);
+      const double2 a2 = ldv(V + (int64_t)(q + 2) * vs + l), a3 = ldv(V + (int64_t)(q + 3) * vs + l);
       const double2 c0 = cf[q], c1 = cf[q + 1], c2 = cf[q + 2], c3 = cf[q + 3];
       s.x -= c0.x * a0.x - c0.y * a0.y; s.y -= c0.x * a0.y + c0.y * a0.x;
       s.x -= c1.x * a1.x - c1.y * a1.y; s.y -= c1.x * a1.y + c1.y * a1.x;
@@ -206,7 +217,7 @@ __global__ __launch_bounds__(RT) void k_multi_axpy_norm_partial(int64_t n, int n
       s.x -= c3.x * a3.x - c3.y * a3.y; s.y -= c3.x * a3.y + c3.y * a3.x;
     }
     for (; q < nv; ++q) {
-      const double2 a = V[(int64_t)q * vs + l], c = cf[q];
+      const double2 a = ldv(V + (int64_t)q * vs + l), c = cf[q];
       s.x -= c.x * a.x - c.y * a.y;
       s.y -= c.x * a.y + c.y * a.x;
     }
@@ -407,7 +418,7 @@ void launch_dot_multi(hz_problem* p, int64_t n, int nv, const double2* x, int64_
   Box box = make_box(p, mask);
   hipLaunchKernelGGL(k_dot_partial, dim3(RB, nv), dim3(RT), 0, s, n, x, xstride, y, ystride, box,
                      p->red.as<double2>());
-  hipLaunchKernelGGL(k_dot_final, dim3(nv), dim3(RT), 0, s, p->red.as<double2>(), y ? 0 : 1, out);
+  hipLaunchKernelGGL(k_dot_final, dim3(nv), dim3(RT), 0, s, p->red.as<double2>(), y ? 0 : 1, out, RB);
   HZ_HIP(hipGetLastError());
 }
 
@@ -420,7 +431,7 @@ void launch_axpy_dot(hz_problem* p, int64_t n, const double2* coef, const double
                      const double2* y, double2* out, hipStream_t s) {
   ensure_red(p, 1);
   hipLaunchKernelGGL(k_axpy_dot_partial, dim3(RB), dim3(RT), 0, s, n, coef, x, w, y, p->red.as<double2>());
-  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RT), 0, s, p->red.as<double2>(), y ? 0 : 1, out);
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RT), 0, s, p->red.as<double2>(), y ? 0 : 1, out, RB);
   HZ_HIP(hipGetLastError());
 }
 
@@ -441,9 +452,9 @@ void launch_multidot(hz_problem* p, int64_t n, int nv, const double2* V, int64_t
 void launch_multi_axpy_norm(hz_problem* p, int64_t n, int nv, const double2* V, int64_t vstride,
                             const double2* coef, double2* w, double2* out_norm, hipStream_t s) {
   ensure_red(p, 1);
-  hipLaunchKernelGGL(k_multi_axpy_norm_partial, dim3(RB), dim3(RT), nv * sizeof(double2), s, n, nv, V, vstride, coef,
-                     w, p->red.as<double2>());
-  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RT), 0, s, p->red.as<double2>(), 1, out_norm);
+  hipLaunchKernelGGL(k_multi_axpy_norm_partial, dim3(RBA), dim3(RT), nv * sizeof(double2), s, n, nv, V, vstride,
+                     coef, w, p->red.as<double2>());
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RT), 0, s, p->red.as<double2>(), 1, out_norm, RBA);
   HZ_HIP(hipGetLastError());
 }
 

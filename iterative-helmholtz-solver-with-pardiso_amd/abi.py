@@ -50,7 +50,7 @@ class KrylovOpts(C.Structure):
         ("batch", C.c_int32),
         ("graph", C.c_int32),
         ("autotune", C.c_int32),
-        ("reserved", C.c_int32),
+        ("chunk", C.c_int32),
     ]
 
 

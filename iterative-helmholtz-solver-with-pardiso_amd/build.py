@@ -16,6 +16,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 SOURCES = [
     "hz_setup.cpp",
     "hz_abi.cpp",
+    "hz_krylov.cpp",
     "hz_ref_shim.cpp",
     "hz_tune.cpp",
     "hz_io.cpp",

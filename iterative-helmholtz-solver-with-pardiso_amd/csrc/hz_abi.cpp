@@ -44,16 +44,34 @@ void DevBuf::release() {
   bytes = 0;
 }
 
+void PinBuf::reserve(size_t b) {
+  if (b <= bytes) return;
+  release();
+  hipError_t e = hipHostMalloc(&ptr, b, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    ptr = nullptr;
+    set_error("hipHostMalloc(%zu bytes): %s", b, hipGetErrorString(e));
+    throw Error{HZ_ERR_NOMEM};
+  }
+  bytes = b;
+}
+
+void PinBuf::release() {
+  if (ptr) (void)hipHostFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+
 typedef std::complex<double> zc;
 
 static inline double2 d2(zc v) { return make_double2(v.real(), v.imag()); }
 static inline zc zz(double2 v) { return zc(v.x, v.y); }
 
-static void grow(DevBuf& b, size_t bytes) {
+void grow(DevBuf& b, size_t bytes) {
   if (b.bytes < bytes) b.alloc(bytes);
 }
 
-static void ensure_work(hz_problem* p, int nrhs) {
+void ensure_work(hz_problem* p, int nrhs) {
   const size_t b = (size_t)p->N * nrhs * sizeof(double2);
   grow(p->w_a, b);
   grow(p->w_b, b);
@@ -64,23 +82,26 @@ static void ensure_work(hz_problem* p, int nrhs) {
 static void sync(hz_problem* p) { HZ_HIP(hipStreamSynchronize(p->stream)); }
 
 // ------------------------------------------------------------ operators
-// out = L0^{-1} in (or its adjoint).  epi: REF right-preconditioned operator
-// out = in - deltaL * L0^{-1} in fused into the backward transform.
-static void precond_apply(hz_problem* p, int prec, int nrhs, const double2* in, double2* out, bool adjoint,
-                          bool epi) {
+// out = L0^{-1} in (or its adjoint), with `work` (nrhs N complex of the
+// factor precision) holding the z-transformed planes.  epi: REF
+// right-preconditioned operator out = in - deltaL * L0^{-1} in fused into the
+// backward transform.
+void precond_apply_ws(hz_problem* p, int prec, int nrhs, const double2* in, double2* out, bool adjoint, bool epi,
+                      void* work) {
+  HZ_CHECK(prec == HZ_PREC_FP64 || prec == HZ_PREC_C64, HZ_ERR_ARG, "bad precision %d", prec);
+  HZ_CHECK(p->fac[prec].ready, HZ_ERR_STATE, "hz_precond_factor(prec=%d) has not run", prec);
+  const double2* dL = p->d_dL.as<double2>();
+  const int wp = prec == HZ_PREC_FP64 ? 0 : 1;  // work precision: double2 or float2
+  launch_fft_z(p, prec, -1, nrhs, in, 0, work, wp, 0, nullptr, nullptr, p->stream);
+  launch_plane_solve(p, prec, nrhs, work, adjoint, p->stream);
+  launch_fft_z(p, prec, +1, nrhs, work, wp, out, 0, epi ? 1 : 0, in, dL, p->stream);
+}
+
+void precond_apply(hz_problem* p, int prec, int nrhs, const double2* in, double2* out, bool adjoint, bool epi) {
   HZ_CHECK(prec == HZ_PREC_FP64 || prec == HZ_PREC_C64, HZ_ERR_ARG, "bad precision %d", prec);
   HZ_CHECK(p->fac[prec].ready, HZ_ERR_STATE, "hz_precond_factor(prec=%d) has not run", prec);
   ensure_work(p, nrhs);
-  const double2* dL = p->d_dL.as<double2>();
-  if (prec == HZ_PREC_FP64) {
-    launch_fft_z(p, prec, -1, nrhs, in, 0, p->w_a.ptr, 0, 0, nullptr, nullptr, p->stream);
-    launch_plane_solve(p, prec, nrhs, p->w_a.ptr, adjoint, p->stream);
-    launch_fft_z(p, prec, +1, nrhs, p->w_a.ptr, 0, out, 0, epi ? 1 : 0, in, dL, p->stream);
-  } else {
-    launch_fft_z(p, prec, -1, nrhs, in, 0, p->w_a.ptr, 1, 0, nullptr, nullptr, p->stream);
-    launch_plane_solve(p, prec, nrhs, p->w_a.ptr, adjoint, p->stream);
-    launch_fft_z(p, prec, +1, nrhs, p->w_a.ptr, 1, out, 0, epi ? 1 : 0, in, dL, p->stream);
-  }
+  precond_apply_ws(p, prec, nrhs, in, out, adjoint, epi, p->w_a.ptr);
 }
 
 static void L0_apply(hz_problem* p, int nrhs, const double2* in, double2* out, bool adjoint) {
@@ -91,7 +112,7 @@ static void L0_apply(hz_problem* p, int nrhs, const double2* in, double2* out, b
 }
 
 // out = A in (true operator)
-static void op_apply(hz_problem* p, int nrhs, const double2* in, double2* out, bool adjoint) {
+void op_apply(hz_problem* p, int nrhs, const double2* in, double2* out, bool adjoint) {
   if (p->d.mode == HZ_MODE_S7) {
     launch_stencil(p, nrhs, in, out, adjoint, nullptr, p->stream);
     return;
@@ -99,12 +120,11 @@ static void op_apply(hz_problem* p, int nrhs, const double2* in, double2* out, b
   L0_apply(p, nrhs, in, out, adjoint);
   // out -= dL .* in  (conj(dL) for the adjoint)
   const double2* dL = adjoint ? p->d_dLc.as<double2>() : p->d_dL.as<double2>();
-  for (int q = 0; q < nrhs; ++q)
-    launch_w_minus_dl(p->N, out + (size_t)q * p->N, dL, in + (size_t)q * p->N, out + (size_t)q * p->N, p->stream);
+  launch_w_minus_dl(p->N, nrhs, out, dL, in, out, p->stream);
 }
 
 // r = f - A u
-static void residual(hz_problem* p, int nrhs, const double2* f, const double2* u, double2* r, bool adjoint) {
+void residual(hz_problem* p, int nrhs, const double2* f, const double2* u, double2* r, bool adjoint) {
   if (p->d.mode == HZ_MODE_S7) {
     launch_stencil(p, nrhs, u, r, adjoint, f, p->stream);
     return;
@@ -125,7 +145,7 @@ static void op_pre_apply(hz_problem* p, int prec, int nrhs, const double2* in, d
 }
 
 // ------------------------------------------------------------ reductions
-static double2* dscal(hz_problem* p, int i) { return p->scal.as<double2>() + i; }
+double2* dscal(hz_problem* p, int i) { return p->scal.as<double2>() + i; }
 
 static zc dot_host(hz_problem* p, const double2* x, const double2* y, int mask = HZ_MASK_FULL) {
   launch_dot(p, p->N, x, y, mask, dscal(p, 0), p->stream);
@@ -177,29 +197,6 @@ static void lsq(int m, int n, std::vector<zc>& H, int ld, std::vector<zc>& b) {
   }
 }
 
-// device scalars of one lockstep member: H (ldh x m), y (m), r = V^H w and
-// l = V^H V_j (m + 1 each), the ICWY factor L (ldh x ldh), ldh = m + 1
-static size_t krylov_scalars(int m) { return 2 * (size_t)(m + 2) * (m + 2) + 4 * (m + 2) + 64; }
-
-static void ensure_krylov(hz_problem* p, int m, int R = 1) {
-  const size_t vb = (size_t)p->N * sizeof(double2) * R;
-  const bool fresh = p->kV.bytes < vb * (m + 1) || p->kZ.bytes < vb * m;
-  grow(p->kV, vb * (m + 1));
-  grow(p->kZ, vb * m);
-  grow(p->kr, vb);
-  grow(p->kw, vb);
-  grow(p->kx, vb);
-  grow(p->kaux, vb);
-  const size_t sb = krylov_scalars(m);
-  grow(p->scal, std::max<size_t>(4096, 64 + sb * R + 4 * R + 256) * sizeof(double2));
-  if (fresh) {
-    // basis slots a batch member never reaches still enter the batched
-    // preconditioner: keep them finite
-    launch_fill((int64_t)(p->kV.bytes / sizeof(double2)), make_double2(0, 0), p->kV.as<double2>(), p->stream);
-    launch_fill((int64_t)(p->kZ.bytes / sizeof(double2)), make_double2(0, 0), p->kZ.as<double2>(), p->stream);
-  }
-}
-
 struct HistW {
   double* buf;
   int cap;
@@ -210,327 +207,13 @@ struct HistW {
   }
 };
 
-// Flexible GMRES (modified Gram-Schmidt, complex Givens) on A u = f with the
-// right preconditioner L0^{-1} in `prec` and FP64 Krylov vectors, for R
-// right-hand sides advanced in lockstep: each Arnoldi step applies the
-// preconditioner and the operator to the R current basis vectors in one
-// batched call (the plane solve streams each factor block once for the batch),
-// then orthogonalises every member against its own basis.  Members stop on
-// their own convergence or breakdown; the others go on.  Basis layout: V_j of
-// member q at V + (j R + q) N (Z likewise), so step j's batch is contiguous.
-// With R = 1 this is the plain FGMRES of FGMRES.cpp:458-661 with a flexible
-// basis Z and the MGS coefficient h_ij = V_i^H w.
-namespace {
-struct Member {
-  std::vector<zc> H, cs, sn, g, y, Lg;
-  std::vector<double2> hcol, yd, rl;
-  double norm_f = 0, rel = 0, res = 0;
-  int jj = 0;
-  bool done = false, in_cycle = false, conv = false, brk = false;
-  HistW hist{nullptr, 0};
-};
-}  // namespace
-
-static void fgmres_lockstep(hz_problem* p, int R, const double2* f, double2* x, const hz_krylov_opts& o,
-                            hz_krylov_stats* st, bool adjoint) {
-  const int m = o.m;
-  const int64_t N = p->N;
-  const int64_t RN = (int64_t)R * N;
-  ensure_krylov(p, m, R);
-  double2* V = p->kV.as<double2>();
-  double2* Z = p->kZ.as<double2>();
-  double2* W = p->kw.as<double2>();
-  double2* Rv = p->kr.as<double2>();
-  const int ldh = m + 1;
-  // per-member device scalars: H (ldh x m), y (m), r = V^H w (m+1), l = V^H V_j (m+1),
-  // the ICWY factor L (ldh x ldh)
-  const size_t SB = krylov_scalars(m);
-  double2* dTop = dscal(p, 64);
-  auto dH = [&](int q) { return dTop + SB * q; };
-  auto dY = [&](int q) { return dH(q) + (size_t)ldh * m; };
-  auto dR = [&](int q) { return dY(q) + m; };
-  auto dL = [&](int q) { return dR(q) + m + 1; };
-  auto dLg = [&](int q) { return dL(q) + m + 1; };
-  // Each step's Hessenberg column comes back through pinned memory behind an
-  // event, so the host's Givens update overlaps the next step's device work.
-  struct HostCols {
-    double2* ptr = nullptr;
-    hipEvent_t ev = nullptr;
-    ~HostCols() {
-      if (ev) (void)hipEventDestroy(ev);
-      if (ptr) (void)hipHostFree(ptr);
-    }
-  } hcols;
-  HZ_HIP(hipHostMalloc((void**)&hcols.ptr, (size_t)R * (m + 2) * sizeof(double2), hipHostMallocDefault));
-  HZ_HIP(hipEventCreateWithFlags(&hcols.ev, hipEventDisableTiming));
-  auto hcol_host = [&](int q) { return hcols.ptr + (size_t)q * (m + 2); };
-  // the ICWY correction runs on the device for restarts up to ICWY_MAX
-  const bool icwy_dev = m <= ICWY_MAX;
-  double2* dNrm = dTop + SB * R;  // 2 R scalars
-  auto Vb = [&](int j, int q) { return V + ((int64_t)j * R + q) * N; };
-  auto Zb = [&](int j, int q) { return Z + ((int64_t)j * R + q) * N; };
-  const int64_t vstride = RN;
-
-  std::vector<Member> M(R);
-  for (int q = 0; q < R; ++q) {
-    Member& a = M[q];
-    a.H.assign((size_t)ldh * m, zc(0, 0));
-    a.cs.resize(m); a.sn.resize(m); a.g.resize(m + 1); a.y.resize(m);
-    a.Lg.assign((size_t)ldh * ldh, zc(0, 0));
-    a.hcol.resize(m + 2); a.yd.resize(m); a.rl.resize(2 * (m + 1));
-    if (q == 0) a.hist = HistW{o.hist, o.hist_cap};
-    memset(&st[q], 0, sizeof(st[q]));
-  }
-  auto t0 = std::chrono::steady_clock::now();
-  std::vector<double2> hn2(2 * R);
-  for (int q = 0; q < R; ++q) launch_dot(p, N, f + q * N, nullptr, HZ_MASK_FULL, dNrm + q, p->stream);
-  HZ_HIP(hipMemcpyAsync(hn2.data(), dNrm, R * sizeof(double2), hipMemcpyDeviceToHost, p->stream));
-  launch_fill(RN, make_double2(0, 0), x, p->stream);
-  sync(p);
-  for (int q = 0; q < R; ++q) {
-    M[q].norm_f = hn2[q].x;
-    if (M[q].norm_f == 0.0) { M[q].done = true; M[q].conv = true; }
-  }
-  auto any = [&](bool Member::*flag, bool want) {
-    for (auto& a : M) if ((a.*flag) == want) return true;
-    return false;
-  };
-  // true residuals of the members still running (all R are computed: one batched call)
-  auto true_residuals = [&](bool only_running) {
-    residual(p, R, f, x, Rv, adjoint);
-    for (int q = 0; q < R; ++q) {
-      if (only_running && M[q].done) continue;
-      launch_dot(p, N, Rv + q * N, nullptr, o.mask, dNrm + q, p->stream);
-      launch_dot(p, N, Rv + q * N, nullptr, HZ_MASK_FULL, dNrm + R + q, p->stream);
-    }
-    HZ_HIP(hipMemcpyAsync(hn2.data(), dNrm, 2 * R * sizeof(double2), hipMemcpyDeviceToHost, p->stream));
-    sync(p);
-  };
-  // Optional HIP graphs: step j's preconditioner + operator launches (fixed
-  // buffers for the whole solve) are captured on their second use -- the first
-  // runs eagerly and sets every kernel attribute -- and replayed from then on.
-  // The chain holds kernel launches only (no memset nodes).
-  struct StepGraphs {
-    std::vector<hipGraphExec_t> exec;
-    std::vector<int> uses;
-    ~StepGraphs() {
-      for (auto e : exec)
-        if (e) (void)hipGraphExecDestroy(e);
-    }
-  } graphs;
-  if (o.autotune && !p->tuned.count({o.prec, R})) autotune(p, o.prec, R, nullptr);
-  if (o.graph) {
-    graphs.exec.assign(m, nullptr);
-    graphs.uses.assign(m, 0);
-    ensure_work(p, R);
-  }
-  auto step_apply = [&](int j) {
-    if (!o.graph || graphs.uses[j]++ == 0) {
-      precond_apply(p, o.prec, R, Vb(j, 0), Zb(j, 0), adjoint, false);
-      op_apply(p, R, Zb(j, 0), W, adjoint);
-      return;
-    }
-    if (!graphs.exec[j]) {
-      hipGraph_t g = nullptr;
-      HZ_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
-      try {
-        precond_apply(p, o.prec, R, Vb(j, 0), Zb(j, 0), adjoint, false);
-        op_apply(p, R, Zb(j, 0), W, adjoint);
-      } catch (...) {
-        (void)hipStreamEndCapture(p->stream, &g);
-        if (g) (void)hipGraphDestroy(g);
-        throw;
-      }
-      HZ_HIP(hipStreamEndCapture(p->stream, &g));
-      const hipError_t e = hipGraphInstantiate(&graphs.exec[j], g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      HZ_HIP(e);
-    }
-    HZ_HIP(hipGraphLaunch(graphs.exec[j], p->stream));
-  };
-  int total = 0;
-  for (int cyc = 0; cyc < o.max_restarts && any(&Member::done, false); ++cyc) {
-    true_residuals(true);
-    for (int q = 0; q < R; ++q) {
-      Member& a = M[q];
-      if (a.done) continue;
-      a.res = hn2[q].x;
-      a.rel = a.res / a.norm_f;
-      if (!std::isfinite(a.rel)) {
-        set_error("non-finite residual in FGMRES (right-hand side %d)", q);
-        throw Error{HZ_ERR_NONFINITE};
-      }
-      if (a.rel < o.tol) { a.conv = true; a.done = true; continue; }
-      st[q].restarts++;
-      launch_scale_inv_dev(N, dNrm + R + q, Rv + q * N, Vb(0, q), p->stream);
-      std::fill(a.g.begin(), a.g.end(), zc(0, 0));
-      a.g[0] = hn2[R + q].x;
-      a.jj = 0;
-      a.in_cycle = true;
-    }
-    if (!any(&Member::in_cycle, true)) break;
-    bool pre = false;  // step j's preconditioner + operator already enqueued
-    for (int j = 0; j < m; ++j) {
-      if (!pre) step_apply(j);
-      pre = false;
-      total++;
-      for (int q = 0; q < R; ++q)
-        if (M[q].in_cycle) st[q].iters++;
-      if (o.ortho == HZ_ORTHO_MGS_SEQ) {
-        // modified Gram-Schmidt: h_ij = V_i^H w, w -= h_ij V_i in sequence; each
-        // update is fused with the next coefficient's dot (the last with ||w||)
-        for (int q = 0; q < R; ++q) {
-          if (!M[q].in_cycle) continue;
-          double2* wq = W + q * N;
-          double2* hc = dH(q) + (size_t)ldh * j;
-          launch_dot(p, N, Vb(0, q), wq, HZ_MASK_FULL, hc, p->stream);
-          for (int i = 0; i <= j; ++i) {
-            const double2* vnext = i < j ? Vb(i + 1, q) : nullptr;
-            launch_axpy_dot(p, N, hc + i, Vb(i, q), wq, vnext, hc + i + 1, p->stream);
-          }
-        }
-      } else {
-        // modified Gram-Schmidt in inverse compact-WY form: r = V^H w and the new
-        // row l = V_j^H V_q of the Gram matrix in one pass, h = (I + L)^{-1} r
-        // (L strict lower part of V^H V; on the device, or on the host past
-        // ICWY_MAX), then w -= V h and ||w|| in a second pass.  Same
-        // coefficients as the sequential form in exact arithmetic and the same
-        // O(eps) loss of orthogonality.
-        if (icwy_dev) {
-          for (int q = 0; q < R; ++q) {
-            if (!M[q].in_cycle) continue;
-            double2* hq = dH(q) + (size_t)ldh * j;
-            launch_multidot(p, N, j + 1, Vb(0, q), vstride, W + q * N, Vb(j, q), dR(q), dL(q), p->stream);
-            launch_icwy(j, ldh, dR(q), dL(q), dLg(q), hq, p->stream);
-            launch_multi_axpy_norm(p, N, j + 1, Vb(0, q), vstride, hq, W + q * N, hq + j + 1, p->stream);
-          }
-        } else {
-        for (int q = 0; q < R; ++q) {
-          if (!M[q].in_cycle) continue;
-          launch_multidot(p, N, j + 1, Vb(0, q), vstride, W + q * N, Vb(j, q), dR(q), dL(q), p->stream);
-          HZ_HIP(hipMemcpyAsync(M[q].rl.data(), dR(q), (size_t)2 * (m + 1) * sizeof(double2), hipMemcpyDeviceToHost,
-                                p->stream));
-        }
-        sync(p);
-        for (int q = 0; q < R; ++q) {
-          Member& a = M[q];
-          if (!a.in_cycle) continue;
-          const double2* rr = a.rl.data();
-          const double2* ll = a.rl.data() + (m + 1);
-          for (int c = 0; c < j; ++c) a.Lg[(size_t)j * ldh + c] = std::conj(zz(ll[c]));  // V_j^H V_c
-          for (int i = 0; i <= j; ++i) {
-            zc acc = zz(rr[i]);
-            for (int c = 0; c < i; ++c) acc -= a.Lg[(size_t)i * ldh + c] * zz(a.hcol[c]);
-            a.hcol[i] = d2(acc);
-          }
-          HZ_HIP(hipMemcpyAsync(dH(q) + (size_t)ldh * j, a.hcol.data(), (j + 1) * sizeof(double2),
-                                hipMemcpyHostToDevice, p->stream));
-          launch_multi_axpy_norm(p, N, j + 1, Vb(0, q), vstride, dH(q) + (size_t)ldh * j, W + q * N,
-                                 dH(q) + (j + 1) + (size_t)ldh * j, p->stream);
-        }
-        }
-      }
-      for (int q = 0; q < R; ++q)
-        if (M[q].in_cycle)
-          HZ_HIP(hipMemcpyAsync(hcol_host(q), dH(q) + (size_t)ldh * j, (j + 2) * sizeof(double2),
-                                hipMemcpyDeviceToHost, p->stream));
-      HZ_HIP(hipEventRecord(hcols.ev, p->stream));
-      // v_{j+1} = w / h_{j+1,j} and step j + 1 are enqueued before the host
-      // sees this step's column; a member that stops here (converged or broken
-      // down, where the guard writes v_{j+1} = 0) leaves both unused.
-      for (int q = 0; q < R; ++q)
-        if (M[q].in_cycle)
-          launch_scale_inv_guard(N, dH(q) + (j + 1) + (size_t)ldh * j, o.breakdown, W + q * N, Vb(j + 1, q),
-                                 p->stream);
-      if (j + 1 < m) {
-        step_apply(j + 1);
-        pre = true;
-      }
-      HZ_HIP(hipEventSynchronize(hcols.ev));
-      for (int q = 0; q < R; ++q) {
-        Member& a = M[q];
-        if (!a.in_cycle) continue;
-        std::copy(hcol_host(q), hcol_host(q) + j + 2, a.hcol.begin());
-        std::vector<zc>& H = a.H;
-        for (int i = 0; i <= j + 1; ++i) H[i + (size_t)ldh * j] = zz(a.hcol[i]);
-        const double hn = a.hcol[j + 1].x;
-        for (int i = 0; i < j; ++i) {
-          zc u = H[i + (size_t)ldh * j], w2 = H[i + 1 + (size_t)ldh * j];
-          H[i + (size_t)ldh * j] = a.cs[i] * u + a.sn[i] * w2;
-          H[i + 1 + (size_t)ldh * j] = -std::conj(a.sn[i]) * u + a.cs[i] * w2;
-        }
-        zc u = H[j + (size_t)ldh * j], w2 = H[j + 1 + (size_t)ldh * j];
-        const double aa = std::abs(u), bb = std::abs(w2);
-        if (bb == 0.0) { a.cs[j] = 1.0; a.sn[j] = 0.0; }
-        else if (aa == 0.0) { a.cs[j] = 0.0; a.sn[j] = std::conj(w2) / bb; }
-        else {
-          const double t = std::sqrt(aa * aa + bb * bb);
-          a.cs[j] = aa / t;
-          a.sn[j] = (u / aa) * std::conj(w2) / t;
-        }
-        H[j + (size_t)ldh * j] = a.cs[j] * u + a.sn[j] * w2;
-        H[j + 1 + (size_t)ldh * j] = 0.0;
-        a.g[j + 1] = -std::conj(a.sn[j]) * a.g[j];
-        a.g[j] = a.cs[j] * a.g[j];
-        a.jj = j + 1;
-        const double est = std::abs(a.g[j + 1]) / a.norm_f;
-        a.hist.push(est);
-        if (!std::isfinite(est)) {
-          set_error("non-finite Arnoldi coefficient in FGMRES (right-hand side %d)", q);
-          throw Error{HZ_ERR_NONFINITE};
-        }
-        if (est < o.tol) { a.in_cycle = false; continue; }
-        if (hn <= o.breakdown) { a.brk = true; a.in_cycle = false; continue; }
-      }
-      if (!any(&Member::in_cycle, true)) break;
-    }
-    // x_q += Z_q y_q for every member that ran this cycle
-    for (int q = 0; q < R; ++q) {
-      Member& a = M[q];
-      if (a.done || a.jj == 0) continue;
-      a.in_cycle = false;
-      for (int i = a.jj - 1; i >= 0; --i) {
-        zc sacc = a.g[i];
-        for (int c = i + 1; c < a.jj; ++c) sacc -= a.H[i + (size_t)ldh * c] * a.y[c];
-        a.y[i] = sacc / a.H[i + (size_t)ldh * i];
-      }
-      for (int i = 0; i < a.jj; ++i) a.yd[i] = d2(a.y[i]);
-      HZ_HIP(hipMemcpyAsync(dY(q), a.yd.data(), a.jj * sizeof(double2), hipMemcpyHostToDevice, p->stream));
-      launch_lincomb(N, a.jj, dY(q), Zb(0, q), vstride, x + q * N, true, p->stream);
-      a.jj = 0;
-      if (a.brk) a.done = true;
-    }
-    sync(p);
-  }
-  // final true residuals (also for members that stopped on breakdown)
-  true_residuals(false);
-  for (int q = 0; q < R; ++q) {
-    Member& a = M[q];
-    if (a.norm_f == 0.0) {
-      st[q].converged = 1;
-      continue;
-    }
-    a.res = hn2[q].x;
-    a.rel = a.res / a.norm_f;
-    a.conv = a.rel < o.tol;
-    st[q].converged = a.conv;
-    st[q].breakdown = a.brk && !a.conv;
-    st[q].rel_res = a.rel;
-    st[q].res = a.res;
-    st[q].n_hist = a.hist.n;
-  }
-  (void)total;
-  const double ts = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  for (int q = 0; q < R; ++q) st[q].t_solve_s = ts;
-}
-
 // The reference's restarted GMRES on (I - deltaL L0^{-1}) w = f
 // (FGMRES.cpp:458-661), u = L0^{-1} w at the end.
 static void gmres_ref_one(hz_problem* p, const double2* f, double2* u, const hz_krylov_opts& o,
                           hz_krylov_stats& st) {
   const int m = o.m;
   const int64_t N = p->N;
-  ensure_krylov(p, m);
+  ensure_krylov(p, m, 1);
   double2* V = p->kV.as<double2>();
   double2* w = p->kw.as<double2>();
   double2* x0 = p->kx.as<double2>();
@@ -636,7 +319,7 @@ static void gmres_ref_one(hz_problem* p, const double2* f, double2* u, const hz_
 static void bicgstab_one(hz_problem* p, int prec, const double2* f, double2* x, int maxit, double tol, int tol_abs,
                          int mask, HistW& hist, hz_krylov_stats& st) {
   const int64_t N = p->N;
-  ensure_krylov(p, 8);
+  ensure_krylov(p, 8, 1);
   double2* V = p->kV.as<double2>();
   double2 *p0 = V, *s0 = V + N, *yn = V + 2 * N, *zn = V + 3 * N, *Ayn = V + 4 * N, *Azn = V + 5 * N,
           *r0 = V + 6 * N, *rst = V + 7 * N, *w = p->kw.as<double2>();
@@ -652,7 +335,7 @@ static void bicgstab_one(hz_problem* p, int prec, const double2* f, double2* x, 
   const bool ref = p->d.mode == HZ_MODE_REF;
   auto apply_pre = [&](const double2* in, double2* pre, double2* out) {
     precond_apply(p, prec, 1, in, pre, false, false);
-    if (ref) launch_w_minus_dl(N, in, p->d_dL.as<double2>(), pre, out, p->stream);
+    if (ref) launch_w_minus_dl(N, 1, in, p->d_dL.as<double2>(), pre, out, p->stream);
     else launch_stencil(p, 1, pre, out, false, nullptr, p->stream);
   };
   double Res = 0, RelRes = 0;
@@ -698,6 +381,25 @@ static void bicgstab_one(hz_problem* p, int prec, const double2* f, double2* x, 
   st.t_solve_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+}  // namespace hz
+
+namespace hz {
+// A lockstep FGMRES job over the ABI's arguments: batches of opts.batch
+// right-hand sides (0: all of them, at most 16).
+static LockJob lock_job(hz_problem* p, int nrhs, const hz_c128* f, hz_c128* u, const hz_krylov_opts& o,
+                        hz_krylov_stats* st, bool adjoint) {
+  LockJob J;
+  J.p = p;
+  J.nrhs = nrhs;
+  J.f = (const double2*)f;
+  J.u = (double2*)u;
+  J.o = o;
+  J.o.driver = HZ_DRIVER_FGMRES;
+  J.st = st;
+  J.adjoint = adjoint;
+  J.batch = std::max(1, std::min(o.batch > 0 ? o.batch : 16, nrhs));
+  return J;
+}
 }  // namespace hz
 
 using namespace hz;
@@ -881,12 +583,12 @@ int hz_fgmres(hz_problem* p, int nrhs, const hz_c128* f, hz_c128* u, const hz_kr
   HZ_CHECK(o->m >= 1 && o->m <= 2000 && o->max_restarts >= 1, HZ_ERR_ARG, "bad restart parameters");
   int rc = HZ_OK;
   if (o->driver != HZ_DRIVER_GMRES_REF) {
-    const int B = std::max(1, std::min(o->batch > 0 ? o->batch : 16, nrhs));
-    for (int q0 = 0; q0 < nrhs; q0 += B) {
-      hz_krylov_opts oq = *o;
-      if (q0 > 0) { oq.hist = nullptr; oq.hist_cap = 0; }
-      fgmres_lockstep(p, std::min(B, nrhs - q0), (const double2*)f + (size_t)q0 * p->N, (double2*)u + (size_t)q0 * p->N,
-                      oq, st + q0, false);
+    std::vector<LockJob> jobs(1);
+    jobs[0] = lock_job(p, nrhs, f, u, *o, st, false);
+    run_lockstep(jobs);
+    if (jobs[0].rc != HZ_OK) {
+      set_error("%s", jobs[0].msg.c_str());
+      return jobs[0].rc;
     }
   }
   for (int q = 0; q < nrhs; ++q) {
@@ -932,28 +634,9 @@ int hz_fft_z(hz_problem* p, int prec, int sign, int nrhs, const hz_c128* in, hz_
 }
 
 // ------------------------------------------------------- several problems
-// Each problem (one frequency on its own grid) is driven by its own host
-// thread on its own stream, so the GPU runs the problems' kernels side by side
-// (their plane counts add up to fill the CUs).  The first failure's code and
-// message are returned on the calling thread.
-static int run_concurrently(int nprob, const std::function<int(int)>& body) {
-  std::vector<int> rc(nprob, HZ_OK);
-  std::vector<std::string> msg(nprob);
-  std::vector<std::thread> th;
-  th.reserve(nprob);
-  for (int i = 0; i < nprob; ++i)
-    th.emplace_back([&, i] {
-      rc[i] = body(i);
-      if (rc[i] != HZ_OK) msg[i] = g_err;
-    });
-  for (auto& t : th) t.join();
-  for (int i = 0; i < nprob; ++i)
-    if (rc[i] != HZ_OK) {
-      set_error("problem %d: %s", i, msg[i].c_str());
-      return rc[i];
-    }
-  return HZ_OK;
-}
+// Every problem (one frequency on its own grid) runs on its own stream; one
+// host thread queues the work of all of them, so the GPU runs the problems'
+// kernels side by side (their plane counts add up to fill the CUs).
 
 // ------------------------------------------------------------- misfit
 // Sources run in lockstep batches (opts.batch, default min(nsrc, 16)); the
@@ -964,9 +647,21 @@ static int misfit_batch(const hz_problem* p) {
 }
 
 // forward fields of sources [s0, s0 + nb) into us (stride N)
+static void solve_lockstep(hz_problem* p, int nb, const double2* f, double2* u, const hz_krylov_opts& o,
+                           hz_krylov_stats* st, bool adjoint) {
+  std::vector<LockJob> jobs(1);
+  jobs[0] = lock_job(p, nb, (const hz_c128*)f, (hz_c128*)u, o, st, adjoint);
+  jobs[0].batch = nb;
+  run_lockstep(jobs);
+  if (jobs[0].rc != HZ_OK) {
+    set_error("%s", jobs[0].msg.c_str());
+    throw Error{jobs[0].rc};
+  }
+}
+
 static void misfit_solve_forward(hz_problem* p, int s0, int nb, double2* us) {
   std::vector<hz_krylov_stats> st(nb);
-  fgmres_lockstep(p, nb, p->src_f.as<double2>() + (size_t)p->N * s0, us, p->mopts, st.data(), false);
+  solve_lockstep(p, nb, p->src_f.as<double2>() + (size_t)p->N * s0, us, p->mopts, st.data(), false);
   for (int q = 0; q < nb; ++q)
     HZ_CHECK(st[q].converged, HZ_ERR_NOT_CONVERGED, "forward solve %d: relative residual %.3e", s0 + q,
              st[q].rel_res);
@@ -982,8 +677,36 @@ int hz_autotune(hz_problem* p, int prec, int nrhs, hz_tune_info* info) {
 int hz_precond_factor_multi(int nprob, hz_problem* const* probs, int prec) {
   HZ_API_BEGIN
   HZ_CHECK(nprob >= 1 && probs, HZ_ERR_ARG, "bad arguments");
+  HZ_CHECK(prec == HZ_PREC_FP64 || prec == HZ_PREC_C64, HZ_ERR_ARG, "bad precision %d", prec);
   for (int i = 0; i < nprob; ++i) HZ_CHECK(probs[i], HZ_ERR_ARG, "null problem %d", i);
-  return run_concurrently(nprob, [&](int i) { return hz_precond_factor(probs[i], prec); });
+  for (int i = 0; i < nprob; ++i)
+    for (int k = 0; k < i; ++k) HZ_CHECK(probs[i] != probs[k], HZ_ERR_ARG, "problem %d listed twice", i);
+  // all factorisations queued first (one stream each), then each finished
+  int rc = HZ_OK;
+  std::string msg;
+  std::vector<bool> begun(nprob, false);
+  for (int i = 0; i < nprob && rc == HZ_OK; ++i) {
+    try {
+      factor_begin(probs[i], prec, probs[i]->stream);
+      begun[i] = true;
+    } catch (const Error& e) {
+      rc = e.code;
+      msg = "problem " + std::to_string(i) + ": " + g_err;
+    }
+  }
+  for (int i = 0; i < nprob; ++i) {
+    if (!begun[i]) continue;
+    try {
+      factor_end(probs[i], prec, probs[i]->stream);
+    } catch (const Error& e) {
+      if (rc == HZ_OK) {
+        rc = e.code;
+        msg = "problem " + std::to_string(i) + ": " + g_err;
+      }
+    }
+  }
+  if (rc != HZ_OK) set_error("%s", msg.c_str());
+  return rc;
   HZ_API_END
 }
 
@@ -995,11 +718,29 @@ int hz_fgmres_multi(int nprob, hz_problem* const* probs, const int* nrhs, const 
                                            "bad arguments for problem %d", i);
   for (int i = 0; i < nprob; ++i)
     for (int k = 0; k < i; ++k) HZ_CHECK(probs[i] != probs[k], HZ_ERR_ARG, "problem %d listed twice", i);
-  return run_concurrently(nprob, [&](int i) {
+  HZ_CHECK(opts->driver == HZ_DRIVER_FGMRES, HZ_ERR_ARG, "hz_fgmres_multi runs the FGMRES driver");
+  HZ_CHECK(opts->m >= 1 && opts->m <= 2000 && opts->max_restarts >= 1, HZ_ERR_ARG, "bad restart parameters");
+  std::vector<LockJob> jobs(nprob);
+  for (int i = 0; i < nprob; ++i) {
     hz_krylov_opts o = *opts;
     if (i > 0) { o.hist = nullptr; o.hist_cap = 0; }
-    return hz_fgmres(probs[i], nrhs[i], f[i], u[i], &o, stats[i]);
-  });
+    jobs[i] = lock_job(probs[i], nrhs[i], f[i], u[i], o, stats[i], false);
+  }
+  run_lockstep(jobs);
+  for (int i = 0; i < nprob; ++i)
+    if (jobs[i].rc != HZ_OK) {
+      set_error("problem %d: %s", i, jobs[i].msg.c_str());
+      return jobs[i].rc;
+    }
+  for (int i = 0; i < nprob; ++i)
+    for (int q = 0; q < nrhs[i]; ++q) {
+      const hz_krylov_stats& s = stats[i][q];
+      if (!s.converged) {
+        set_error("problem %d: Krylov solve %d stopped at relative residual %.3e after %d iterations", i, q,
+                  s.rel_res, s.iters);
+        return s.breakdown ? HZ_ERR_BREAKDOWN : HZ_ERR_NOT_CONVERGED;
+      }
+    }
   HZ_API_END
 }
 
@@ -1073,7 +814,7 @@ int hz_misfit_backward(hz_problem* p, double* grad) {
                         p->d_res.as<double2>(), p->stream);
     launch_fill((int64_t)nb * N, make_double2(0, 0), rhs, p->stream);
     launch_rec_scatter(nrec, nb, p->d_rec.as<int64_t>(), p->d_res.as<double2>(), rhs, N, p->stream);
-    fgmres_lockstep(p, nb, rhs, lam, p->mopts, st.data(), true);
+    solve_lockstep(p, nb, rhs, lam, p->mopts, st.data(), true);
     for (int q = 0; q < nb; ++q)
       HZ_CHECK(st[q].converged, HZ_ERR_NOT_CONVERGED, "adjoint solve %d: relative residual %.3e", s0 + q,
                st[q].rel_res);

@@ -48,6 +48,18 @@ struct DevBuf {
   T* as() const { return reinterpret_cast<T*>(ptr); }
 };
 
+// Pinned host memory (hipHostMalloc), grown outside the solve loops; the owner
+// synchronises its stream before freeing it.
+struct PinBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  void reserve(size_t b);
+  void release();
+  ~PinBuf() { release(); }
+  template <class T>
+  T* as() const { return reinterpret_cast<T*>(ptr); }
+};
+
 // FFT plan for one length: radix sequence and twiddle tables.
 struct FftPlan {
   int n = 0;
@@ -109,6 +121,8 @@ struct hz_problem {
   hz::DevBuf w_a, w_b, w_c;    // c128 N * nrhs each
   hz::DevBuf red;              // reduction partials
   hz::DevBuf scal;             // device scalars (double2)
+  hz::PinBuf pin;              // pinned host copies of Krylov scalars (statuses, norms)
+  hz::PinBuf pin_misc;         // pinned host word of the factorisation status
 
   // autotuned launch choices per (precision, right-hand sides)
   std::map<std::pair<int, int>, hz::Tuning> tuned;
@@ -186,6 +200,12 @@ void launch_plane_matvec(hz_problem* p, int nrhs, const double2* x, double2* y, 
 void set_max_lds(const void* fn, size_t bytes);
 // block-tridiagonal factorisation of every plane, precision prec.
 void launch_factor(hz_problem* p, int prec, hipStream_t s);
+// The same in two halves, so that one host thread can factor several problems
+// at once: factor_begin enqueues the factorisation and the copy of its status
+// word (pinned) on s; factor_end waits for it, checks the status and packs the
+// symmetric factors.
+void factor_begin(hz_problem* p, int prec, hipStream_t s);
+void factor_end(hz_problem* p, int prec, hipStream_t s);
 // in-place plane solves on z-transformed data (prec 0: double2, 1: float2).
 void launch_plane_solve(hz_problem* p, int prec, int nrhs, void* x, bool adjoint, hipStream_t s);
 // the same with explicit launch choices (autotuner)
@@ -228,6 +248,17 @@ void launch_scale_inv_guard(int64_t n, const double2* coef, double thr, const do
 // (stores row j of L from l = V^H V_j first); one 64-lane workgroup
 constexpr int ICWY_MAX = 64;
 void launch_icwy(int j, int ldl, const double2* r, const double2* l, double2* L, double2* h, hipStream_t s);
+// Complex Givens update of Arnoldi step j on the device, one thread per member
+// q of `mask` (q < 64).  Member q's scalars start at base + q * sb: column j of
+// H (leading dimension ldh) at offH + ldh * j, the cosines at offCS, sines at
+// offSN, the least-squares right-hand side g at offG.  At j = 0, g = beta[q].x e_1.
+// Writes est[q * m + j] = (|g_{j+1}| / nf[q].x, status): 0 go on, 1 below tol,
+// 2 breakdown (h_{j+1,j} <= brk), 3 non-finite.
+void launch_givens(int j, int m, int ldh, uint64_t mask, double2* base, int64_t sb, int offH, int offCS, int offSN,
+                   int offG, const double2* beta, const double2* nf, double tol, double brk, double2* est,
+                   hipStream_t s);
+// y = R^{-1} g over the leading jj x jj triangle of the rotated H (one thread)
+void launch_backsub(int jj, int ldh, const double2* H, const double2* g, double2* y, hipStream_t s);
 // y = sum_i c[i] * X_i  + (accumulate ? y : 0)
 void launch_lincomb(int64_t n, int nv, const double2* c, const double2* X, int64_t xstride, double2* y,
                     bool accumulate, hipStream_t s);
@@ -238,8 +269,8 @@ void launch_axpbypcz(int64_t n, const double2* x, double2 a, const double2* y, d
                      double2* z, hipStream_t s);
 // out = a - b (elementwise)
 void launch_sub(int64_t n, const double2* a, const double2* b, double2* out, hipStream_t s);
-// out = w - dL .* x
-void launch_w_minus_dl(int64_t n, const double2* w, const double2* dL, const double2* x, double2* out,
+// out = w - dL .* x over nrhs vectors of N points (dL has N entries)
+void launch_w_minus_dl(int64_t N, int nrhs, const double2* w, const double2* dL, const double2* x, double2* out,
                        hipStream_t s);
 // grad[l] += scale * Re(conj(u[l]) * lam[l])
 void launch_grad_acc_multi(int64_t n, int ns, double scale, const double2* u, int64_t us, const double2* lam,
@@ -251,5 +282,46 @@ void launch_rec_scatter(int64_t nrec, int ns, const int64_t* idx, const double2*
 void launch_grad_acc(int64_t n, double scale, const double2* u, const double2* lam, double* grad, hipStream_t s);
 void launch_fill(int64_t n, double2 v, double2* y, hipStream_t s);
 void launch_fill_real(int64_t n, double v, double* y, hipStream_t s);
+
+// ------------------------------------------------------------ operators (hz_abi.cpp)
+void grow(DevBuf& b, size_t bytes);
+void ensure_work(hz_problem* p, int nrhs);
+double2* dscal(hz_problem* p, int i);
+// out = L0^{-1} in (adjoint: (L0^{-1})^H in); epi: REF out = in - deltaL L0^{-1} in
+void precond_apply(hz_problem* p, int prec, int nrhs, const double2* in, double2* out, bool adjoint, bool epi);
+// the same with the caller's work buffer (nrhs N complex128 for HZ_PREC_FP64,
+// complex64 for HZ_PREC_C64); allocates nothing
+void precond_apply_ws(hz_problem* p, int prec, int nrhs, const double2* in, double2* out, bool adjoint, bool epi,
+                      void* work);
+// out = A in (adjoint: A^H in), the true operator
+void op_apply(hz_problem* p, int nrhs, const double2* in, double2* out, bool adjoint);
+// r = f - A u
+void residual(hz_problem* p, int nrhs, const double2* f, const double2* u, double2* r, bool adjoint);
+
+// ------------------------------------------------------------ lockstep FGMRES (hz_krylov.cpp)
+// Largest lockstep batch (right-hand sides sharing one batched apply).
+constexpr int LOCKSTEP_MAX = 64;
+// One problem's share of a lockstep solve: nrhs right-hand sides f -> u
+// (device, stride N) in batches of `batch`, stats (host, nrhs).
+struct LockJob {
+  hz_problem* p = nullptr;
+  int nrhs = 0;
+  const double2* f = nullptr;
+  double2* u = nullptr;
+  hz_krylov_opts o{};
+  hz_krylov_stats* st = nullptr;
+  bool adjoint = false;
+  int batch = 1;
+  int rc = HZ_OK;   // outcome: HZ_OK or the code of the error that stopped it
+  std::string msg;  // message of that error
+};
+// Runs every job from the calling thread: each problem's batches go one after
+// the other on its own stream, the problems side by side.  Errors stop only
+// their own job (rc, msg); the stream of a failed job is synchronised before
+// its state is released.
+void run_lockstep(std::vector<LockJob>& jobs);
+// Krylov device storage of a lockstep batch of R members with restart m
+// (basis, scalars, reduction partials), sized before the solve loop.
+void ensure_krylov(hz_problem* p, int m, int R);
 
 }  // namespace hz

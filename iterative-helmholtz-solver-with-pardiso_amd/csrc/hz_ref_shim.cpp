@@ -184,7 +184,7 @@ void ApplyCoeffMatrixA_CSR(size_m x, size_m y, size_m z, int* iparm, int* perm, 
   up(deltaL, dl, N);
   int rc = hz_precond_apply(p, g_cfg.prec, 1, (const hz_c128*)dw, (hz_c128*)dt);
   if (rc) { fail("ApplyCoeffMatrixA_CSR", rc); return; }
-  hz::launch_w_minus_dl((int64_t)N, dw, dl, dt, dt, p->stream);
+  hz::launch_w_minus_dl((int64_t)N, 1, dw, dl, dt, dt, p->stream);
   hz_problem_sync(p);
   down(dt, g, N);
 }

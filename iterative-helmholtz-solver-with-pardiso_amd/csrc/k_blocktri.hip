@@ -2468,18 +2468,24 @@ size_t factor_lds(int n, int nb) {
 
 }  // namespace
 
-void launch_factor(hz_problem* p, int prec, hipStream_t s) {
+void factor_begin(hz_problem* p, int prec, hipStream_t s) {
   const int n = (int)p->nx, ny = (int)p->ny;
   Factors& f = p->fac[prec];
+  f.ready = false;
+  f.sym_ready = false;
   f.elem = prec == HZ_PREC_FP64 ? sizeof(double2) : sizeof(float2);
   const size_t bytes = (size_t)p->nz * ny * (size_t)n * n * f.elem;
   if (f.G.bytes != bytes) {
     f.G.release();
     f.G.alloc(bytes);
   }
-  DevBuf st;
-  st.alloc(sizeof(int));
-  HZ_HIP(hipMemsetAsync(st.ptr, 0, sizeof(int), s));
+  // status word: device int at the end of the scalar block's misc area, copied
+  // to pinned host memory
+  grow(p->scal, 4096 * sizeof(double2));
+  int* st = reinterpret_cast<int*>(dscal(p, 63));
+  p->pin_misc.reserve(64);
+  *p->pin_misc.as<int>() = -1;
+  launch_fill(1, make_double2(0, 0), dscal(p, 63), s);
   PlaneTabs tb = plane_tabs(p);
   int nb = 32;
   if (prec == HZ_PREC_FP64) {
@@ -2488,27 +2494,35 @@ void launch_factor(hz_problem* p, int prec, hipStream_t s) {
     HZ_CHECK(lds <= 160 * 1024, HZ_ERR_ARG, "nx=%d too large for the LDS panel", n);
     set_max_lds((const void*)&k_factor<double>, lds);
     hipLaunchKernelGGL(k_factor<double>, dim3((unsigned)p->nz), dim3(FACT_THREADS), lds, s, f.G.as<double2>(), tb,
-                       n, ny, nb, st.as<int>());
+                       n, ny, nb, st);
   } else {
     while (nb > 4 && factor_lds<float>(n, nb) > 160 * 1024) nb /= 2;
     const size_t lds = factor_lds<float>(n, nb);
     HZ_CHECK(lds <= 160 * 1024, HZ_ERR_ARG, "nx=%d too large for the LDS panel", n);
     set_max_lds((const void*)&k_factor<float>, lds);
     hipLaunchKernelGGL(k_factor<float>, dim3((unsigned)p->nz), dim3(FACT_THREADS), lds, s, f.G.as<float2>(), tb, n,
-                       ny, nb, st.as<int>());
+                       ny, nb, st);
   }
   HZ_HIP(hipGetLastError());
-  int hs = 0;
-  HZ_HIP(hipMemcpyAsync(&hs, st.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+  HZ_HIP(hipMemcpyAsync(p->pin_misc.ptr, st, sizeof(int), hipMemcpyDeviceToHost, s));
+}
+
+void factor_end(hz_problem* p, int prec, hipStream_t s) {
+  Factors& f = p->fac[prec];
   HZ_HIP(hipStreamSynchronize(s));
-  HZ_CHECK(hs == 0, HZ_ERR_SINGULAR, "zero pivot while factoring a plane");
+  HZ_CHECK(*p->pin_misc.as<int>() == 0, HZ_ERR_SINGULAR, "zero pivot while factoring a plane");
   // symmetric packed copy for the widths the packed solve is instantiated for
   f.sym_ready = false;
-  if (sym_solve_available(prec, n) && !getenv("HZ_NO_SYM_PACK")) {
+  if (sym_solve_available(prec, (int)p->nx) && !getenv("HZ_NO_SYM_PACK")) {
     f.sym_ready = pack_sym_any(p, prec, s);
     HZ_HIP(hipStreamSynchronize(s));
   }
   f.ready = true;
+}
+
+void launch_factor(hz_problem* p, int prec, hipStream_t s) {
+  factor_begin(p, prec, s);
+  factor_end(p, prec, s);
 }
 
 void launch_plane_solve(hz_problem* p, int prec, int nrhs, void* x, bool adjoint, hipStream_t s) {

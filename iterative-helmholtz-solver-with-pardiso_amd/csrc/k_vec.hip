@@ -301,6 +301,94 @@ __global__ void k_lincomb(int64_t n, int nv, const double2* __restrict__ c, cons
   }
 }
 
+__device__ __forceinline__ double2 zmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 zconj(double2 a) { return make_double2(a.x, -a.y); }
+// a / b by Smith's algorithm (scaled by the larger component of b)
+__device__ __forceinline__ double2 zdiv(double2 a, double2 b) {
+  if (fabs(b.x) >= fabs(b.y)) {
+    const double r = b.y / b.x, d = b.x + b.y * r;
+    return make_double2((a.x + a.y * r) / d, (a.y - a.x * r) / d);
+  }
+  const double r = b.x / b.y, d = b.x * r + b.y;
+  return make_double2((a.x * r + a.y) / d, (a.y * r - a.x) / d);
+}
+
+// Complex Givens least-squares update of Arnoldi step j (the per-iteration
+// rotation of FGMRES's Hessenberg column and of g = ||r|| e_1), one thread per
+// member of `mask`.  The earlier rotations (c_i real, s_i complex) act on
+// rows (i, i+1) as [c s; -conj(s) c]; the new one zeroes h_{j+1,j}.
+__global__ __launch_bounds__(64) void k_givens(int j, int m, int ldh, uint64_t mask, double2* __restrict__ base,
+                                              int64_t sb, int offH, int offCS, int offSN, int offG,
+                                              const double2* __restrict__ beta, const double2* __restrict__ nf,
+                                              double tol, double brk, double2* __restrict__ est) {
+  const int q = threadIdx.x;
+  if (q >= 64 || !((mask >> q) & 1ull)) return;
+  double2* mb = base + sb * q;
+  double2* H = mb + offH + (int64_t)ldh * j;
+  double2* cs = mb + offCS;
+  double2* sn = mb + offSN;
+  double2* g = mb + offG;
+  if (j == 0) {
+    g[0] = make_double2(beta[q].x, 0.0);
+    for (int i = 1; i <= m; ++i) g[i] = make_double2(0.0, 0.0);
+  }
+  const double hn = H[j + 1].x;
+  for (int i = 0; i < j; ++i) {
+    const double2 u = H[i], w2 = H[i + 1];
+    const double c = cs[i].x;
+    const double2 sv = sn[i];
+    const double2 a = zmul(sv, w2), b = zmul(zconj(sv), u);
+    H[i] = make_double2(c * u.x + a.x, c * u.y + a.y);
+    H[i + 1] = make_double2(c * w2.x - b.x, c * w2.y - b.y);
+  }
+  const double2 u = H[j], w2 = H[j + 1];
+  const double aa = hypot(u.x, u.y), bb = hypot(w2.x, w2.y);
+  double c;
+  double2 sv;
+  if (bb == 0.0) {
+    c = 1.0;
+    sv = make_double2(0.0, 0.0);
+  } else if (aa == 0.0) {
+    c = 0.0;
+    sv = make_double2(w2.x / bb, -w2.y / bb);
+  } else {
+    const double t = sqrt(aa * aa + bb * bb);
+    c = aa / t;
+    const double2 ph = make_double2(u.x / aa, u.y / aa);
+    const double2 z = zmul(ph, zconj(w2));
+    sv = make_double2(z.x / t, z.y / t);
+  }
+  cs[j] = make_double2(c, 0.0);
+  sn[j] = sv;
+  const double2 a = zmul(sv, w2);
+  H[j] = make_double2(c * u.x + a.x, c * u.y + a.y);
+  H[j + 1] = make_double2(0.0, 0.0);
+  const double2 gj = g[j], b = zmul(zconj(sv), gj);
+  g[j + 1] = make_double2(-b.x, -b.y);
+  g[j] = make_double2(c * gj.x, c * gj.y);
+  const double e = hypot(b.x, b.y) / nf[q].x;
+  const double status = !isfinite(e) ? 3.0 : (e < tol ? 1.0 : (hn <= brk ? 2.0 : 0.0));
+  est[(int64_t)q * m + j] = make_double2(e, status);
+}
+
+// y_i = (g_i - sum_{c > i} R_ic y_c) / R_ii for i = jj-1 .. 0 (R: the rotated
+// Hessenberg matrix, leading dimension ldh)
+__global__ void k_backsub(int jj, int ldh, const double2* __restrict__ H, const double2* __restrict__ g,
+                          double2* __restrict__ y) {
+  if (threadIdx.x != 0) return;
+  for (int i = jj - 1; i >= 0; --i) {
+    double2 s = g[i];
+    for (int c = i + 1; c < jj; ++c) {
+      const double2 t = zmul(H[i + (int64_t)ldh * c], y[c]);
+      s.x -= t.x;
+      s.y -= t.y;
+    }
+    y[i] = zdiv(s, H[i + (int64_t)ldh * i]);
+  }
+}
+
 __global__ void k_axpby(int64_t n, double2 a, const double2* __restrict__ x, double2 b, double2* __restrict__ y) {
   for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += (int64_t)gridDim.x * blockDim.x) {
     const double2 xv = x[l], yv = y[l];
@@ -481,6 +569,21 @@ void launch_icwy(int j, int ldl, const double2* r, const double2* l, double2* L,
   HZ_HIP(hipGetLastError());
 }
 
+void launch_givens(int j, int m, int ldh, uint64_t mask, double2* base, int64_t sb, int offH, int offCS, int offSN,
+                   int offG, const double2* beta, const double2* nf, double tol, double brk, double2* est,
+                   hipStream_t s) {
+  HZ_CHECK(j >= 0 && j < m, HZ_ERR_ARG, "Givens step %d of %d", j, m);
+  hipLaunchKernelGGL(k_givens, dim3(1), dim3(64), 0, s, j, m, ldh, mask, base, sb, offH, offCS, offSN, offG, beta, nf,
+                     tol, brk, est);
+  HZ_HIP(hipGetLastError());
+}
+
+void launch_backsub(int jj, int ldh, const double2* H, const double2* g, double2* y, hipStream_t s) {
+  HZ_CHECK(jj >= 1 && jj < ldh, HZ_ERR_ARG, "back substitution over %d columns (ldh %d)", jj, ldh);
+  hipLaunchKernelGGL(k_backsub, dim3(1), dim3(64), 0, s, jj, ldh, H, g, y);
+  HZ_HIP(hipGetLastError());
+}
+
 void launch_lincomb(int64_t n, int nv, const double2* c, const double2* X, int64_t xstride, double2* y,
                     bool accumulate, hipStream_t s) {
   hipLaunchKernelGGL(k_lincomb, dim3(grid_for(n)), dim3(256), 0, s, n, nv, c, X, xstride, y, accumulate ? 1 : 0);
@@ -503,11 +606,12 @@ void launch_sub(int64_t n, const double2* a, const double2* b, double2* out, hip
   HZ_HIP(hipGetLastError());
 }
 
-void launch_w_minus_dl(int64_t n, const double2* w, const double2* dL, const double2* x, double2* out,
+void launch_w_minus_dl(int64_t N, int nrhs, const double2* w, const double2* dL, const double2* x, double2* out,
                        hipStream_t s) {
-  // n may span several right-hand sides; dL repeats with period N = n / nrhs,
-  // which callers pass through the problem: handled by the modulo below.
-  hipLaunchKernelGGL(k_w_minus_dl, dim3(grid_for(n)), dim3(256), 0, s, n, n, w, dL, x, out);
+  // nrhs vectors of N points back to back; dL (N entries) repeats per vector
+  HZ_CHECK(N >= 1 && nrhs >= 1, HZ_ERR_ARG, "w - dL x over %d vectors of %lld points", nrhs, (long long)N);
+  const int64_t n = N * nrhs;
+  hipLaunchKernelGGL(k_w_minus_dl, dim3(grid_for(n)), dim3(256), 0, s, n, N, w, dL, x, out);
   HZ_HIP(hipGetLastError());
 }
 

@@ -160,17 +160,18 @@ class Solver:
     # ----------------------------------------------------------- drivers
     def fgmres(self, f, m=30, restarts=20, tol=1e-8, driver=DRIVER_FGMRES, prec=PREC_FP64, ortho=ORTHO_MGS,
                mask=MASK_FULL, monitor=0, breakdown=0.0, hist_cap=8192, check=False, batch=0, graph=False,
-               autotune=False):
+               autotune=False, chunk=0):
         """Solve for every row of f (shape (N,) or (nrhs, N)); FGMRES advances
         `batch` right-hand sides in lockstep (0: up to 16), `graph` replays each
-        Arnoldi step's launches from a captured HIP graph."""
+        chunk of `chunk` Arnoldi steps (0: 1, or 5 with graphs) from a captured
+        HIP graph."""
         f, nr = self._vec(f)
         u = _torch().zeros_like(f)
         hist = np.zeros(hist_cap)
         o = abi.KrylovOpts(driver=int(driver), m=int(m), max_restarts=int(restarts), ortho=int(ortho),
                            mask=int(mask), monitor=int(monitor), tol=float(tol), breakdown=float(breakdown),
                            hist=hist.ctypes.data_as(C.POINTER(C.c_double)), hist_cap=hist_cap, prec=int(prec),
-                           batch=int(batch), graph=int(bool(graph)), autotune=int(bool(autotune)))
+                           batch=int(batch), graph=int(bool(graph)), autotune=int(bool(autotune)), chunk=int(chunk))
         st = (abi.KrylovStats * nr)()
         _torch().cuda.synchronize()
         rc = abi.lib().hz_fgmres(self.h, nr, _ptr(f), _ptr(u), C.byref(o), st)
@@ -223,10 +224,11 @@ def factor_many(solvers, prec=PREC_C64):
 
 
 def fgmres_many(solvers, fs, m=30, restarts=20, tol=1e-8, prec=PREC_C64, ortho=ORTHO_MGS, mask=MASK_FULL,
-                batch=0, graph=False, autotune=False, hist_cap=8192):
+                batch=0, graph=False, autotune=False, hist_cap=8192, chunk=0):
     """FGMRES on several problems at once: fs[i] (complex128 CUDA tensor of
     shape (N_i,) or (nrhs_i, N_i)) is solved on solvers[i]; every problem runs
-    its lockstep batches on its own stream.  Returns [(u_i, [stats...])]."""
+    its lockstep batches on its own stream, all queued from the calling
+    thread.  Returns [(u_i, [stats...])]."""
     torch = _torch()
     n = len(solvers)
     fv, nr, us, sts = [], [], [], []
@@ -240,7 +242,7 @@ def fgmres_many(solvers, fs, m=30, restarts=20, tol=1e-8, prec=PREC_C64, ortho=O
     o = abi.KrylovOpts(driver=DRIVER_FGMRES, m=int(m), max_restarts=int(restarts), ortho=int(ortho), mask=int(mask),
                        monitor=0, tol=float(tol), breakdown=0.0, hist=hist.ctypes.data_as(C.POINTER(C.c_double)),
                        hist_cap=hist_cap, prec=int(prec), batch=int(batch), graph=int(bool(graph)),
-                       autotune=int(bool(autotune)))
+                       autotune=int(bool(autotune)), chunk=int(chunk))
     hs = (C.c_void_p * n)(*[S.h.value for S in solvers])
     nra = (C.c_int * n)(*nr)
     fa = (C.c_void_p * n)(*[f.data_ptr() for f in fv])

@@ -611,8 +611,9 @@ def test_fgmres_fp64_and_mixed_vs_oracle(hz, oracle):
 @pytest.mark.parametrize("m", [64, 70])
 def test_fgmres_icwy_device_and_host_corrections(hz, oracle, m):
     """Restarts up to 64 correct the one-pass Gram-Schmidt coefficients on the
-    device, longer ones on the host: both give the oracle's FGMRES(m)
-    (iterations +-1, residual history, solution)."""
+    device (compact-WY), longer ones run the sequential modified Gram-Schmidt:
+    both give the oracle's FGMRES(m) (iterations +-1, residual history,
+    solution)."""
     print("case icwy m=%d" % m, flush=True)
     d, c = s7_case(hz, n=(20, 18, 22), npml=4, smooth=True)
     S, P = hz.Solver(d, c), oracle.Problem(d, c)
@@ -684,27 +685,41 @@ def test_fgmres_lockstep_batch_matches_single(hz, oracle, shape):
     assert _rel(_host(ub[0]), u_o) < 1e-7
 
 
-def test_fgmres_graph_replay_matches_eager(hz):
-    """Arnoldi steps replayed from captured HIP graphs over several restart
-    cycles (eager orthogonalisation and host syncs between the replays) give
-    the eager run's residual history and solution."""
+@pytest.mark.parametrize("chunk", [0, 2, 4])
+def test_fgmres_graph_replay_matches_eager(hz, oracle, chunk):
+    """Chunks of whole Arnoldi steps (preconditioner, operator, Gram-Schmidt,
+    normalisation, device Givens update) replayed from captured HIP graphs over
+    several restart cycles, with the host's eager cycle ends and syncs between
+    the replays, give the eager step-by-step run's residual history and
+    solution; members stop inside a chunk.  The oracle's FGMRES pins the first
+    member."""
+    print("case graph chunk %d" % chunk, flush=True)
     d, c = s7_case(hz, n=(20, 18, 22), npml=4, smooth=True)
     S = hz.Solver(d, c)
     S.factor(hz.abi.PREC_C64)
     f = np.stack([hz.problem.point_source(d, p, amplitude=1.0) for p in [(9, 8, 11), (5, 12, 7), (14, 6, 15)]])
     ue, ste = S.fgmres(_dev(f), m=6, restarts=60, tol=1e-9, prec=hz.abi.PREC_C64)
-    ug, stg = S.fgmres(_dev(f), m=6, restarts=60, tol=1e-9, prec=hz.abi.PREC_C64, graph=True)
-    assert ste[0].restarts >= 3  # each step's graph is replayed at least twice
+    ug, stg = S.fgmres(_dev(f), m=6, restarts=60, tol=1e-9, prec=hz.abi.PREC_C64, graph=True, chunk=chunk)
+    uc, stc = S.fgmres(_dev(f), m=6, restarts=60, tol=1e-9, prec=hz.abi.PREC_C64, chunk=max(chunk, 3))
+    assert ste[0].restarts >= 3  # each chunk's graph is replayed at least twice
     for q in range(3):
         assert stg[q].converged and stg[q].iters == ste[q].iters
+        assert stc[q].converged and stc[q].iters == ste[q].iters
     assert np.allclose(stg[0].hist, ste[0].hist, rtol=1e-12, atol=0)
+    assert np.allclose(stc[0].hist, ste[0].hist, rtol=1e-12, atol=0)
     assert (torch.linalg.vector_norm(ug - ue) / torch.linalg.vector_norm(ue)).item() < 1e-12
+    assert (torch.linalg.vector_norm(uc - ue) / torch.linalg.vector_norm(ue)).item() < 1e-12
+    P = oracle.Problem(d, c)
+    u_o, st_o = P.fgmres(f[0], m=6, rest=60, tol=1e-9)
+    assert abs(st_o.iters - stg[0].iters) <= 1
+    assert _rel(_host(ug[0]), u_o) < 1e-6
 
 
 def test_fgmres_many_problems_match_separate_solves(hz):
     """Three frequencies on grids of different sizes, factored and solved
-    concurrently (one stream and host thread each, several right-hand sides per
-    grid), give the separate solves' iteration counts and solutions."""
+    concurrently (one stream each, all queued from one host thread, several
+    right-hand sides per grid), give the separate solves' iteration counts and
+    solutions."""
     pr = hz.problem
     shapes = [((20, 18, 22), 2), ((64, 14, 12), 3), ((24, 20, 16), 2)]
     solvers, fs = [], []
@@ -721,6 +736,49 @@ def test_fgmres_many_problems_match_separate_solves(hz):
         for q in range(f.shape[0]):
             assert st[q].converged and st[q].iters == st1[q].iters
         assert (torch.linalg.vector_norm(u - u1) / torch.linalg.vector_norm(u1)).item() < 1e-12
+
+
+def test_config4_composed_batched_uneven_vs_oracle(hz, oracle):
+    """The batched configuration at sizes the oracle finishes quickly: four
+    frequencies on four uneven grids (10 points per minimum wavelength, quadratic
+    PML), 8 seeded interior point sources each, seeded random velocity, all
+    grids factored and solved concurrently with the complex64 preconditioner,
+    FP64 FGMRES in lockstep batches of 8, graph-replayed chunks of Arnoldi
+    steps and autotuned launch choices.  Every source's solution matches the
+    oracle's FP64 FGMRES (1e-6) and its iteration count matches its single
+    eager solve (+-1)."""
+    pr = hz.problem
+    grids = [((20, 18, 22), 4), ((24, 20, 16), 4), ((28, 22, 26), 5), ((64, 14, 12), (8, 3, 3))]
+    nsrc, tol, m = 8, 1e-10, 12
+    solvers, probs, fs = [], [], []
+    for i, (shape, npml) in enumerate(grids):
+        h = 1.0 / min(shape)  # the coarsest spacing sets the points per wavelength
+        d0 = pr.cube_geometry(shape, npml, 1.0)
+        c = pr.random_velocity(d0, 40 + i, corr_len=0.25, amp=0.1, cmin=0.8, cmax=1.2)
+        om = pr.omega_for_ppw(float(c.min()), h, 10.0)
+        np_max = max(npml) if isinstance(npml, tuple) else npml
+        d = pr.cube_geometry(shape, npml, om, beta=0.5, pml_c=3.0 * np.log(1e6) / (2.0 * np_max * h))
+        rng = np.random.default_rng(500 + i)
+        lo = [npml[a] if isinstance(npml, tuple) else npml for a in range(3)]
+        pts = [tuple(int(rng.integers(lo[a], shape[a] - lo[a])) for a in range(3)) for _ in range(nsrc)]
+        fs.append(np.stack([pr.point_source(d, p, amplitude=1.0) for p in pts]))
+        solvers.append(hz.Solver(d, c))
+        probs.append(oracle.Problem(d, c))
+        print("config4 grid %s omega %.2f" % (shape, om), flush=True)
+    hz.factor_many(solvers, hz.abi.PREC_C64)
+    res = hz.fgmres_many(solvers, [_dev(f) for f in fs], m=m, restarts=200, tol=tol, prec=hz.abi.PREC_C64,
+                         batch=nsrc, graph=True, autotune=True)
+    for i, (S, P, f, (u, st)) in enumerate(zip(solvers, probs, fs, res)):
+        uh = _host(u)
+        for q in range(nsrc):
+            u1, st1 = S.fgmres(_dev(f[q]), m=m, restarts=200, tol=tol, prec=hz.abi.PREC_C64)
+            u_o, st_o = P.fgmres(f[q], m=m, rest=200, tol=tol)
+            err = _rel(uh[q], u_o)
+            print("grid %d source %d: iters batched %d single %d oracle %d, err vs oracle %.2e"
+                  % (i, q, st[q].iters, st1.iters, st_o.iters, err))
+            assert st[q].converged and st1.converged and st_o.converged
+            assert abs(st[q].iters - st1.iters) <= 1
+            assert err < 1e-6
 
 
 def test_config1_point_source_matches_green_function(hz):

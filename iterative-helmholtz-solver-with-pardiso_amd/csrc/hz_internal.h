@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <complex>
 #include <cstdint>
 #include <cstdio>
 #include <map>
@@ -174,6 +175,10 @@ void autotune(hz_problem* p, int prec, int nrhs, hz_tune_info* info);
 
 // ------------------------------------------------------------ setup (host)
 void setup_tables(hz_problem* p, const double* c_host);
+// PML stretching factor 1 / (1 + i pml_c x^2 / omega) at (possibly half-integer)
+// node s of an axis of n nodes with `pml` layer nodes per side (alpha,
+// functions.cpp:1839-1878)
+std::complex<double> stretch(int64_t n, int pml, double omega, double pml_c, double s);
 
 // ------------------------------------------------------------ kernels
 // z-FFT over every (x,y) line.  in/out: complex arrays of `in_prec`/`out_prec`

@@ -21,7 +21,7 @@ static inline zc zz(double2 v) { return zc(v.x, v.y); }
 // damping is pml_c x^2 and the factor 1 / (1 + i d / omega).  Half-integer
 // points just inside n - pml on the right count as interior.
 // (alpha and d, functions.cpp:1839-1878, 893-906.)
-static zc stretch(int64_t n, int pml, double omega, double pml_c, double s) {
+zc stretch(int64_t n, int pml, double omega, double pml_c, double s) {
   if (pml <= 0) return zc(1.0, 0.0);
   bool left = s < (double)pml;
   bool right = s >= (double)(n - pml);

@@ -547,35 +547,25 @@ def test_gmres_ref_matches_reference_path(hz, oracle, homo):
         assert _rel(_host(u), u_o) < 1e-10
 
 
-@pytest.mark.parametrize("homo", [1, 0])
-def test_reference_signatures_drop_in(hz, oracle, tmp_path, homo):
-    """A C++ caller written against the reference's own signatures (hz_ref.h:
-    Solve3DSparseUsingFT_CSR, ApplyCoeffMatrixA_CSR, Multiply3DSparseUsingFT,
-    FGMRES), linked against libhz.so, gives the oracle's results on the
-    reference grid (1e-10; GMRES within the reference's own tolerance)."""
-    import subprocess
-    drv = os.path.join(os.path.dirname(hz.abi.LIB_PATH), "_build", "ref_shim_driver")
-    if not os.path.exists(drv):
-        pytest.fail("the reference-signature test driver is not built (run __graft_entry__.build())")
-    n_nopml, nz_nopml, pml, spg, length, nu, beta, c_z = 8, 8, 4, 3, 240.0, 4.0, 0.5, 1280.0
-    pr = hz.problem
-    d = pr.ref_geometry(n_nopml=n_nopml, pml=pml, spg=spg, nz_nopml=nz_nopml, length=(length,) * 3, nu=nu,
-                        beta=beta, c_z=c_z, homo=bool(homo))
-    c = pr.ref_sound_speed(d)
+@pytest.mark.parametrize("kind", ["homo", "linear", "array", "file"])
+def test_reference_signatures_drop_in(hz, oracle, tmp_path, kind):
+    """A C++ caller written against the reference's own signatures (hz_ref.h),
+    linked against libhz.so: the set-up sequence of FGMRES.cpp:77-96 (medium,
+    layer extensions, c0, delta-L), then Solve3DSparseUsingFT_CSR,
+    ApplyCoeffMatrixA_CSR with the delta-L it computed, Multiply3DSparseUsingFT
+    and FGMRES give the oracle's results on the reference grid (1e-10; GMRES
+    within the reference's own tolerance), for the built-in media, a caller's
+    array (hz_ref_set_sound_speed) and an OverThrust file."""
+    from ref_driver import medium, run_driver
+    homo, arg, c = medium(kind, tmp_path)
+    d, load, r = run_driver(tmp_path, homo, arg)
+    assert "status 0" in r.stdout, r.stdout
     P = oracle.Problem(d, c)
-    f = pr.point_source(d)
-    fin = tmp_path / "f.bin"
-    f.astype(np.complex128).tofile(fin)
-    out = str(tmp_path / "res")
-    r = subprocess.run([drv, str(n_nopml), str(nz_nopml), str(pml), str(spg), repr(length), repr(nu), repr(beta),
-                        str(homo), repr(c_z), "10", "5", "1e-8", str(fin), out], capture_output=True, text=True,
-                       timeout=120)
-    assert r.returncode == 0, (r.returncode, r.stdout, r.stderr)
-    load = lambda name: np.fromfile(out + "_%s.bin" % name, dtype=np.complex128)
+    f = hz.problem.point_source(d)
+    assert _rel(load("dl"), P.deltaL()) < 1e-13
     assert _rel(load("precond"), P.precond_apply(f)) < 1e-10
     assert _rel(load("L0"), P.L0_apply(f)) < 1e-10
-    dl = load("dl")
-    assert _rel(load("applyA"), f - dl * P.precond_apply(f)) < 1e-10
+    assert _rel(load("applyA"), P.op_pre_apply(f)) < 1e-10
     u_o, st_o = P.gmres_ref(f, m=10, rest=5, thresh=1e-12, tol=1e-8, mask=1, ortho=0)
     assert _rel(load("fgmres"), u_o) < 1e-8
 

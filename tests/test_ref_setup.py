@@ -1,0 +1,131 @@
+"""The reference-signature set-up functions (include/hz_ref.h: SetSoundSpeed3D,
+SetSoundSpeed3DFromFile, the two layer extensions, SetSoundSpeed2D,
+GenerateDeltaL, GenSparseMatrixOnline2DwithPMLFast, alpha, beta2D_pml, beta3D)
+called by a C++ program written against the reference's signatures
+(tests/ref_shim_driver.cpp, the call sequence of FGMRES.cpp:77-96), against the
+oracle on the same grid and medium.  Host code only: the driver's device calls
+are not checked here (on a machine without a GPU they fail and leave NaN)."""
+import numpy as np
+import pytest
+
+from ref_driver import medium, run_driver
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def check_setup(d, load, c, oracle):
+    """sound3D, sound2D (c0), delta-L, one plane's CSR and the coefficient
+    functions against the oracle built on the medium c."""
+    P = oracle.Problem(d, c)
+    nx, ny, nz = d.n
+    s3 = load("sound3d")
+    assert np.array_equal(s3.real, c) and not np.any(s3.imag)
+    s2 = load("sound2d")
+    assert _rel(s2, P.c0()) < 1e-14
+    assert _rel(load("dl"), P.deltaL()) < 1e-13
+    # the plane CSR (1-based, 5 nx ny - 2 nx - 2 ny entries, rows in order)
+    ia, ja, vals = load("csr_ia", np.int32), load("csr_ja", np.int32), load("csr_val")
+    n2 = nx * ny
+    assert ia[0] == 1 and ia[-1] == vals.size + 1 and vals.size == 5 * n2 - 2 * nx - 2 * ny
+    M = np.zeros((n2, n2), np.complex128)
+    for r in range(n2):
+        cols = ja[ia[r] - 1:ia[r + 1] - 1] - 1
+        assert np.all(np.diff(cols) > 0)
+        M[r, cols] = vals[ia[r] - 1:ia[r + 1] - 1]
+    import dense
+    Dx, Dy = dense.axis_matrix(P.tables(0)), dense.axis_matrix(P.tables(1))
+    kwave = 0.75e-3 + 0.2e-3j
+    M_ref = np.kron(np.eye(ny), Dx) + np.kron(Dy, np.eye(nx)) + kwave * np.eye(n2)
+    assert np.abs(M - M_ref).max() <= 1e-14 * np.abs(M_ref).max()
+    # alpha at the half-nodes -1/2 .. n of x
+    s = 0.5 * np.arange(-1, 2 * nx + 1)
+    al = load("alpha")
+    al_ref = np.array([oracle.alpha(nx, d.pml[0], d.omega, d.pml_c, v) for v in s])
+    assert np.abs(al - al_ref).max() <= 1e-15
+    # beta2D_pml: (j, i, case -2..2) and beta3D: (k step 3, j step 2, i, case -3..3)
+    cmx, cpx, cdx = P.tables(0)
+    cmy, cpy, cdy = P.tables(1)
+    cmz, cpz, cdz = P.tables(2)
+    b2 = load("beta2d").reshape(ny, nx, 5)
+    J, I = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    ref2 = np.stack([cmy[J], cmx[I], cdx[I] + cdy[J] + kwave, cpx[I], cpy[J]], axis=-1)
+    assert np.abs(b2 - ref2).max() <= 1e-15 * np.abs(ref2).max()
+    ks, js = np.arange(0, nz, 3), np.arange(0, ny, 2)
+    b3 = load("beta3d").reshape(ks.size, js.size, nx, 7)
+    K, J, I = np.meshgrid(ks, js, np.arange(nx), indexing="ij")
+    ref3 = np.stack([cmz[K], cmy[J], cmx[I], cdx[I] + cdy[J] + cdz[K], cpx[I], cpy[J], cpz[K]], axis=-1)
+    assert np.abs(b3 - ref3).max() <= 1e-15 * np.abs(ref3).max()
+
+
+@pytest.mark.parametrize("kind", ["homo", "linear", "array", "file"])
+def test_setup_functions_vs_oracle(tmp_path, oracle, kind):
+    """Built-in media (HOMO, linear), a caller's array (hz_ref_set_sound_speed)
+    and an OverThrust file (hz_ref_config.velocity_file,
+    SetSoundSpeed3DFromFile)."""
+    homo, arg, c = medium(kind, tmp_path)
+    d, load, _ = run_driver(tmp_path, homo, arg)
+    check_setup(d, load, c, oracle)
+
+
+def test_failed_drop_in_calls_leave_nan(tmp_path):
+    """Without a device the drop-in solver calls report the failure through
+    hz_ref_last_status and leave NaN in their outputs."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a device is present: the drop-in calls succeed")
+    _, load, r = run_driver(tmp_path, 0)
+    assert "status 0" not in r.stdout
+    for name in ("precond", "applyA", "L0", "fgmres"):
+        assert np.all(np.isnan(load(name)))
+
+
+# The functions of the reference's Source/functions.cpp that libhz.so replaces
+# (the integration recipe renames them with -D<name>=ref_<name> when it
+# compiles functions.cpp; FGMRES.cpp and BcGSTAB.cpp are left out of the build).
+REPLACED = ["Solve3DSparseUsingFT_CSR", "ApplyCoeffMatrixA_CSR", "Multiply3DSparseUsingFT", "alpha", "beta3D",
+            "beta2D_pml", "GenSparseMatrixOnline2DwithPMLFast", "SetSoundSpeed3D", "SetSoundSpeed3DFromFile",
+            "SetSoundSpeed2D", "GenerateDeltaL", "HeteroSoundSpeed3DExtensionToPML",
+            "HeteroSoundSpeed2DExtensionToPML"]
+
+
+@pytest.mark.parametrize("rename", [True, False])
+def test_integration_recipe_binds_libhz(tmp_path, oracle, rename):
+    """The integration recipe on a program with a translation unit that defines
+    every replaced function (tests/ref_link_check.cpp, in place of the
+    reference's functions.cpp): with the renaming flags the program's calls
+    bind to libhz.so (the set-up results match the oracle, the unit's own
+    definitions carry the ref_ names); without them the unit's definitions
+    interpose and the first call aborts."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    pkg = os.path.join(root, "iterative-helmholtz-solver-with-pardiso_amd")
+    inc = os.path.join(root, "include")
+    flags = ["-D%s=ref_%s" % (n, n) for n in REPLACED] if rename else []
+    obj = str(tmp_path / "functions.o")
+    exe = str(tmp_path / "driver")
+    subprocess.run(["g++", "-O1", "-std=c++17", "-I" + inc, "-c"] + flags +
+                   [os.path.join(root, "tests", "ref_link_check.cpp"), "-o", obj], check=True)
+    subprocess.run(["g++", "-O1", "-std=c++17", "-I" + inc, os.path.join(root, "tests", "ref_shim_driver.cpp"), obj,
+                    "-o", exe, "-L" + pkg, "-lhz", "-Wl,-rpath," + pkg], check=True)
+    syms = subprocess.run(["nm", "-C", exe], capture_output=True, text=True, check=True).stdout
+    import ref_driver
+    if rename:
+        for n in REPLACED:
+            assert ("T ref_%s(" % n) in syms, n
+        homo, arg, c = medium("linear", tmp_path)
+        old = ref_driver.DRV
+        ref_driver.DRV = exe
+        try:
+            d, load, _ = run_driver(tmp_path, homo, arg)
+        finally:
+            ref_driver.DRV = old
+        check_setup(d, load, c, oracle)
+    else:
+        d = ref_driver.ref_desc(0)
+        fin = tmp_path / "f.bin"
+        np.zeros(d.N, np.complex128).tofile(fin)
+        r = subprocess.run([exe] + [str(v) for v in (8, 8, 4, 3, 240.0, 4.0, 0.5, 0, 1280.0, 10, 5, 1e-8)] +
+                           [str(fin), str(tmp_path / "res")], capture_output=True, text=True, timeout=120)
+        assert r.returncode != 0 and "instead of libhz.so's" in r.stderr

@@ -9,7 +9,14 @@ error of the mixed solution against it, the live roofline of the dominant
 kernel (the block-tridiagonal plane solve) and the CPU baseline (the FP64
 oracle on the host cores, on a bounded sample of the same workload).
 
-    python bench.py [--gpus 1] [--steps 1] [--warmup 1] [--size 200] [--m 45] [--config mixed200|batched|adjoint]
+The default run adds the other configurations of the metric as keys of the
+same JSON line, each with its own small step count (--extra-steps,
+--extra-warmup): "config2" (128^3 incl. 16-point PML, FP64 preconditioner),
+"batched" (four frequencies on 96^3..192^3 grids, 8 sources each, right-hand
+sides per second) and "adjoint" (128^3 16-source misfit gradients per second,
+checked against the FP64 gradient and central finite differences).
+
+    python bench.py [--gpus 1] [--steps 1] [--warmup 1] [--size 200] [--m 35] [--config all|mixed200|batched|adjoint]
 """
 import argparse
 import importlib
@@ -40,10 +47,13 @@ def parse():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=None)
-    ap.add_argument("--config", choices=["mixed200", "batched", "adjoint"], default="mixed200",
-                    help="mixed200: the 200^3 mixed-precision solve (default); batched: four frequencies on "
-                         "96^3..192^3 grids with 8 sources each, solved concurrently; adjoint: the 128^3 "
-                         "16-source misfit gradient")
+    ap.add_argument("--config", choices=["all", "mixed200", "batched", "adjoint"], default="all",
+                    help="all (default): the 200^3 mixed-precision solve as the headline, with the 128^3 FP64, "
+                         "batched and adjoint configurations as extra keys; mixed200: the headline only; "
+                         "batched: four frequencies on 96^3..192^3 grids with 8 sources each, solved "
+                         "concurrently; adjoint: the 128^3 16-source misfit gradient")
+    ap.add_argument("--extra-steps", type=int, default=1, help="timed steps of each extra configuration")
+    ap.add_argument("--extra-warmup", type=int, default=1, help="untimed steps of each extra configuration")
     ap.add_argument("--fd-dirs", type=int, default=2, help="adjoint: finite-difference directions checked")
     ap.add_argument("--kernel-only", action="store_true",
                     help="factor, then run a 1 GiB device copy (counter reference) and 5 plane solves; "
@@ -70,10 +80,10 @@ def main():
     pr = hz.problem
     A = hz.abi
     if a.config == "batched":
-        bench_batched(a, hz, np, torch)
+        print(json.dumps(bench_batched(a, hz, np, torch, a.steps, a.warmup)), flush=True)
         return
     if a.config == "adjoint":
-        bench_adjoint(a, hz, np, torch)
+        print(json.dumps(bench_adjoint(a, hz, np, torch, a.steps, a.warmup)), flush=True)
         return
 
     n = a.size
@@ -181,6 +191,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(n, fam),
+            "traffic_source": traffic_source(n, fam),
             "bytes_per_launch": bytes_kernel,
             "ms_per_launch": round(ms_kernel, 4),
         },
@@ -203,6 +214,17 @@ def main():
     if not a.skip_cpu:
         note("cpu baseline sample")
         out["cpu_baseline"] = cpu_baseline(d, c, st32.iters, a.m, a.cpu_threads)
+    del u32, f, buf
+    S.close()
+    torch.cuda.empty_cache()
+    if a.config == "all":
+        for key, fn in (("config2", bench_config2), ("batched", bench_batched), ("adjoint", bench_adjoint)):
+            note("extra configuration: %s" % key)
+            try:
+                out[key] = fn(a, hz, np, torch, a.extra_steps, a.extra_warmup)
+            except Exception as e:  # the headline stands on its own
+                out[key] = {"error": "%s: %s" % (type(e).__name__, e)}
+            torch.cuda.empty_cache()
     print(json.dumps(out), flush=True)
 
 
@@ -222,7 +244,7 @@ def note(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def bench_batched(a, hz, np, torch):
+def bench_batched(a, hz, np, torch, steps, warmup):
     """Four frequencies, each on its own grid scaled for 10 points per minimum
     wavelength (96^3, 128^3, 160^3, 192^3 incl. 12..24-point PML), 8 point
     sources each at seeded uniform-random interior nodes (32 right-hand sides),
@@ -253,13 +275,13 @@ def bench_batched(a, hz, np, torch):
     for S, inf in zip(solvers, info):
         inf["tuning"] = S.autotune(A.PREC_C64, nsrc)
         note("grid %d^3 tuned: %s" % (inf["grid"], inf["tuning"]))
-    for w in range(a.warmup):
+    for w in range(warmup):
         hz.factor_many(solvers, A.PREC_C64)
         hz.fgmres_many(solvers, fs, m=a.m, restarts=max(1, a.max_iters // a.m), tol=a.tol, prec=A.PREC_C64,
                        batch=nsrc, graph=True)
         note("warmup %d done" % w)
     times = []
-    for _ in range(a.steps):
+    for _ in range(steps):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         hz.factor_many(solvers, A.PREC_C64)
@@ -282,19 +304,22 @@ def bench_batched(a, hz, np, torch):
     total_rhs = nsrc * len(grids)
     out = {
         "metric": "batched uneven solves: right-hand sides to rel-res 1e-8 per second (4 grids x 8 sources)",
-        "value": round(total_rhs / t_tot, 4), "unit": "rhs/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "value": round(total_rhs / t_tot, 4), "unit": "rhs/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
         "ms_per_step": round(1e3 * t_tot, 2), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "c64 preconditioner + f64 Krylov", "data": "synthetic",
         "config": {"workload": "96^3/128^3/160^3/192^3 incl. 12-24-point PML, seeded random c in [0.7,1.5], "
                                "10 ppw at c_min, 8 seeded interior sources per grid, FGMRES(%d) to %.0e, lockstep "
-                               "batches of 8, graph-replayed steps, grids concurrent" % (a.m, a.tol),
+                               "batches of 8, graph-replayed chunks of Arnoldi steps, grids concurrent from one "
+                               "host thread" % (a.m, a.tol),
                    "grids": info},
         "factor_s": round(t_fac, 4), "converged": conv,
     }
-    print(json.dumps(out), flush=True)
+    for S in solvers:
+        S.close()
+    return out
 
 
-def bench_adjoint(a, hz, np, torch):
+def bench_adjoint(a, hz, np, torch, steps, warmup):
     """Adjoint misfit gradient on 128^3 incl. 16-point PML.  True model c = 1 +
     3 seeded Gaussian blobs (+-10%, width 0.08); current model its periodic
     Gaussian smoothing (sigma 0.15).  16 sources on a regular 4x4 grid on the
@@ -335,11 +360,11 @@ def bench_adjoint(a, hz, np, torch):
     note("data from the true model: %d iterations (max over sources)" % max(s.iters for s in stt))
     S = hz.Solver(d, c_cur)
     S.factor(A.PREC_C64)
-    for w in range(a.warmup):
+    for w in range(warmup):
         S.misfit_forward(fd, rec, data, prec=A.PREC_C64, batch=16, **kw)
         S.misfit_backward()
     times = []
-    for _ in range(a.steps):
+    for _ in range(steps):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         J = S.misfit_forward(fd, rec, data, prec=A.PREC_C64, batch=16, **kw)
@@ -378,7 +403,7 @@ def bench_adjoint(a, hz, np, torch):
         note("fd direction %d: adjoint %.6e fd %.6e rel %.2e" % (k, dJ_adj, dJ_fd, fd_rel[-1]))
     out = {
         "metric": "adjoint misfit gradients per second (128^3, 16 sources, mixed precision, 1e-8)",
-        "value": round(1.0 / t, 5), "unit": "grads/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "value": round(1.0 / t, 5), "unit": "grads/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
         "ms_per_step": round(1e3 * t, 2), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "c64 preconditioner + f64 Krylov", "data": "synthetic",
         "config": {"workload": "128^3 incl. 16-point PML, 3 Gaussian blobs +-10%% (w 0.08) smoothed by sigma 0.15, "
@@ -387,7 +412,65 @@ def bench_adjoint(a, hz, np, torch):
                    "omega": omega},
         "J": J, "J_fp64": J64, "grad_rel_err_vs_fp64": err64, "fd_rel_err": fd_rel,
     }
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def bench_config2(a, hz, np, torch, steps, warmup):
+    """128^3 incl. 16-point PML, seeded random c in [0.7, 1.5] (correlation 0.1),
+    10 points per minimum wavelength, one centre source, FP64 preconditioner
+    and FGMRES to relative residual 1e-8 (the value); the mixed-precision
+    solve and its error against the FP64 one beside it.  The true residual of
+    the FP64 solution is recomputed with the 7-point operator."""
+    pr, A = hz.problem, hz.abi
+    n, npml = 128, 16
+    h = 1.0 / n
+    c = pr.random_velocity(pr.cube_geometry(n, npml, 1.0), seed=a.seed)
+    omega = pr.omega_for_ppw(float(c.min()), h, 10.0)
+    d = pr.cube_geometry(n, npml, omega, beta=a.beta, pml_c=pml_strength(npml, h))
+    f = torch.from_numpy(pr.point_source(d, (n // 2 - 1,) * 3, amplitude=1.0 / h ** 3)).cuda()
+    S = hz.Solver(d, c)
+    kw = dict(m=a.m, restarts=max(1, a.max_iters // a.m), tol=a.tol)
+    res = {}
+    for prec, name in ((A.PREC_FP64, "fp64"), (A.PREC_C64, "mixed")):
+        S.factor(prec)
+        S.autotune(prec, 1)
+        for _ in range(warmup):
+            S.fgmres(f, prec=prec, **kw)
+        times = []
+        for _ in range(steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            S.factor(prec)
+            u, st = S.fgmres(f, prec=prec, **kw)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        res[name] = (float(np.mean(times)), u, st)
+        note("config2 %s: %.2f s, %d iterations" % (name, res[name][0], st.iters))
+    t64, u64, st64 = res["fp64"]
+    t32, u32, st32 = res["mixed"]
+    _, rel = S.residual(f, u64)
+    out = {
+        "metric": "time-to-rel-res 1e-8, FP64 preconditioner (128^3 heterogeneous)",
+        "value": round(t64, 4), "unit": "s", "steps": steps, "warmup": warmup, "higher_is_better": False,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "128^3 incl. 16-point PML, seeded random c in [0.7,1.5] (corr. 0.1), 10 ppw at "
+                               "c_min, 1 centre source, FGMRES(%d) to %.0e" % (a.m, a.tol),
+                   "omega": omega, "beta": a.beta, "seed": a.seed},
+        "iters_fp64": st64.iters, "true_rel_res_fp64": float(rel[0]), "converged": bool(st64.converged),
+        "mixed_s": round(t32, 4), "iters_mixed": st32.iters,
+        "err_mixed_vs_fp64": (torch.linalg.vector_norm(u32 - u64) / torch.linalg.vector_norm(u64)).item(),
+    }
+    S.close()
+    return out
+
+
+def traffic_source(n, fam):
+    """Where roofline.traffic comes from: it is not measured in this run (the
+    counter passes need rocprofv3), it is read from the committed summary."""
+    names = {3: "k_solve_dma_c64_200cube_pmc.md", 5: "k_solve_sym2_c64_200cube_pmc.md"}
+    if n != 200 or fam not in names:
+        return None
+    return "committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary profiles/%s (not measured in this run)" % names[fam]
 
 
 def plane_solve_bytes(dims, fam):

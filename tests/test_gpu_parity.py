@@ -547,6 +547,38 @@ def test_gmres_ref_matches_reference_path(hz, oracle, homo):
         assert _rel(_host(u), u_o) < 1e-10
 
 
+@pytest.mark.parametrize("n_nopml,spg", [(24, 6), (40, 10)])
+@pytest.mark.parametrize("homo", [True, False])
+def test_gmres_ref_reference_sizes(hz, oracle, n_nopml, spg, homo):
+    """The reference's own path at scaled copies of its defaults: x/y PML of 8
+    points, z sponge of spg points, beta = 0.5, the kz map, the running-sum c0,
+    the delta-L band and taper, HOMO (c_z = 1280) and the linear hetero medium,
+    nu = 4 with the domain sized for 10 points per wavelength at c_z.  FP64
+    preconditioner within 1e-10 of the oracle; the reference GMRES (m = 10,
+    conj coefficient, physical-box residual) to 1e-8 with the same iteration
+    count (+-1), per-iteration residuals and solution (1e-10)."""
+    print("case ref sizes n_nopml=%d spg=%d homo=%s" % (n_nopml, spg, homo), flush=True)
+    pr = hz.problem
+    length = n_nopml * 1280.0 / (10 * 4.0)
+    d = pr.ref_geometry(n_nopml=n_nopml, pml=8, spg=spg, length=(length,) * 3, nu=4.0, beta=0.5, homo=homo)
+    c = pr.ref_sound_speed(d)
+    S, P = hz.Solver(d, c), oracle.Problem(d, c)
+    S.factor(hz.abi.PREC_FP64)
+    f = pr.point_source(d)
+    assert _rel(_host(S.precond_apply(_dev(f), hz.abi.PREC_FP64)), P.precond_apply(f)) < 1e-10
+    assert _rel(_host(S.op_pre_apply(_dev(f), hz.abi.PREC_FP64)), P.op_pre_apply(f)) < 1e-10
+    u_o, st_o = P.gmres_ref(f, m=10, rest=30, thresh=1e-12, tol=1e-8, mask=1, ortho=0)
+    u, st = S.fgmres(_dev(f), m=10, restarts=30, tol=1e-8, driver=hz.abi.DRIVER_GMRES_REF,
+                     ortho=hz.abi.ORTHO_REF_CONJ, mask=hz.abi.MASK_PHYS, breakdown=1e-12)
+    err = _rel(_host(u), u_o)
+    print("ref sizes: grid %s iters gpu %d oracle %d, u err %.2e" % (d.n, st.iters, st_o.iters, err))
+    assert st.converged and st_o.converged
+    assert abs(st.iters - st_o.iters) <= 1
+    k = min(len(st.hist), len(st_o.hist), 8)
+    assert np.allclose(st.hist[:k], st_o.hist[:k], rtol=1e-7)
+    assert err < (1e-10 if st.iters == st_o.iters else 1e-7)
+
+
 @pytest.mark.parametrize("kind", ["homo", "linear", "array", "file"])
 def test_reference_signatures_drop_in(hz, oracle, tmp_path, kind):
     """A C++ caller written against the reference's own signatures (hz_ref.h),
@@ -771,18 +803,21 @@ def test_config4_composed_batched_uneven_vs_oracle(hz, oracle):
             assert err < 1e-6
 
 
-def test_config1_point_source_matches_green_function(hz):
+@pytest.mark.parametrize("beta", [0.5, 1.0])
+def test_config1_point_source_matches_green_function(hz, beta):
     """64^3 incl. 8-point PML, c = 1, omega = 2 pi 6.4 (10 points per
-    wavelength), f = +1/h^3 at the centre node: FGMRES to 1e-8 in FP64 and in
-    mixed precision; away from the source and the layers the solution is
-    -e^{ikr}/(4 pi r) to discretisation accuracy, and the two precisions agree
-    to 1e-6."""
+    wavelength), f = +1/h^3 at the centre node, the preconditioner shift of the
+    reference's default (beta = 0.5, main.cpp:70) and of the benchmark (1.0):
+    FGMRES to 1e-8 in FP64 and in mixed precision; away from the source and the
+    layers the solution is -e^{ikr}/(4 pi r) to discretisation accuracy, and the
+    two precisions agree to 1e-6."""
+    print("case config1 beta=%g" % beta, flush=True)
     pr = hz.problem
     n, npml = 64, 8
     h = 1.0 / n
     om = pr.omega_for_ppw(1.0, h, 10)
     assert abs(om - 2 * np.pi * 6.4) < 1e-9
-    d = pr.cube_geometry(n, npml, om, beta=1.0, pml_c=3.0 * np.log(1e6) / (2.0 * npml * h))
+    d = pr.cube_geometry(n, npml, om, beta=beta, pml_c=3.0 * np.log(1e6) / (2.0 * npml * h))
     c = np.ones(d.N)
     idx = (n // 2 - 1,) * 3
     f = _dev(pr.point_source(d, idx, amplitude=1.0 / h ** 3))
@@ -803,6 +838,38 @@ def test_config1_point_source_matches_green_function(hz):
         print("config 1: iterations %d / %d, Green's function error %.3e" % (st64.iters, st32.iters, err))
         assert err < 0.1, err
     assert (torch.linalg.vector_norm(u32 - u64) / torch.linalg.vector_norm(u64)).item() < 1e-6
+
+
+def test_config2_fp64_solve_128cube(hz, oracle):
+    """The 128^3 configuration as stated: 16-point PML, seeded random c in
+    [0.7, 1.5] (correlation 0.1), 10 points per minimum wavelength, centre point
+    source, FP64 preconditioner and FGMRES to 1e-8.  The true residual of the
+    solution, recomputed by the oracle's 7-point operator on the host, is below
+    the tolerance; the mixed-precision solve agrees to 1e-6."""
+    print("case config2 128^3 fp64", flush=True)
+    pr = hz.problem
+    n, npml = 128, 16
+    h = 1.0 / n
+    c = pr.random_velocity(pr.cube_geometry(n, npml, 1.0), 1234)
+    om = pr.omega_for_ppw(float(c.min()), h, 10.0)
+    d = pr.cube_geometry(n, npml, om, beta=1.0, pml_c=3.0 * np.log(1e6) / (2.0 * npml * h))
+    f = pr.point_source(d, (n // 2 - 1,) * 3, amplitude=1.0 / h ** 3)
+    S = hz.Solver(d, c)
+    S.factor(hz.abi.PREC_FP64)
+    u64, st64 = S.fgmres(_dev(f), m=35, restarts=400, tol=1e-8, prec=hz.abi.PREC_FP64)
+    S.factor(hz.abi.PREC_C64)
+    u32, st32 = S.fgmres(_dev(f), m=35, restarts=400, tol=1e-8, prec=hz.abi.PREC_C64)
+    P = oracle.Problem(d, c)
+    uh = _host(u64)
+    rel = np.linalg.norm(P.op_apply(uh) - f) / np.linalg.norm(f)
+    e32 = (torch.linalg.vector_norm(u32 - u64) / torch.linalg.vector_norm(u64)).item()
+    print("config 2: FP64 %d iterations (%.2f s), true residual %.3e; mixed %d iterations, diff %.2e"
+          % (st64.iters, st64.t_solve_s, rel, st32.iters, e32))
+    assert st64.converged and st32.converged
+    assert rel < 1.01e-8
+    assert e32 < 1e-6
+    S.close()
+    torch.cuda.empty_cache()
 
 
 @pytest.mark.parametrize("mode", ["ref", "s7"])

@@ -89,8 +89,12 @@ typedef struct {
   int32_t mode;     /* HZ_MODE_* */
   int32_t kz_symbol;/* HZ_KZ_* */
   int32_t c0_rule;  /* HZ_C0_* */
-  int32_t reserved;
+  int32_t skip_ratio; /* > 0: planes with z symbol lambda_k >= skip_ratio * k0^2 are not factored
+                         and the preconditioner is zero on them (SAVE_MEM, FGMRES.cpp:196-206;
+                         the reference's ratio is 5, or 15 at nu = 2); 0: every plane */
   double c_bg;      /* > 0: constant background speed of the planes (reference HOMO) */
+  double skip_k0;   /* wavenumber k0 of the skip test (the reference's omega / c_z); 0: omega / c_bg,
+                       or omega over the slowest background speed when c_bg = 0 */
 } hz_desc;
 
 typedef struct hz_problem hz_problem;

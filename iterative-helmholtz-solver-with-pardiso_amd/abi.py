@@ -29,8 +29,9 @@ class HzDesc(C.Structure):
         ("mode", C.c_int32),
         ("kz_symbol", C.c_int32),
         ("c0_rule", C.c_int32),
-        ("reserved", C.c_int32),
+        ("skip_ratio", C.c_int32),
         ("c_bg", C.c_double),
+        ("skip_k0", C.c_double),
     ]
 
 
@@ -173,4 +174,6 @@ def make_desc(d):
     o.kz_symbol = int(d.kz_symbol)
     o.c0_rule = int(d.c0_rule)
     o.c_bg = float(d.c_bg)
+    o.skip_ratio = int(getattr(d, "skip_ratio", 0))
+    o.skip_k0 = float(getattr(d, "skip_k0", 0.0))
     return o

@@ -101,6 +101,14 @@ struct hz_problem {
   // host tables
   std::vector<double2> cm[3], cp[3], cdg[3];  // stencil coefficients per axis
   std::vector<double> lamz;                    // z symbol per plane
+  // Plane map: fmap[k] = factor slot of plane k (planes with equal z symbols
+  // share one, k and nz - k for the symmetric symbols), -1 for a plane skipped
+  // by the SAVE_MEM rule (no factor, zero preconditioner output);
+  // splane[s] = the plane whose symbol slot s factors.
+  std::vector<int> fmap, splane;
+  int nslot = 0, nskip = 0;
+  hz::DevBuf d_fmap;     // int[nz]
+  hz::DevBuf d_splane;   // int[nslot]
   std::vector<double2> kappa;                  // (nx*ny) (w/cb)^2 (1 + i beta)
   std::vector<double2> c0;                     // (nx*ny) background speed
 
@@ -211,6 +219,10 @@ void launch_factor(hz_problem* p, int prec, hipStream_t s);
 // symmetric factors.
 void factor_begin(hz_problem* p, int prec, hipStream_t s);
 void factor_end(hz_problem* p, int prec, hipStream_t s);
+// zero the planes that the plane map skips, in nrhs vectors of complex of
+// `elem` bytes (the preconditioner's output there, FGMRES.cpp:196-206 and
+// functions.cpp:6023-6027)
+void launch_zero_skipped(hz_problem* p, int nrhs, void* x, size_t elem, hipStream_t s);
 // in-place plane solves on z-transformed data (prec 0: double2, 1: float2).
 void launch_plane_solve(hz_problem* p, int prec, int nrhs, void* x, bool adjoint, hipStream_t s);
 // the same with explicit launch choices (autotuner)

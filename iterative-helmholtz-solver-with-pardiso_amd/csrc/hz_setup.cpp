@@ -4,6 +4,8 @@
 #include <cmath>
 #include <complex>
 
+#include <algorithm>
+#include <map>
 #include <mutex>
 #include <unordered_map>
 
@@ -65,7 +67,9 @@ static double z_symbol(const hz_desc& d, int64_t k) {
       return (2.0 * pi * q / d.l[2]) * (2.0 * pi * q / d.l[2]);
     }
     default: {
-      double s = std::sin(pi * (double)k / (double)nz);
+      // with q = min(k, nz - k) the symbol of k and nz - k is the same double
+      const int64_t q = k < nz - k ? k : nz - k;
+      double s = std::sin(pi * (double)q / (double)nz);
       return 4.0 * s * s / (d.h[2] * d.h[2]);
     }
   }
@@ -150,6 +154,51 @@ static void build_deltaL(hz_problem* p, const double* c, std::vector<double2>& d
   }
 }
 
+// Factor slots of the planes: one per distinct z symbol among the planes the
+// SAVE_MEM rule keeps (FGMRES.cpp:196-206: a plane whose kz^2 = lambda_k is at
+// least skip_ratio k0^2 is neither factored nor solved).
+static void build_plane_map(hz_problem* p) {
+  const hz_desc& d = p->d;
+  double k0 = d.skip_k0;
+  if (k0 <= 0.0) {
+    if (d.c_bg > 0.0) {
+      k0 = d.omega / d.c_bg;
+    } else {
+      double cmin = 0.0;
+      for (const double2& v : p->c0) {
+        const double a = std::abs(zz(v));
+        if (cmin == 0.0 || a < cmin) cmin = a;
+      }
+      k0 = d.omega / cmin;
+    }
+  }
+  const double limit = (double)d.skip_ratio * k0 * k0;
+  p->fmap.assign(p->nz, -1);
+  p->splane.clear();
+  std::map<double, int> slot_of;
+  for (int64_t k = 0; k < p->nz; ++k) {
+    const double lam = p->lamz[k];
+    if (d.skip_ratio > 0 && lam >= limit) continue;
+    auto it = slot_of.find(lam);
+    if (it != slot_of.end()) {
+      p->fmap[k] = it->second;
+    } else {
+      const int s = (int)p->splane.size();
+      slot_of[lam] = s;
+      p->splane.push_back((int)k);
+      p->fmap[k] = s;
+    }
+  }
+  p->nslot = (int)p->splane.size();
+  p->nskip = (int)std::count(p->fmap.begin(), p->fmap.end(), -1);
+  p->d_fmap.alloc(p->nz * sizeof(int));
+  HZ_HIP(hipMemcpy(p->d_fmap.ptr, p->fmap.data(), p->nz * sizeof(int), hipMemcpyHostToDevice));
+  if (p->nslot > 0) {
+    p->d_splane.alloc(p->nslot * sizeof(int));
+    HZ_HIP(hipMemcpy(p->d_splane.ptr, p->splane.data(), p->nslot * sizeof(int), hipMemcpyHostToDevice));
+  }
+}
+
 static void build_fft_plan(hz_problem* p) {
   FftPlan& f = p->fft;
   int n = (int)p->nz;
@@ -214,6 +263,7 @@ void setup_tables(hz_problem* p, const double* c) {
   HZ_HIP(hipMemcpy(p->d_kappa.ptr, p->kappa.data(), p->n2 * sizeof(double2), hipMemcpyHostToDevice));
   p->d_lamz.alloc(p->nz * sizeof(double));
   HZ_HIP(hipMemcpy(p->d_lamz.ptr, p->lamz.data(), p->nz * sizeof(double), hipMemcpyHostToDevice));
+  build_plane_map(p);
 
   // Schur coupling of y-line j to j-1: cm_y(j) * cp_y(j-1)
   std::vector<double2> sig(p->ny, make_double2(0, 0));

@@ -76,7 +76,8 @@ constexpr int CB = 4;    // columns per thread in the trailing update
 
 template <typename T>
 __global__ __launch_bounds__(FACT_THREADS) void k_factor(typename Cx<T>::t* __restrict__ G, PlaneTabs tb, int n,
-                                                         int ny, int nb, int* __restrict__ status) {
+                                                         int ny, int nb, int* __restrict__ status,
+                                                         const int* __restrict__ splane) {
   typedef typename Cx<T>::t C;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C* P = reinterpret_cast<C*>(smem_raw);     // nb * n, column-major panel
@@ -86,7 +87,7 @@ __global__ __launch_bounds__(FACT_THREADS) void k_factor(typename Cx<T>::t* __re
   int* piv = reinterpret_cast<int*>(pcol + n);  // n
   int* sh = piv + n;                         // scratch ints
 
-  const int k = blockIdx.x;
+  const int k = splane[blockIdx.x];  // the plane whose z symbol this factor slot carries
   const int tid = threadIdx.x;
   const int nt = blockDim.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -100,7 +101,7 @@ __global__ __launch_bounds__(FACT_THREADS) void k_factor(typename Cx<T>::t* __re
   __syncthreads();
 
   for (int j = 0; j < ny; ++j) {
-    C* Gj = G + ((size_t)k * ny + j) * nn;
+    C* Gj = G + ((size_t)blockIdx.x * ny + j) * nn;
     const C* Gp = Gj - nn;
     // ---- S_j = D_j - sig_j G_{j-1}
     const double2 sg = tb.sig[j];
@@ -381,16 +382,19 @@ __global__ __launch_bounds__(APPLY_THREADS) void k_apply(const typename Cx<T>::t
                                                          typename Cx<T>::t* __restrict__ X, PlaneTabs tb,
                                                          const double2* __restrict__ cmy,
                                                          const double2* __restrict__ cpy, int n, int ny,
-                                                         int64_t n2, int64_t N) {
+                                                         int64_t n2, int64_t N,
+    const int* __restrict__ fmap) {
   typedef typename Cx<T>::t C;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C* v = reinterpret_cast<C*>(smem_raw);  // n
   C* cur = v + n;                          // n
   C* part = cur + n;                       // ngroups * n
   const int k = blockIdx.x;
+  const int kf = fmap[k];  // factor slot; -1: a skipped plane (zeroed after the solve)
+  if (kf < 0) return;
   const int64_t rhs = blockIdx.y;
   C* x = X + rhs * N + (int64_t)k * n2;
-  const C* Gk = G + (size_t)k * ny * n * n;
+  const C* Gk = G + (size_t)kf * ny * n * n;
   const size_t nn = (size_t)n * n;
   const int tid = threadIdx.x;
   (void)tb;
@@ -529,17 +533,20 @@ __global__ __launch_bounds__(1024) void k_solve_fast(const typename Cx<T>::t* __
                                                      typename Cx<T>::t* __restrict__ X,
                                                      const double2* __restrict__ cmy,
                                                      const double2* __restrict__ cpy, int n, int ny, int64_t n2,
-                                                     int64_t N, int nru, int ncg, int cpg) {
+                                                     int64_t N, int nru, int ncg, int cpg,
+    const int* __restrict__ fmap) {
   typedef typename Cx<T>::t C;
   typedef typename Unit<T>::t U;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C* v = reinterpret_cast<C*>(smem_raw);  // n
   C* part = v + n;                        // ncg * n
   const int k = blockIdx.x;
+  const int kf = fmap[k];  // factor slot; -1: a skipped plane (zeroed after the solve)
+  if (kf < 0) return;
   const int64_t rhs = blockIdx.y;
   C* x = X + rhs * N + (int64_t)k * n2;
   const size_t nn = (size_t)n * n;
-  const C* Gk = G + (size_t)k * ny * nn;
+  const C* Gk = G + (size_t)kf * ny * nn;
   const int tid = threadIdx.x;
   const int g = tid / nru, ru = tid - g * nru;
   const bool worker = g < ncg;
@@ -669,7 +676,8 @@ __global__ __launch_bounds__(dma::NT) void k_solve_dma(const typename Cx<T>::t* 
                                                        typename Cx<T>::t* __restrict__ X,
                                                        const double2* __restrict__ cmy,
                                                        const double2* __restrict__ cpy, int ny, int64_t n2,
-                                                       int64_t N) {
+                                                       int64_t N,
+    const int* __restrict__ fmap) {
   typedef typename Cx<T>::t C;
   typedef typename Unit<T>::t U;
   constexpr int n = NX;
@@ -687,10 +695,12 @@ __global__ __launch_bounds__(dma::NT) void k_solve_dma(const typename Cx<T>::t* 
   __shared__ C part[NCG * n];
 
   const int k = blockIdx.x;
+  const int kf = fmap[k];  // factor slot; -1: a skipped plane (zeroed after the solve)
+  if (kf < 0) return;
   const int64_t rhs = blockIdx.y;
   C* x = X + rhs * N + (int64_t)k * n2;
   const size_t nn = (size_t)n * n;
-  const C* Gk = G + (size_t)k * ny * nn;
+  const C* Gk = G + (size_t)kf * ny * nn;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
   const int cg = tid / nru, ru = tid - cg * nru;
@@ -822,11 +832,11 @@ __global__ __launch_bounds__(dma::NT) void k_solve_dma(const typename Cx<T>::t* 
 
 template <typename T, int RPL, int NCG, int TCOL, int NX>
 bool try_launch_dma(int n, int ny, dim3 grid, hipStream_t s, const void* G, void* x, const double2* cmy,
-                    const double2* cpy, int64_t n2, int64_t N) {
+                    const double2* cpy, int64_t n2, int64_t N, const int* fmap) {
   if (n != NX) return false;
   typedef typename Cx<T>::t C;
   hipLaunchKernelGGL((k_solve_dma<T, RPL, NCG, TCOL, NX>), grid, dim3(dma::NT), 0, s, (const C*)G, (C*)x, cmy, cpy,
-                     ny, n2, N);
+                     ny, n2, N, fmap);
   HZ_HIP(hipGetLastError());
   return true;
 }
@@ -869,7 +879,8 @@ template <typename T, int RPL, int NCG, int TCOL, int NX, int NR>
 __global__ __launch_bounds__(dma::NT) void k_solve_mr(const typename Cx<T>::t* __restrict__ G,
                                                       typename Cx<T>::t* __restrict__ X,
                                                       const double2* __restrict__ cmy,
-                                                      const double2* __restrict__ cpy, int ny, int64_t n2, int64_t N) {
+                                                      const double2* __restrict__ cpy, int ny, int64_t n2, int64_t N,
+    const int* __restrict__ fmap) {
   typedef MrShape<T, RPL, NCG, TCOL, NX, NR> S;
   typedef typename Cx<T>::t C;
   typedef typename Unit<T>::t U;
@@ -880,10 +891,12 @@ __global__ __launch_bounds__(dma::NT) void k_solve_mr(const typename Cx<T>::t* _
   __shared__ C v[n * NR];
 
   const int k = blockIdx.x;
+  const int kf = fmap[k];  // factor slot; -1: a skipped plane (zeroed after the solve)
+  if (kf < 0) return;
   const int64_t rhs0 = (int64_t)blockIdx.y * NR;
   C* xk = X + rhs0 * N + (int64_t)k * n2;   // right-hand side q of this plane: xk + q * N
   const size_t nn = (size_t)n * n;
-  const C* Gk = G + (size_t)k * ny * nn;
+  const C* Gk = G + (size_t)kf * ny * nn;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
   const int cg = tid & (NCG - 1), ru = tid / NCG;
@@ -1066,7 +1079,8 @@ __global__ __launch_bounds__(dma::NT) void k_solve_mfma8(const float2* __restric
                                                          const double2* __restrict__ cmy,
                                                          const double2* __restrict__ cpy,
                                                          const double2* __restrict__ symd, int ny, int64_t n2,
-                                                         int64_t N) {
+                                                         int64_t N,
+    const int* __restrict__ fmap) {
   typedef MfShape<NX, TCOL> S;
   typedef float4 U;
   constexpr int n = S::n, RT = S::RT, KS = S::KS, tps = S::tps, units = S::units, R = S::R, TPW = S::TPW;
@@ -1077,10 +1091,12 @@ __global__ __launch_bounds__(dma::NT) void k_solve_mfma8(const float2* __restric
   __shared__ float2 sdx[ADJ ? n : 1], sdxi[ADJ ? n : 1];  // dx, 1/dx (adjoint only)
 
   const int k = blockIdx.x;
+  const int kf = fmap[k];  // factor slot; -1: a skipped plane (zeroed after the solve)
+  if (kf < 0) return;
   const int64_t rhs0 = (int64_t)blockIdx.y * NR;
   float2* xk = X + rhs0 * N + (int64_t)k * n2;
   const size_t nn = (size_t)n * n;
-  const float2* Gk = G + (size_t)k * ny * nn;
+  const float2* Gk = G + (size_t)kf * ny * nn;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
   const int li = lane & 15, lg = lane >> 4;  // operand row/column index, k / row-group index
@@ -1271,27 +1287,27 @@ __global__ __launch_bounds__(dma::NT) void k_solve_mfma8(const float2* __restric
 
 template <int NX, int TCOL>
 bool try_launch_mfma8(int n, int ny, int nz, int groups, hipStream_t s, const void* G, void* x, const double2* cmy,
-                      const double2* cpy, const double2* symd, bool adjoint, int64_t n2, int64_t N) {
+                      const double2* cpy, const double2* symd, bool adjoint, int64_t n2, int64_t N, const int* fmap) {
   if (n != NX) return false;
   const dim3 grid((unsigned)nz, (unsigned)groups);
   if (adjoint)
     hipLaunchKernelGGL((k_solve_mfma8<NX, TCOL, true>), grid, dim3(dma::NT), 0, s, (const float2*)G, (float2*)x, cmy,
-                       cpy, symd, ny, n2, N);
+                       cpy, symd, ny, n2, N, fmap);
   else
     hipLaunchKernelGGL((k_solve_mfma8<NX, TCOL, false>), grid, dim3(dma::NT), 0, s, (const float2*)G, (float2*)x,
-                       cmy, cpy, symd, ny, n2, N);
+                       cmy, cpy, symd, ny, n2, N, fmap);
   HZ_HIP(hipGetLastError());
   return true;
 }
 
 bool launch_mfma8_any(int n, int ny, int nz, int groups, hipStream_t s, const void* G, void* x, const double2* cmy,
-                      const double2* cpy, const double2* symd, bool adjoint, int64_t n2, int64_t N) {
-  return try_launch_mfma8<200, 20>(n, ny, nz, groups, s, G, x, cmy, cpy, symd, adjoint, n2, N) ||
-         try_launch_mfma8<192, 16>(n, ny, nz, groups, s, G, x, cmy, cpy, symd, adjoint, n2, N) ||
-         try_launch_mfma8<160, 20>(n, ny, nz, groups, s, G, x, cmy, cpy, symd, adjoint, n2, N) ||
-         try_launch_mfma8<128, 32>(n, ny, nz, groups, s, G, x, cmy, cpy, symd, adjoint, n2, N) ||
-         try_launch_mfma8<96, 32>(n, ny, nz, groups, s, G, x, cmy, cpy, symd, adjoint, n2, N) ||
-         try_launch_mfma8<64, 32>(n, ny, nz, groups, s, G, x, cmy, cpy, symd, adjoint, n2, N);
+                      const double2* cpy, const double2* symd, bool adjoint, int64_t n2, int64_t N, const int* fmap) {
+  return try_launch_mfma8<200, 20>(n, ny, nz, groups, s, G, x, cmy, cpy, symd, adjoint, n2, N, fmap) ||
+         try_launch_mfma8<192, 16>(n, ny, nz, groups, s, G, x, cmy, cpy, symd, adjoint, n2, N, fmap) ||
+         try_launch_mfma8<160, 20>(n, ny, nz, groups, s, G, x, cmy, cpy, symd, adjoint, n2, N, fmap) ||
+         try_launch_mfma8<128, 32>(n, ny, nz, groups, s, G, x, cmy, cpy, symd, adjoint, n2, N, fmap) ||
+         try_launch_mfma8<96, 32>(n, ny, nz, groups, s, G, x, cmy, cpy, symd, adjoint, n2, N, fmap) ||
+         try_launch_mfma8<64, 32>(n, ny, nz, groups, s, G, x, cmy, cpy, symd, adjoint, n2, N, fmap);
 }
 
 // ------------------------------------------ symmetric packed plane solve
@@ -1449,7 +1465,8 @@ __global__ __launch_bounds__(dma::NT) void k_solve_sym2(const float2* __restrict
                                                         const double2* __restrict__ cmy,
                                                         const double2* __restrict__ cpy,
                                                         const double2* __restrict__ symd, int ny, int64_t n2,
-                                                        int64_t N) {
+                                                        int64_t N,
+    const int* __restrict__ fmap) {
   typedef float T;
   typedef float2 C;
   typedef float4 U;
@@ -1472,9 +1489,11 @@ __global__ __launch_bounds__(dma::NT) void k_solve_sym2(const float2* __restrict
   const double2* pscal = reinterpret_cast<const double2*>(&pbuf[128]);
 
   const int k = blockIdx.x;
+  const int kf = fmap[k];  // factor slot; -1: a skipped plane (zeroed after the solve)
+  if (kf < 0) return;
   const int64_t rhs = blockIdx.y;
   C* x = X + rhs * N + (int64_t)k * n2;
-  const C* Pk = P + (size_t)k * ny * BS;
+  const C* Pk = P + (size_t)kf * ny * BS;
   const double2* dxi = symd + n + ny;
   const double2* dyi = symd + 2 * n + ny;
   const int tid = threadIdx.x;
@@ -1784,7 +1803,8 @@ __global__ __launch_bounds__(dma::NT) void k_solve_sym2d(const double2* __restri
                                                          const double2* __restrict__ cmy,
                                                          const double2* __restrict__ cpy,
                                                          const double2* __restrict__ symd, int ny, int64_t n2,
-                                                         int64_t N) {
+                                                         int64_t N,
+    const int* __restrict__ fmap) {
   typedef double T;
   typedef double2 C;
   typedef double2 U;
@@ -1811,9 +1831,11 @@ __global__ __launch_bounds__(dma::NT) void k_solve_sym2d(const double2* __restri
   const double2* pscal = &pbuf[LW * 64];
 
   const int k = blockIdx.x;
+  const int kf = fmap[k];  // factor slot; -1: a skipped plane (zeroed after the solve)
+  if (kf < 0) return;
   const int64_t rhs = blockIdx.y;
   C* x = X + rhs * N + (int64_t)k * n2;
-  const C* Pk = P + (size_t)k * ny * BS;
+  const C* Pk = P + (size_t)kf * ny * BS;
   const double2* dxi = symd + n + ny;
   const double2* dyi = symd + 2 * n + ny;
   const int tid = threadIdx.x;
@@ -2000,7 +2022,7 @@ bool try_solve_sym2d(hz_problem* p, int prec, int nrhs, void* x, hipStream_t s) 
   if (p->nx != NX || prec != HZ_PREC_FP64) return false;
   hipLaunchKernelGGL((k_solve_sym2d<NX>), dim3((unsigned)p->nz, (unsigned)nrhs), dim3(dma::NT), 0, s,
                      p->fac[prec].Gsym.as<double2>(), (double2*)x, p->tab(1, 0), p->tab(1, 1),
-                     p->d_symd.as<double2>(), (int)p->ny, p->n2, p->N);
+                     p->d_symd.as<double2>(), (int)p->ny, p->n2, p->N, p->d_fmap.as<int>());
   HZ_HIP(hipGetLastError());
   return true;
 }
@@ -2020,7 +2042,8 @@ __global__ __launch_bounds__(dma::NT) void k_solve_sym_mfma8(const float2* __res
                                                              const double2* __restrict__ cmy,
                                                              const double2* __restrict__ cpy,
                                                              const double2* __restrict__ symd, int ny, int64_t n2,
-                                                             int64_t N) {
+                                                             int64_t N,
+    const int* __restrict__ fmap) {
   typedef float4 U;
   constexpr int TB = 8, NR = 8;
   typedef SymLayout<NX, TB> L;
@@ -2042,9 +2065,11 @@ __global__ __launch_bounds__(dma::NT) void k_solve_sym_mfma8(const float2* __res
   __shared__ float2 sdxi[n];
 
   const int k = blockIdx.x;
+  const int kf = fmap[k];  // factor slot; -1: a skipped plane (zeroed after the solve)
+  if (kf < 0) return;
   const int64_t rhs0 = (int64_t)blockIdx.y * NR;
   float2* xk = X + rhs0 * N + (int64_t)k * n2;
-  const float2* Pk = P + (size_t)k * ny * BS;
+  const float2* Pk = P + (size_t)kf * ny * BS;
   const double2* dxi = symd + n + ny;
   const double2* dyi = symd + 2 * n + ny;
   const int tid = threadIdx.x;
@@ -2271,7 +2296,7 @@ bool try_solve_sym_mfma8(hz_problem* p, int groups, void* x, hipStream_t s) {
   if (p->nx != NX) return false;
   hipLaunchKernelGGL((k_solve_sym_mfma8<NX>), dim3((unsigned)p->nz, (unsigned)groups), dim3(dma::NT), 0, s,
                      p->fac[HZ_PREC_C64].Gsym.as<float2>(), (float2*)x, p->tab(1, 0), p->tab(1, 1),
-                     p->d_symd.as<double2>(), (int)p->ny, p->n2, p->N);
+                     p->d_symd.as<double2>(), (int)p->ny, p->n2, p->N, p->d_fmap.as<int>());
   HZ_HIP(hipGetLastError());
   return true;
 }
@@ -2287,14 +2312,15 @@ bool try_pack_sym(hz_problem* p, int prec, hipStream_t s) {
   typedef typename Cx<T>::t C;
   typedef SymLayout<NX, TB> L;
   Factors& f = p->fac[prec];
-  const size_t bytes = (size_t)p->nz * p->ny * L::BS * sizeof(C);
+  const size_t bytes = (size_t)p->nslot * p->ny * L::BS * sizeof(C);
   if (f.Gsym.bytes != bytes) {
     f.Gsym.release();
     f.Gsym.alloc(bytes);
   }
   const double2* dx = p->d_symd.as<double2>();
   const double2* dy = dx + NX;
-  hipLaunchKernelGGL((k_pack_sym<C, NX, TB>), dim3(L::TPS, (unsigned)(p->nz * p->ny)), dim3(256), 0, s, f.G.as<C>(),
+  if (p->nslot == 0) return true;
+  hipLaunchKernelGGL((k_pack_sym<C, NX, TB>), dim3(L::TPS, (unsigned)(p->nslot * p->ny)), dim3(256), 0, s, f.G.as<C>(),
                      f.Gsym.as<C>(), dx, dy, (int)p->ny);
   HZ_HIP(hipGetLastError());
   return true;
@@ -2307,10 +2333,10 @@ bool try_solve_sym2(hz_problem* p, int prec, int nrhs, void* x, bool adjoint, hi
   const float2* P = p->fac[prec].Gsym.as<float2>();
   if (adjoint)
     hipLaunchKernelGGL((k_solve_sym2<NX, true>), grid, dim3(dma::NT), 0, s, P, (float2*)x, p->tab(1, 0), p->tab(1, 1),
-                       p->d_symd.as<double2>(), (int)p->ny, p->n2, p->N);
+                       p->d_symd.as<double2>(), (int)p->ny, p->n2, p->N, p->d_fmap.as<int>());
   else
     hipLaunchKernelGGL((k_solve_sym2<NX, false>), grid, dim3(dma::NT), 0, s, P, (float2*)x, p->tab(1, 0),
-                       p->tab(1, 1), p->d_symd.as<double2>(), (int)p->ny, p->n2, p->N);
+                       p->tab(1, 1), p->d_symd.as<double2>(), (int)p->ny, p->n2, p->N, p->d_fmap.as<int>());
   HZ_HIP(hipGetLastError());
   return true;
 }
@@ -2346,10 +2372,10 @@ bool solve_sym_any(hz_problem* p, int prec, int nrhs, void* x, bool adjoint, hip
 // Launch of one right-hand-side group size NR over nrhs_g * NR vectors.
 template <typename T, int RPL, int NCG, int TCOL, int NX, int NR>
 void launch_mr(int ny, int nz, int groups, hipStream_t s, const void* G, void* x, const double2* cmy,
-               const double2* cpy, int64_t n2, int64_t N) {
+               const double2* cpy, int64_t n2, int64_t N, const int* fmap) {
   typedef typename Cx<T>::t C;
   hipLaunchKernelGGL((k_solve_mr<T, RPL, NCG, TCOL, NX, NR>), dim3((unsigned)nz, (unsigned)groups), dim3(dma::NT), 0,
-                     s, (const C*)G, (C*)x, cmy, cpy, ny, n2, N);
+                     s, (const C*)G, (C*)x, cmy, cpy, ny, n2, N, fmap);
   HZ_HIP(hipGetLastError());
 }
 
@@ -2358,17 +2384,17 @@ void launch_mr(int ny, int nz, int groups, hipStream_t s, const void* G, void* x
 // instantiation.
 template <typename T, int RPL, int NCG, int TCOL, int NX, int NRMAX>
 bool try_launch_mr(int n, int ny, int nz, int nrhs, int cap, hipStream_t s, const void* G, void* x,
-                   const double2* cmy, const double2* cpy, int64_t n2, int64_t N) {
+                   const double2* cmy, const double2* cpy, int64_t n2, int64_t N, const int* fmap) {
   if (n != NX) return false;
   typedef typename Cx<T>::t C;
   C* xp = (C*)x;
   int done = 0;
   auto groups = [&](int g, int count) {
     switch (g) {
-      case 8: launch_mr<T, RPL, NCG, TCOL, NX, (NRMAX >= 8 ? 8 : 1)>(ny, nz, count, s, G, xp + done * N, cmy, cpy, n2, N); break;
-      case 4: launch_mr<T, RPL, NCG, TCOL, NX, (NRMAX >= 4 ? 4 : 1)>(ny, nz, count, s, G, xp + done * N, cmy, cpy, n2, N); break;
-      case 2: launch_mr<T, RPL, NCG, TCOL, NX, 2>(ny, nz, count, s, G, xp + done * N, cmy, cpy, n2, N); break;
-      default: launch_mr<T, RPL, NCG, TCOL, NX, 1>(ny, nz, count, s, G, xp + done * N, cmy, cpy, n2, N); break;
+      case 8: launch_mr<T, RPL, NCG, TCOL, NX, (NRMAX >= 8 ? 8 : 1)>(ny, nz, count, s, G, xp + done * N, cmy, cpy, n2, N, fmap); break;
+      case 4: launch_mr<T, RPL, NCG, TCOL, NX, (NRMAX >= 4 ? 4 : 1)>(ny, nz, count, s, G, xp + done * N, cmy, cpy, n2, N, fmap); break;
+      case 2: launch_mr<T, RPL, NCG, TCOL, NX, 2>(ny, nz, count, s, G, xp + done * N, cmy, cpy, n2, N, fmap); break;
+      default: launch_mr<T, RPL, NCG, TCOL, NX, 1>(ny, nz, count, s, G, xp + done * N, cmy, cpy, n2, N, fmap); break;
     }
     done += g * count;
   };
@@ -2412,10 +2438,10 @@ SolveKernel solve_kernel_choice() {
   X(HZ_PREC_FP64, double, 1, 8, 32, 64, 4)
 
 bool launch_mr_any(int prec, int n, int ny, int nz, int nrhs, int cap, hipStream_t s, const void* G, void* x,
-                   const double2* cmy, const double2* cpy, int64_t n2, int64_t N) {
+                   const double2* cmy, const double2* cpy, int64_t n2, int64_t N, const int* fmap) {
   if (reinterpret_cast<uintptr_t>(x) % 16 != 0) return false;
 #define HZ_MR_TRY(P, T, RPL, NCG, TCOL, NX, NRMAX)                                                        \
-  if (prec == P && try_launch_mr<T, RPL, NCG, TCOL, NX, NRMAX>(n, ny, nz, nrhs, cap, s, G, x, cmy, cpy, n2, N)) \
+  if (prec == P && try_launch_mr<T, RPL, NCG, TCOL, NX, NRMAX>(n, ny, nz, nrhs, cap, s, G, x, cmy, cpy, n2, N, fmap)) \
     return true;
   HZ_MR_WIDTHS(HZ_MR_TRY)
 #undef HZ_MR_TRY
@@ -2474,7 +2500,7 @@ void factor_begin(hz_problem* p, int prec, hipStream_t s) {
   f.ready = false;
   f.sym_ready = false;
   f.elem = prec == HZ_PREC_FP64 ? sizeof(double2) : sizeof(float2);
-  const size_t bytes = (size_t)p->nz * ny * (size_t)n * n * f.elem;
+  const size_t bytes = (size_t)p->nslot * ny * (size_t)n * n * f.elem;
   if (f.G.bytes != bytes) {
     f.G.release();
     f.G.alloc(bytes);
@@ -2493,15 +2519,17 @@ void factor_begin(hz_problem* p, int prec, hipStream_t s) {
     const size_t lds = factor_lds<double>(n, nb);
     HZ_CHECK(lds <= 160 * 1024, HZ_ERR_ARG, "nx=%d too large for the LDS panel", n);
     set_max_lds((const void*)&k_factor<double>, lds);
-    hipLaunchKernelGGL(k_factor<double>, dim3((unsigned)p->nz), dim3(FACT_THREADS), lds, s, f.G.as<double2>(), tb,
-                       n, ny, nb, st);
+    if (p->nslot > 0)
+      hipLaunchKernelGGL(k_factor<double>, dim3((unsigned)p->nslot), dim3(FACT_THREADS), lds, s, f.G.as<double2>(),
+                         tb, n, ny, nb, st, p->d_splane.as<int>());
   } else {
     while (nb > 4 && factor_lds<float>(n, nb) > 160 * 1024) nb /= 2;
     const size_t lds = factor_lds<float>(n, nb);
     HZ_CHECK(lds <= 160 * 1024, HZ_ERR_ARG, "nx=%d too large for the LDS panel", n);
     set_max_lds((const void*)&k_factor<float>, lds);
-    hipLaunchKernelGGL(k_factor<float>, dim3((unsigned)p->nz), dim3(FACT_THREADS), lds, s, f.G.as<float2>(), tb, n,
-                       ny, nb, st);
+    if (p->nslot > 0)
+      hipLaunchKernelGGL(k_factor<float>, dim3((unsigned)p->nslot), dim3(FACT_THREADS), lds, s, f.G.as<float2>(), tb,
+                         n, ny, nb, st, p->d_splane.as<int>());
   }
   HZ_HIP(hipGetLastError());
   HZ_HIP(hipMemcpyAsync(p->pin_misc.ptr, st, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -2529,8 +2557,39 @@ void launch_plane_solve(hz_problem* p, int prec, int nrhs, void* x, bool adjoint
   launch_plane_solve_with(p, prec, nrhs, x, adjoint, p->tuning(prec, nrhs), s);
 }
 
+namespace {
+__global__ void k_zero_skipped(unsigned char* __restrict__ x, const int* __restrict__ fmap, int64_t n2, int64_t N,
+                               int elem) {
+  const int k = blockIdx.x;
+  if (fmap[k] >= 0) return;
+  const int64_t q = blockIdx.y;
+  // elem is 8 or 16 bytes: clear in 8-byte words
+  unsigned long long* w = reinterpret_cast<unsigned long long*>(x + ((size_t)q * N + (size_t)k * n2) * elem);
+  const int64_t nw = n2 * (elem / 8);
+  for (int64_t i = threadIdx.x; i < nw; i += blockDim.x) w[i] = 0ull;
+}
+
+// the tuned kernel family for every plane with a factor slot
+void plane_solve_kernels(hz_problem* p, int prec, int nrhs, void* x, bool adjoint, const Tuning& t, hipStream_t s);
+}  // namespace
+
+void launch_zero_skipped(hz_problem* p, int nrhs, void* x, size_t elem, hipStream_t s) {
+  if (p->nskip == 0) return;
+  HZ_CHECK(elem == 8 || elem == 16, HZ_ERR_ARG, "element of %zu bytes", elem);
+  hipLaunchKernelGGL(k_zero_skipped, dim3((unsigned)p->nz, (unsigned)nrhs), dim3(256), 0, s, (unsigned char*)x,
+                     p->d_fmap.as<int>(), p->n2, p->N, (int)elem);
+  HZ_HIP(hipGetLastError());
+}
+
 void launch_plane_solve_with(hz_problem* p, int prec, int nrhs, void* x, bool adjoint, const Tuning& t,
                              hipStream_t s) {
+  HZ_CHECK(p->fac[prec].ready, HZ_ERR_STATE, "preconditioner of precision %d not factored", prec);
+  if (p->nslot > 0) plane_solve_kernels(p, prec, nrhs, x, adjoint, t, s);
+  launch_zero_skipped(p, nrhs, x, prec == HZ_PREC_FP64 ? sizeof(double2) : sizeof(float2), s);
+}
+
+namespace {
+void plane_solve_kernels(hz_problem* p, int prec, int nrhs, void* x, bool adjoint, const Tuning& t, hipStream_t s) {
   Factors& f = p->fac[prec];
   HZ_CHECK(f.ready, HZ_ERR_STATE, "preconditioner of precision %d not factored", prec);
   const int n = (int)p->nx, ny = (int)p->ny;
@@ -2548,7 +2607,8 @@ void launch_plane_solve_with(hz_problem* p, int prec, int nrhs, void* x, bool ad
       (sk == SK_DEFAULT || sk == SK_SYM || sk == SK_MFMA)) {
     if (sk != SK_SYM && nrhs >= 8 && mfma_solve_available(prec, n)) {
       const int g = nrhs / 8;
-      launch_mfma8_any(n, ny, (int)p->nz, g, s, f.G.ptr, x, cmy, cpy, p->d_symd.as<double2>(), true, p->n2, p->N);
+      launch_mfma8_any(n, ny, (int)p->nz, g, s, f.G.ptr, x, cmy, cpy, p->d_symd.as<double2>(), true, p->n2, p->N,
+                       p->d_fmap.as<int>());
       nrhs -= 8 * g;
       if (nrhs == 0) return;
       x = reinterpret_cast<float2*>(x) + (size_t)8 * g * p->N;
@@ -2582,7 +2642,8 @@ void launch_plane_solve_with(hz_problem* p, int prec, int nrhs, void* x, bool ad
     // transposed tiles' idle rows: slower at 8 right-hand sides)
     const int g = nrhs / 8;
     if (!(sk == SK_MFMA_SYM && f.sym_ready && solve_sym_mfma8_any(p, g, x, s)))
-      launch_mfma8_any(n, ny, (int)p->nz, g, s, f.G.ptr, x, cmy, cpy, p->d_symd.as<double2>(), false, p->n2, p->N);
+      launch_mfma8_any(n, ny, (int)p->nz, g, s, f.G.ptr, x, cmy, cpy, p->d_symd.as<double2>(), false, p->n2, p->N,
+                       p->d_fmap.as<int>());
     const int rest = nrhs - 8 * g;
     if (rest == 0) return;
     x = reinterpret_cast<float2*>(x) + (size_t)8 * g * p->N;
@@ -2592,16 +2653,16 @@ void launch_plane_solve_with(hz_problem* p, int prec, int nrhs, void* x, bool ad
   }
   if (!adjoint && ny >= 1 && (sk == SK_DEFAULT || sk == SK_MR || sk == SK_MFMA) &&
       launch_mr_any(prec, n, ny, (int)p->nz, nrhs, (sk == SK_MFMA || sk == SK_MFMA_SYM) ? 4 : t.group_cap, s, f.G.ptr, x, cmy, cpy, p->n2,
-                    p->N))
+                    p->N, p->d_fmap.as<int>()))
     return;
   if (!adjoint && ny >= 1 && sk == SK_DMA && reinterpret_cast<uintptr_t>(x) % 16 == 0) {
     // single-RHS LDS-DMA ring for the configurations it is instantiated for
     const void* Gp = f.G.ptr;
     if (prec == HZ_PREC_C64 &&
-        (try_launch_dma<float, 2, 5, 20, 200>(n, ny, grid, s, Gp, x, cmy, cpy, p->n2, p->N)))
+        (try_launch_dma<float, 2, 5, 20, 200>(n, ny, grid, s, Gp, x, cmy, cpy, p->n2, p->N, p->d_fmap.as<int>())))
       return;
     if (prec == HZ_PREC_FP64 &&
-        (try_launch_dma<double, 1, 2, 10, 200>(n, ny, grid, s, Gp, x, cmy, cpy, p->n2, p->N)))
+        (try_launch_dma<double, 1, 2, 10, 200>(n, ny, grid, s, Gp, x, cmy, cpy, p->n2, p->N, p->d_fmap.as<int>())))
       return;
   }
   if (!adjoint && ny >= 1) {
@@ -2612,7 +2673,7 @@ void launch_plane_solve_with(hz_problem* p, int prec, int nrhs, void* x, bool ad
       const size_t lds = sizeof(float2) * (size_t)n * (1 + ncg);
       set_max_lds((const void*)&k_solve_fast<float, 2, CH>, lds);
       hipLaunchKernelGGL((k_solve_fast<float, 2, CH>), grid, dim3(1024), lds, s, f.G.as<float2>(), (float2*)x, cmy, cpy,
-                         n, ny, p->n2, p->N, nru, ncg, cpg);
+                         n, ny, p->n2, p->N, nru, ncg, cpg, p->d_fmap.as<int>());
       HZ_HIP(hipGetLastError());
       return;
     }
@@ -2621,7 +2682,7 @@ void launch_plane_solve_with(hz_problem* p, int prec, int nrhs, void* x, bool ad
       const size_t lds = sizeof(double2) * (size_t)n * (1 + ncg);
       set_max_lds((const void*)&k_solve_fast<double, 1, CH>, lds);
       hipLaunchKernelGGL((k_solve_fast<double, 1, CH>), grid, dim3(1024), lds, s, f.G.as<double2>(), (double2*)x, cmy,
-                         cpy, n, ny, p->n2, p->N, nru, ncg, cpg);
+                         cpy, n, ny, p->n2, p->N, nru, ncg, cpg, p->d_fmap.as<int>());
       HZ_HIP(hipGetLastError());
       return;
     }
@@ -2632,25 +2693,26 @@ void launch_plane_solve_with(hz_problem* p, int prec, int nrhs, void* x, bool ad
     if (adjoint) {
       set_max_lds((const void*)&k_apply<double, true>, lds);
       hipLaunchKernelGGL((k_apply<double, true>), grid, dim3(APPLY_THREADS), lds, s, f.G.as<double2>(), (double2*)x, tb,
-                         cmy, cpy, n, ny, p->n2, p->N);
+                         cmy, cpy, n, ny, p->n2, p->N, p->d_fmap.as<int>());
     } else {
       set_max_lds((const void*)&k_apply<double, false>, lds);
       hipLaunchKernelGGL((k_apply<double, false>), grid, dim3(APPLY_THREADS), lds, s, f.G.as<double2>(), (double2*)x,
-                         tb, cmy, cpy, n, ny, p->n2, p->N);
+                         tb, cmy, cpy, n, ny, p->n2, p->N, p->d_fmap.as<int>());
     }
   } else {
     const size_t lds = sizeof(float2) * (size_t)n * (2 + ng);
     if (adjoint) {
       set_max_lds((const void*)&k_apply<float, true>, lds);
       hipLaunchKernelGGL((k_apply<float, true>), grid, dim3(APPLY_THREADS), lds, s, f.G.as<float2>(), (float2*)x, tb,
-                         cmy, cpy, n, ny, p->n2, p->N);
+                         cmy, cpy, n, ny, p->n2, p->N, p->d_fmap.as<int>());
     } else {
       set_max_lds((const void*)&k_apply<float, false>, lds);
       hipLaunchKernelGGL((k_apply<float, false>), grid, dim3(APPLY_THREADS), lds, s, f.G.as<float2>(), (float2*)x, tb,
-                         cmy, cpy, n, ny, p->n2, p->N);
+                         cmy, cpy, n, ny, p->n2, p->N, p->d_fmap.as<int>());
     }
   }
   HZ_HIP(hipGetLastError());
 }
+}  // namespace
 
 }  // namespace hz

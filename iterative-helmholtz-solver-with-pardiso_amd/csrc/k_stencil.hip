@@ -172,6 +172,8 @@ void launch_plane_matvec(hz_problem* p, int nrhs, const double2* x, double2* y, 
     hipLaunchKernelGGL((k_plane5<false>), dim3((unsigned)blocks), dim3(256), 0, s, x, y, tx, ty,
                        p->d_kappa.as<double2>(), p->d_lamz.as<double>(), (int)p->nx, (int)p->ny, (int)p->nz, p->N, total);
   HZ_HIP(hipGetLastError());
+  // planes that the SAVE_MEM rule skips are zero in L0 too (functions.cpp:6286-6295)
+  launch_zero_skipped(p, nrhs, y, sizeof(double2), s);
 }
 
 }  // namespace hz

@@ -43,6 +43,8 @@ class HzDesc:
     kz_symbol: int = KZ_FD
     c0_rule: int = C0_HARMONIC
     c_bg: float = 0.0
+    skip_ratio: int = 0      # SAVE_MEM plane skipping (FGMRES.cpp:196-206): 0 = off
+    skip_k0: float = 0.0     # wavenumber of the skip test (0: omega / c_bg or over the slowest c0)
     # extra host-side facts (not part of the ABI struct)
     meta: dict = field(default_factory=dict)
 

@@ -43,8 +43,9 @@ typedef struct {
   int32_t mode;       // 0 = REF (spectral z + delta-L), 1 = S7 (7-point PML stencil)
   int32_t kz_symbol;  // 0 = reference kz map, 1 = corrected spectral, 2 = periodic 2nd difference
   int32_t c0_rule;    // 0 = reference running sum, 1 = per-column harmonic mean
-  int32_t reserved;
+  int32_t skip_ratio; // > 0: SAVE_MEM plane skipping (FGMRES.cpp:196-206)
   double c_bg;        // > 0: constant background speed for kappa (reference HOMO)
+  double skip_k0;     // wavenumber of the skip test (0: omega / c_bg, or omega / min c0)
 } or_desc;
 
 typedef struct {
@@ -94,6 +95,7 @@ struct OrProblem {
   std::vector<double> lamz;               // z symbol per plane
   std::vector<cd> kappa;                  // (nx*ny) background term without the z symbol
   std::vector<cd> dL;                     // delta-L (N), REF mode
+  std::vector<char> solve;                // per plane: 1 = factored and solved, 0 = skipped (SAVE_MEM)
   // plane factors (banded LU)
   bool factored = false;
   int64_t kl = 0, ku = 0, ldab = 0;
@@ -132,9 +134,34 @@ static double z_symbol(const or_desc& d, int64_t k) {
     double kk = (double)std::min<int64_t>(k, nz - k);
     return 4.0 * OR_PI * OR_PI * kk * kk / (d.l[2] * d.l[2]);
   } else {
-    double s = std::sin(OR_PI * (double)k / (double)nz);
+    double q = (double)std::min<int64_t>(k, nz - k);
+    double s = std::sin(OR_PI * q / (double)nz);
     return 4.0 * s * s / (d.h[2] * d.h[2]);
   }
+}
+
+// SAVE_MEM (FGMRES.cpp:196-206): plane k is factored and solved only while its
+// kz^2 = lambda_k is below ratio * k2, k2 = (omega / c_z)^2 (here skip_k0^2, or
+// (omega / c_bg)^2, or omega^2 over the smallest |c0|^2); the preconditioner is
+// zero on the other planes (functions.cpp:6023-6027), and so is L0
+// (functions.cpp:6286-6295).
+static void plane_flags(OrProblem& p) {
+  const or_desc& d = p.d;
+  double k0 = d.skip_k0;
+  if (k0 <= 0.0) {
+    if (d.c_bg > 0.0) {
+      k0 = d.omega / d.c_bg;
+    } else {
+      double cmin = 0.0;
+      for (const cd& v : p.c0)
+        if (cmin == 0.0 || std::abs(v) < cmin) cmin = std::abs(v);
+      k0 = d.omega / cmin;
+    }
+  }
+  p.solve.assign(p.nz, 1);
+  if (d.skip_ratio > 0)
+    for (int64_t k = 0; k < p.nz; ++k)
+      if (p.lamz[k] >= (double)d.skip_ratio * k0 * k0) p.solve[k] = 0;
 }
 
 // Background speed c0(i,j).
@@ -398,7 +425,8 @@ static void ensure_factored(OrProblem& p) {
   p.ab.assign((size_t)(p.nz * p.ldab * p.n2), cd(0, 0));
   p.ipiv.assign((size_t)(p.nz * p.n2), 0);
 #pragma omp parallel for schedule(dynamic)
-  for (int64_t k = 0; k < p.nz; ++k) plane_factor(p, k);
+  for (int64_t k = 0; k < p.nz; ++k)
+    if (p.solve[k]) plane_factor(p, k);
   p.factored = true;
 }
 
@@ -410,7 +438,10 @@ static void precond_apply(OrProblem& p, const cd* f, cd* x, bool adjoint) {
   std::vector<cd> t(p.N);
   fft_z(p, f, t.data(), -1);
 #pragma omp parallel for schedule(dynamic)
-  for (int64_t k = 0; k < p.nz; ++k) plane_solve(p, k, &t[k * p.n2], adjoint);
+  for (int64_t k = 0; k < p.nz; ++k) {
+    if (p.solve[k]) plane_solve(p, k, &t[k * p.n2], adjoint);
+    else std::fill(t.begin() + k * p.n2, t.begin() + (k + 1) * p.n2, cd(0.0, 0.0));
+  }
   fft_z(p, t.data(), x, +1);
 }
 
@@ -419,7 +450,8 @@ static void L0_apply(OrProblem& p, const cd* u, cd* y) {
   std::vector<cd> t(p.N), s(p.N);
   fft_z(p, u, t.data(), -1);
 #pragma omp parallel for schedule(static)
-  for (int64_t k = 0; k < p.nz; ++k) plane_matvec(p, k, &t[k * p.n2], &s[k * p.n2]);
+  for (int64_t k = 0; k < p.nz; ++k)
+    if (p.solve[k]) plane_matvec(p, k, &t[k * p.n2], &s[k * p.n2]);
   fft_z(p, s.data(), y, +1);
 }
 
@@ -471,6 +503,7 @@ static void op_apply(OrProblem& p, const cd* u, cd* y, bool adjoint) {
     int64_t nx = p.nx, ny = p.ny;
 #pragma omp parallel for schedule(static)
     for (int64_t k = 0; k < p.nz; ++k) {
+      if (!p.solve[k]) continue;  // skipped planes are zero in L0 (and L0^H)
       const cd* x = &t[k * p.n2];
       cd* yy = &s[k * p.n2];
       for (int64_t j = 0; j < ny; ++j)
@@ -821,6 +854,7 @@ void* or_create(const or_desc* d, const double* c) {
   compute_c0(*p);
   p->lamz.resize(p->nz);
   for (int64_t k = 0; k < p->nz; ++k) p->lamz[k] = z_symbol(*d, k);
+  plane_flags(*p);
   p->kappa.resize(p->n2);
   for (int64_t ij = 0; ij < p->n2; ++ij) {
     cd kx = (d->c_bg > 0.0) ? cd(d->omega / d->c_bg, 0.0) : d->omega / p->c0[ij];
@@ -856,6 +890,10 @@ void or_tables(void* h, int ax, double* cm, double* cp, double* cdg) {
 
 void or_c0(void* h, double* out) { OrProblem* p = (OrProblem*)h; memcpy(out, p->c0.data(), p->n2 * sizeof(cd)); }
 void or_kappa(void* h, double* out) { OrProblem* p = (OrProblem*)h; memcpy(out, p->kappa.data(), p->n2 * sizeof(cd)); }
+void or_plane_solved(void* h, int* out) {
+  OrProblem* p = (OrProblem*)h;
+  for (int64_t k = 0; k < p->nz; ++k) out[k] = p->solve[k];
+}
 void or_lamz(void* h, double* out) { OrProblem* p = (OrProblem*)h; memcpy(out, p->lamz.data(), p->nz * sizeof(double)); }
 void or_deltaL(void* h, double* out) {
   OrProblem* p = (OrProblem*)h;

@@ -29,8 +29,9 @@ class OrDesc(C.Structure):
         ("mode", C.c_int32),
         ("kz_symbol", C.c_int32),
         ("c0_rule", C.c_int32),
-        ("reserved", C.c_int32),
+        ("skip_ratio", C.c_int32),
         ("c_bg", C.c_double),
+        ("skip_k0", C.c_double),
     ]
 
 
@@ -69,6 +70,7 @@ def lib():
         L.or_set_threads.argtypes = [i32]
         L.or_alpha.argtypes = [i64, i32, d, d, d, dp]
         L.or_tables.argtypes = [vp, i32, dp, dp, dp]
+        L.or_plane_solved.argtypes = [vp, C.POINTER(C.c_int)]
         for nm in ("or_c0", "or_kappa", "or_lamz", "or_deltaL", "or_factor"):
             getattr(L, nm).argtypes = [vp] + ([dp] if nm != "or_factor" else [])
         L.or_fft_line.argtypes = [i64, dp, dp, i32]
@@ -118,6 +120,8 @@ def make_desc(desc):
     o.kz_symbol = int(desc.kz_symbol)
     o.c0_rule = int(desc.c0_rule)
     o.c_bg = float(desc.c_bg)
+    o.skip_ratio = int(getattr(desc, "skip_ratio", 0))
+    o.skip_k0 = float(getattr(desc, "skip_k0", 0.0))
     return o
 
 
@@ -193,6 +197,12 @@ class Problem:
     def kappa(self):
         out = np.zeros(self.n[0] * self.n[1], np.complex128)
         lib().or_kappa(self.h, _dp(out))
+        return out
+
+    def plane_solved(self):
+        """1 for the planes the preconditioner solves, 0 for those SAVE_MEM skips."""
+        out = np.zeros(self.n[2], dtype=np.int32)
+        lib().or_plane_solved(self.h, out.ctypes.data_as(C.POINTER(C.c_int)))
         return out
 
     def lamz(self):

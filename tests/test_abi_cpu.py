@@ -30,10 +30,14 @@ def test_reference_shim_symbols():
     path = hz.abi.LIB_PATH
     import subprocess
     out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True).stdout
-    for fn in ("FGMRES", "BCGSTAB", "Solve3DSparseUsingFT_CSR", "ApplyCoeffMatrixA_CSR", "Multiply3DSparseUsingFT"):
+    for fn in ("FGMRES", "BCGSTAB", "Solve3DSparseUsingFT_CSR", "ApplyCoeffMatrixA_CSR", "Multiply3DSparseUsingFT",
+               "alpha", "beta3D", "beta2D_pml", "GenSparseMatrixOnline2DwithPMLFast", "SetSoundSpeed3D",
+               "SetSoundSpeed3DFromFile", "SetSoundSpeed2D", "GenerateDeltaL", "HeteroSoundSpeed3DExtensionToPML",
+               "HeteroSoundSpeed2DExtensionToPML"):
         assert re.search(r"_Z\d+%s" % fn, out), fn
     L = hz.abi.lib()
-    assert hasattr(L, "hz_ref_configure") and hasattr(L, "hz_ref_release")
+    for fn in ("hz_ref_configure", "hz_ref_release", "hz_ref_set_sound_speed", "hz_ref_last_status"):
+        assert hasattr(L, fn), fn
 
 
 def test_struct_layouts_match_header(tmp_path):
@@ -46,10 +50,11 @@ def test_struct_layouts_match_header(tmp_path):
 #include <stdio.h>
 #include "hz.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(hz_desc), sizeof(hz_krylov_opts),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(hz_desc), sizeof(hz_krylov_opts),
          sizeof(hz_krylov_stats), offsetof(hz_desc, c_bg), offsetof(hz_krylov_opts, hist),
          offsetof(hz_krylov_stats, t_solve_s), offsetof(hz_krylov_opts, batch), offsetof(hz_krylov_opts, autotune),
-         sizeof(hz_tune_info), offsetof(hz_tune_info, fft_ms));
+         sizeof(hz_tune_info), offsetof(hz_tune_info, fft_ms), offsetof(hz_desc, skip_ratio),
+         offsetof(hz_desc, skip_k0), offsetof(hz_krylov_opts, chunk));
   return 0;
 }
 """)
@@ -60,7 +65,7 @@ int main(void) {
     want = [ctypes.sizeof(A.HzDesc), ctypes.sizeof(A.KrylovOpts), ctypes.sizeof(A.KrylovStats),
             A.HzDesc.c_bg.offset, A.KrylovOpts.hist.offset, A.KrylovStats.t_solve_s.offset,
             A.KrylovOpts.batch.offset, A.KrylovOpts.autotune.offset, ctypes.sizeof(A.TuneInfo),
-            A.TuneInfo.fft_ms.offset]
+            A.TuneInfo.fft_ms.offset, A.HzDesc.skip_ratio.offset, A.HzDesc.skip_k0.offset, A.KrylovOpts.chunk.offset]
     assert got == want
 
 

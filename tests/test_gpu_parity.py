@@ -579,6 +579,61 @@ def test_gmres_ref_reference_sizes(hz, oracle, n_nopml, spg, homo):
     assert err < (1e-10 if st.iters == st_o.iters else 1e-7)
 
 
+@pytest.mark.parametrize("mode", ["ref", "s7"])
+def test_save_mem_and_shared_plane_factors(hz, oracle, mode):
+    """Planes with equal z symbols share one factor slot (k and nz - k) and the
+    planes that SAVE_MEM skips (kz^2 >= ratio k0^2, FGMRES.cpp:196-206) have
+    none: factor memory scales with the slot count; the FP64 and complex64
+    preconditioners and L0 (zero on skipped planes) match the oracle's (1e-10,
+    1e-4); the reference GMRES with SAVE_MEM (REF) matches the oracle's solve
+    (iterations +-1, solution 1e-10).  The S7 case is 128 wide, so the packed
+    symmetric kernels run on the shared slots."""
+    print("case save_mem %s" % mode, flush=True)
+    pr = hz.problem
+    if mode == "ref":
+        d = pr.ref_geometry(n_nopml=12, pml=4, spg=4, nz_nopml=16, length=(480.0,) * 3, nu=4.0, homo=True)
+        d = d.replace(skip_ratio=5, skip_k0=2 * np.pi * 4.0 / 1280.0)
+        c = pr.ref_sound_speed(d)
+    else:
+        n = (128, 6, 16)
+        d = pr.cube_geometry(n, (8, 2, 2), pr.omega_for_ppw(0.8, 1.0 / 16, 8), beta=0.5, pml_c=30.0)
+        c = 0.8 + 0.4 * np.random.default_rng(7).random(d.N)
+        d = d.replace(skip_ratio=2)
+    P, P0 = oracle.Problem(d, c), oracle.Problem(d.replace(skip_ratio=0), c)
+    keep = P.plane_solved().astype(bool)
+    lam = P0.lamz()
+    slots, slots0 = len(set(lam[keep])), len(set(lam))
+    print("planes %d kept %d, slots %d (without skipping %d)" % (lam.size, keep.sum(), slots, slots0))
+    assert 0 < keep.sum() < lam.size and slots0 < lam.size
+    S, S0 = hz.Solver(d, c), hz.Solver(d.replace(skip_ratio=0), c)
+    for Sx in (S, S0):
+        Sx.factor(hz.abi.PREC_FP64)
+        Sx.factor(hz.abi.PREC_C64)
+    assert abs(S.factor_bytes() / S0.factor_bytes() - slots / slots0) < 1e-12
+    f = rand_vec(d.N, 21)
+    for prec, tol in ((hz.abi.PREC_FP64, 1e-10), (hz.abi.PREC_C64, 1e-4)):
+        assert _rel(_host(S.precond_apply(_dev(f), prec)), P.precond_apply(f)) < tol
+        assert _rel(_host(S0.precond_apply(_dev(f), prec)), P0.precond_apply(f)) < tol
+    assert _rel(_host(S.L0_apply(_dev(f))), P.L0_apply(f)) < 1e-12
+    if mode != "ref":
+        # With right preconditioning the FGMRES iterate lies in the range of the
+        # preconditioner, which has no skipped z modes: SAVE_MEM belongs to the
+        # reference's w-space GMRES, where the solves are checked
+        return
+    src = pr.point_source(d)
+    u_o, st_o = P.gmres_ref(src, m=10, rest=30, thresh=1e-12, tol=1e-8, mask=1, ortho=0)
+    u, st = S.fgmres(_dev(src), m=10, restarts=30, tol=1e-8, driver=hz.abi.DRIVER_GMRES_REF,
+                     ortho=hz.abi.ORTHO_REF_CONJ, mask=hz.abi.MASK_PHYS, breakdown=1e-12)
+    u0, st0 = S0.fgmres(_dev(src), m=10, restarts=30, tol=1e-8, driver=hz.abi.DRIVER_GMRES_REF,
+                        ortho=hz.abi.ORTHO_REF_CONJ, mask=hz.abi.MASK_PHYS, breakdown=1e-12)
+    err = _rel(_host(u), u_o)
+    print("SAVE_MEM %s: iterations %d (oracle %d), without skipping %d; solution err %.2e"
+          % (mode, st.iters, st_o.iters, st0.iters, err))
+    assert st.converged and st_o.converged
+    assert abs(st.iters - st_o.iters) <= 1
+    assert err < (1e-10 if st.iters == st_o.iters else 1e-7)
+
+
 @pytest.mark.parametrize("kind", ["homo", "linear", "array", "file"])
 def test_reference_signatures_drop_in(hz, oracle, tmp_path, kind):
     """A C++ caller written against the reference's own signatures (hz_ref.h),

@@ -457,3 +457,44 @@ def test_ref_operator_beta_cancels_in_band(hz, oracle):
     q = diff[sponge] / X[sponge]
     assert np.abs(q.real).max() < 1e-9 * np.abs(q.imag).max()
     assert np.all(q.imag > 0)
+
+
+@pytest.mark.parametrize("mode", ["ref", "s7"])
+def test_save_mem_plane_skipping(hz, oracle, mode):
+    """SAVE_MEM (FGMRES.cpp:196-206, functions.cpp:6023-6027, 6286-6295): the
+    planes with kz^2 = lambda_k >= ratio k0^2 are skipped, and the preconditioner
+    and L0 with skipping equal the full ones followed by zeroing those z
+    wavenumbers: P_skip f = F^-1 mask F P f, with F the unscaled DFT along z
+    (numpy.fft), so no skipped plane leaks into the result and no kept plane
+    changes."""
+    pr = hz.problem
+    if mode == "ref":
+        d = pr.ref_geometry(n_nopml=8, pml=3, spg=4, nz_nopml=12, length=(240.0,) * 3, nu=4.0, homo=True)
+        k0 = 2 * np.pi * 4.0 / 1280.0
+        d = d.replace(skip_ratio=5, skip_k0=k0)
+        # L = 400 here: planes with (2 pi q / L)^2 >= 5 k0^2 have q >= 5^0.5 L / 320 = 2.8
+        c = pr.ref_sound_speed(d)
+    else:
+        n = (10, 9, 16)
+        d = pr.cube_geometry(n, 2, pr.omega_for_ppw(0.8, 1.0 / 16, 8), beta=0.5, pml_c=30.0)
+        c = 0.8 + 0.4 * np.random.default_rng(4).random(d.N)
+        d = d.replace(skip_ratio=2)
+    full = oracle.Problem(d.replace(skip_ratio=0), c)
+    skip = oracle.Problem(d, c)
+    lam = full.lamz()
+    if d.skip_k0 > 0:
+        k0 = d.skip_k0
+    else:
+        k0 = d.omega / np.abs(full.c0()).min()
+    keep = lam < d.skip_ratio * k0 ** 2
+    assert np.array_equal(skip.plane_solved().astype(bool), keep)
+    assert 0 < keep.sum() < keep.size
+    nz = d.n[2]
+    f = np.random.default_rng(9).standard_normal(d.N) + 1j * np.random.default_rng(10).standard_normal(d.N)
+
+    def zmask(v):
+        V = np.fft.fft(v.reshape(nz, -1), axis=0)
+        V[~keep] = 0
+        return np.fft.ifft(V, axis=0).reshape(-1)
+    for ps, pf in ((skip.precond_apply(f), full.precond_apply(f)), (skip.L0_apply(f), full.L0_apply(f))):
+        assert np.linalg.norm(ps - zmask(pf)) <= 1e-12 * np.linalg.norm(pf)

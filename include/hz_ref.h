@@ -96,6 +96,25 @@ dtype beta3D(size_m x, size_m y, size_m z, int diag_case, int i, int j, int k);
 // values sized by the caller, Acsr->non_zeros = 5 nx ny - 2 nx - 2 ny.
 // functions.cpp:3270 / templates.h:186
 void GenSparseMatrixOnline2DwithPMLFast(int w, size_m x, size_m y, zcsr* Acsr, dtype kwave_beta2, int* freqs);
+// 9-point plane operators (DIAG_PATTERN 9, definitions.h:188-189).  diag_case
+// as the reference's DIAG9 (functions.cpp:3373-3412): 0 the node, +-1 x
+// neighbours, +-3 y neighbours, 4 / -4 the (x+1, y+1) / (x-1, y-1) corners,
+// 2 / -2 the (x-1, y+1) / (x+1, y-1) corners.
+// functions.cpp:2404 / templates.h:73 (weight sigma moved from x to the corners)
+dtype beta2D_pml_9pts(size_m x, size_m y, int diag_case, dtype kwave_beta2, int i, int j, double sigma);
+// functions.cpp:2467 / templates.h:74 (stencil weights a, c, d)
+dtype beta2D_pml_9pts_opt(size_m x, size_m y, int diag_case, dtype kwave_beta2, int i, int j, double a, double c,
+                          double d);
+// CSR (1-based) of one 9-point plane with beta2D_pml_9pts_opt at the
+// reference's weights a = 1, c = 1, d = 0 (functions.cpp:3553-3555; its corner
+// coefficients are then zero and the values those of the 5-point plane):
+// w == -1 pattern + values + diagonal positions, w >= 0 adds kwave_beta2 to the
+// diagonal.  Acsr->non_zeros = 9 nx ny - 6 nx - 6 ny + 4.
+// functions.cpp:3532 / templates.h:71
+void GenSparseMatrixOnline2DwithPMLand9Pts(int w, size_m x, size_m y, zcsr* Acsr, dtype kwave_beta2, int* freqs);
+// The same with beta2D_pml_9pts at weight sigma.  functions.cpp:3414 / templates.h:69
+void GenSparseMatrixOnline2DwithPMLand9Points(int w, size_m x, size_m y, size_m z, zcsr* Acsr, dtype kwave_beta2,
+                                              int* freqs, double sigma);
 // Sound speed of the built-in model on the physical x/y nodes of every plane
 // (HOMO: c_z; else c = c1 x + c2 y + c3 with c(0,0) = 600, c(0,Ly) = 1500,
 // c(Lx,Ly) = 1800).  functions.cpp:1362 / templates.h:172

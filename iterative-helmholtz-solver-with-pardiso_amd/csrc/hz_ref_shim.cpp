@@ -342,6 +342,93 @@ void GenSparseMatrixOnline2DwithPMLFast(int w, size_m x, size_m y, zcsr* Acsr, d
   Acsr->ia[size] = cnt + 1;
 }
 
+// d2(a, i, side): the stretched second difference of axis a at node i: side 0
+// the centre -a(i)(a(i+1/2) + a(i-1/2))/h^2, +1 / -1 the coupling a(i) a(i +- 1/2)/h^2
+dtype beta2D_pml_9pts(size_m x, size_m y, int diag_case, dtype kwave_beta2, int i, int j, double sigma) {
+  switch (diag_case) {
+    case 0: return second_diff(y, j, 0) + (1.0 - 2.0 * sigma) * second_diff(x, i, 0) + kwave_beta2;
+    case 1: return (1.0 - 2.0 * sigma) * second_diff(x, i, 1);
+    case -1: return (1.0 - 2.0 * sigma) * second_diff(x, i, -1);
+    case 3: return sigma * second_diff(x, i, 0) + second_diff(y, j, 1);
+    case -3: return sigma * second_diff(x, i, 0) + second_diff(y, j, -1);
+    case 2: case 4: return sigma * second_diff(x, i, 1);
+    case -2: case -4: return sigma * second_diff(x, i, -1);
+    default: return 0.0;
+  }
+}
+
+dtype beta2D_pml_9pts_opt(size_m x, size_m y, int diag_case, dtype kwave_beta2, int i, int j, double a, double c,
+                          double d) {
+  const dtype corner = (1.0 - c - 4.0 * d) / 4.0 * kwave_beta2;
+  switch (diag_case) {
+    case 0:
+      return a * (second_diff(y, j, 0) + second_diff(x, i, 0)) +
+             (1.0 - a) * 0.5 * (second_diff(y, j, 0) + second_diff(x, i, 0)) + kwave_beta2 * c;
+    case 1: return a * second_diff(x, i, 1) + kwave_beta2 * d;
+    case -1: return a * second_diff(x, i, -1) + kwave_beta2 * d;
+    case 3: return a * second_diff(y, j, 1) + kwave_beta2 * d;
+    case -3: return a * second_diff(y, j, -1) + kwave_beta2 * d;
+    case 2: return (1.0 - a) * 0.5 * second_diff(y, j, 1) + corner;
+    case -2: return (1.0 - a) * 0.5 * second_diff(y, j, -1) + corner;
+    case 4: return (1.0 - a) * 0.5 * second_diff(x, i, 1) + corner;
+    case -4: return (1.0 - a) * 0.5 * second_diff(x, i, -1) + corner;
+    default: return 0.0;
+  }
+}
+
+// 9-point CSR with coefficient function coef(diag_case, i, j) (columns ascending,
+// the neighbour classification of CheckDiag9Pts, functions.cpp:3373-3412)
+template <class F>
+static void csr9(int w, const size_m& x, const size_m& y, zcsr* Acsr, dtype kwave_beta2, int* freqs, F coef,
+                 const char* where) {
+  g_status = HZ_OK;
+  const int nx = x.n, size = x.n * y.n;
+  const int nnz = size + 2 * (size - y.n) + 2 * (size - nx) + 4 * (size - nx - y.n + 1);
+  if (!Acsr || !Acsr->values || !freqs || (w == -1 && (!Acsr->ia || !Acsr->ja)) || Acsr->non_zeros != nnz) {
+    hz::set_error("9-point plane CSR: non_zeros %d (pattern has %d) or a null array", Acsr ? Acsr->non_zeros : -1,
+                  nnz);
+    fail(where, HZ_ERR_ARG);
+    return;
+  }
+  if (w != -1) {
+    for (int l = 0; l < size; ++l) Acsr->values[freqs[l]] += kwave_beta2;
+    return;
+  }
+  int cnt = 0;
+  for (int l = 0; l < size; ++l) {
+    const int i = l % nx, j = l / nx;
+    const bool L = i > 0, R = i < nx - 1, D = l - nx >= 0, U = l + nx < size;
+    // ascending columns: (x-1,y-1) (x,y-1) (x+1,y-1) (x-1,y) (x,y) (x+1,y) (x-1,y+1) (x,y+1) (x+1,y+1)
+    const int col[9] = {l - nx - 1, l - nx, l - nx + 1, l - 1, l, l + 1, l + nx - 1, l + nx, l + nx + 1};
+    const int dc[9] = {-4, -3, -2, -1, 0, 1, 2, 3, 4};
+    const bool has[9] = {D && L, D, D && R, L, true, R, U && L, U, U && R};
+    Acsr->ia[l] = cnt + 1;
+    for (int e = 0; e < 9; ++e) {
+      if (!has[e]) continue;
+      Acsr->ja[cnt] = col[e] + 1;
+      if (dc[e] == 0) freqs[l] = cnt;
+      Acsr->values[cnt++] = coef(dc[e], i, j);
+    }
+  }
+  Acsr->ia[size] = cnt + 1;
+}
+
+void GenSparseMatrixOnline2DwithPMLand9Pts(int w, size_m x, size_m y, zcsr* Acsr, dtype kwave_beta2, int* freqs) {
+  // the reference's weights (functions.cpp:3553-3555)
+  const double a = 1.0, c = 1.0, d = 0.0;
+  csr9(w, x, y, Acsr, kwave_beta2, freqs,
+       [&](int dc, int i, int j) { return beta2D_pml_9pts_opt(x, y, dc, kwave_beta2, i, j, a, c, d); },
+       "GenSparseMatrixOnline2DwithPMLand9Pts");
+}
+
+void GenSparseMatrixOnline2DwithPMLand9Points(int w, size_m x, size_m y, size_m z, zcsr* Acsr, dtype kwave_beta2,
+                                              int* freqs, double sigma) {
+  (void)z;
+  csr9(w, x, y, Acsr, kwave_beta2, freqs,
+       [&](int dc, int i, int j) { return beta2D_pml_9pts(x, y, dc, kwave_beta2, i, j, sigma); },
+       "GenSparseMatrixOnline2DwithPMLand9Points");
+}
+
 void SetSoundSpeed3D(size_m x, size_m y, size_m z, dtype* sound3D, point source) {
   (void)source;
   const int nx = x.n, ny = y.n, n2 = x.n * y.n;

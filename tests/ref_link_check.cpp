@@ -37,3 +37,13 @@ void SetSoundSpeed2D(size_m, size_m, size_m, dtype*, dtype*, point) { bound("Set
 void GenerateDeltaL(size_m, size_m, size_m, dtype*, dtype*, dtype*, double) { bound("GenerateDeltaL"); }
 void HeteroSoundSpeed3DExtensionToPML(size_m, size_m, size_m, dtype*) { bound("HeteroSoundSpeed3DExtensionToPML"); }
 void HeteroSoundSpeed2DExtensionToPML(size_m, size_m, dtype*) { bound("HeteroSoundSpeed2DExtensionToPML"); }
+dtype beta2D_pml_9pts(size_m, size_m, int, dtype, int, int, double) { bound("beta2D_pml_9pts"); }
+dtype beta2D_pml_9pts_opt(size_m, size_m, int, dtype, int, int, double, double, double) {
+  bound("beta2D_pml_9pts_opt");
+}
+void GenSparseMatrixOnline2DwithPMLand9Pts(int, size_m, size_m, zcsr*, dtype, int*) {
+  bound("GenSparseMatrixOnline2DwithPMLand9Pts");
+}
+void GenSparseMatrixOnline2DwithPMLand9Points(int, size_m, size_m, size_m, zcsr*, dtype, int*, double) {
+  bound("GenSparseMatrixOnline2DwithPMLand9Points");
+}

@@ -114,6 +114,19 @@ int main(int argc, char** argv) {
   GenSparseMatrixOnline2DwithPMLFast(3, x, y, &A, kwave, freqs.data());
   if (hz_ref_last_status() != HZ_OK) return 7;
 
+  // ---- the 9-point planes (DIAG_PATTERN 9): the reference's weights, and sigma = 0.25
+  //      (FGMRES.cpp:140, 144)
+  const int nnz9 = 9 * x.n * y.n - 6 * x.n - 6 * y.n + 4;
+  std::vector<int> ia9(n2 + 1), ja9(nnz9), fr9(n2), ia9s(n2 + 1), ja9s(nnz9), fr9s(n2);
+  std::vector<dtype> v9(nnz9), v9s(nnz9);
+  zcsr A9, A9s;
+  A9.ia = ia9.data(); A9.ja = ja9.data(); A9.values = v9.data(); A9.non_zeros = nnz9;
+  A9s.ia = ia9s.data(); A9s.ja = ja9s.data(); A9s.values = v9s.data(); A9s.non_zeros = nnz9;
+  GenSparseMatrixOnline2DwithPMLand9Pts(-1, x, y, &A9, dtype(0.0), fr9.data());
+  GenSparseMatrixOnline2DwithPMLand9Pts(3, x, y, &A9, kwave, fr9.data());
+  GenSparseMatrixOnline2DwithPMLand9Points(-1, x, y, z, &A9s, kwave, fr9s.data(), 0.25);
+  if (hz_ref_last_status() != HZ_OK) return 8;
+
   // ---- coefficient functions at every (half-)node of x and a set of 3D nodes
   std::vector<dtype> alph, b2, b3;
   for (int t = -1; t <= 2 * x.n; ++t) alph.push_back(alpha(x, 0.5 * t));
@@ -145,7 +158,10 @@ int main(int argc, char** argv) {
       !write_all(out + "_sound3d.bin", sound3D) || !write_all(out + "_sound2d.bin", sound2D) ||
       !write_all(out + "_csr_ia.bin", ia) || !write_all(out + "_csr_ja.bin", ja) ||
       !write_all(out + "_csr_val.bin", vals) || !write_all(out + "_alpha.bin", alph) ||
-      !write_all(out + "_beta2d.bin", b2) || !write_all(out + "_beta3d.bin", b3))
+      !write_all(out + "_beta2d.bin", b2) || !write_all(out + "_beta3d.bin", b3) ||
+      !write_all(out + "_csr9_ia.bin", ia9) || !write_all(out + "_csr9_ja.bin", ja9) ||
+      !write_all(out + "_csr9_val.bin", v9) || !write_all(out + "_csr9s_ia.bin", ia9s) ||
+      !write_all(out + "_csr9s_ja.bin", ja9s) || !write_all(out + "_csr9s_val.bin", v9s))
     return 5;
   printf("diff_sol %.17g status %d\n", diff_sol, st);
   return 0;

@@ -1,6 +1,7 @@
 """The reference-signature set-up functions (include/hz_ref.h: SetSoundSpeed3D,
 SetSoundSpeed3DFromFile, the two layer extensions, SetSoundSpeed2D,
-GenerateDeltaL, GenSparseMatrixOnline2DwithPMLFast, alpha, beta2D_pml, beta3D)
+GenerateDeltaL, GenSparseMatrixOnline2DwithPMLFast, the 9-point plane
+assemblies, alpha, beta2D_pml, beta3D)
 called by a C++ program written against the reference's signatures
 (tests/ref_shim_driver.cpp, the call sequence of FGMRES.cpp:77-96), against the
 oracle on the same grid and medium.  Host code only: the driver's device calls
@@ -56,6 +57,30 @@ def check_setup(d, load, c, oracle):
     K, J, I = np.meshgrid(ks, js, np.arange(nx), indexing="ij")
     ref3 = np.stack([cmz[K], cmy[J], cmx[I], cdx[I] + cdy[J] + cdz[K], cpx[I], cpy[J], cpz[K]], axis=-1)
     assert np.abs(b3 - ref3).max() <= 1e-15 * np.abs(ref3).max()
+    # the 9-point planes: at the reference's weights (a = c = 1, d = 0,
+    # functions.cpp:3553-3555) the corner coefficients vanish and the matrix is
+    # the 5-point one; with sigma the x second difference is weighted 1 - 2 sigma
+    # on the node's row and sigma on the rows above and below (the corners
+    # above take the x coupling toward i + 1/2, those below toward i - 1/2,
+    # functions.cpp:2430-2461)
+    def dense(tag):
+        ia_, ja_, v_ = load(tag + "_ia", np.int32), load(tag + "_ja", np.int32), load(tag + "_val")
+        assert ia_[0] == 1 and ia_[-1] == v_.size + 1 and v_.size == 9 * n2 - 6 * nx - 6 * ny + 4
+        A = np.zeros((n2, n2), np.complex128)
+        for r in range(n2):
+            cols = ja_[ia_[r] - 1:ia_[r + 1] - 1] - 1
+            assert np.all(np.diff(cols) > 0)
+            A[r, cols] = v_[ia_[r] - 1:ia_[r + 1] - 1]
+        return A
+    assert np.abs(dense("csr9") - M_ref).max() <= 1e-14 * np.abs(M_ref).max()
+    sig = 0.25
+    Sx = np.diag(np.ones(ny - 1), 1)  # row j couples to j + 1
+    Cp = np.diag(cpx) @ (np.diag(np.ones(nx - 1), 1) + np.diag(np.ones(nx - 1), -1))
+    Cm = np.diag(cmx) @ (np.diag(np.ones(nx - 1), 1) + np.diag(np.ones(nx - 1), -1))
+    Dxc = np.diag(cdx)
+    M9 = (np.kron(np.eye(ny), (1 - 2 * sig) * Dx) + np.kron(Dy, np.eye(nx)) + kwave * np.eye(n2)
+          + sig * np.kron(Sx, Dxc + Cp) + sig * np.kron(Sx.T, Dxc + Cm))
+    assert np.abs(dense("csr9s") - M9).max() <= 1e-14 * np.abs(M9).max()
 
 
 @pytest.mark.parametrize("kind", ["homo", "linear", "array", "file"])
@@ -86,7 +111,8 @@ def test_failed_drop_in_calls_leave_nan(tmp_path):
 REPLACED = ["Solve3DSparseUsingFT_CSR", "ApplyCoeffMatrixA_CSR", "Multiply3DSparseUsingFT", "alpha", "beta3D",
             "beta2D_pml", "GenSparseMatrixOnline2DwithPMLFast", "SetSoundSpeed3D", "SetSoundSpeed3DFromFile",
             "SetSoundSpeed2D", "GenerateDeltaL", "HeteroSoundSpeed3DExtensionToPML",
-            "HeteroSoundSpeed2DExtensionToPML"]
+            "HeteroSoundSpeed2DExtensionToPML", "beta2D_pml_9pts", "beta2D_pml_9pts_opt",
+            "GenSparseMatrixOnline2DwithPMLand9Pts", "GenSparseMatrixOnline2DwithPMLand9Points"]
 
 
 @pytest.mark.parametrize("rename", [True, False])

@@ -183,6 +183,15 @@ int hz_precond_factor(hz_problem* p, int prec);
  * (must have been factored).  Replaces Solve3DSparseUsingFT_CSR,
  * functions.cpp:5837-6061. */
 int hz_precond_apply(hz_problem* p, int prec, int nrhs, const hz_c128* in, hz_c128* out);
+/* hz_precond_apply without any device allocation: the z-transformed planes
+ * live in a fixed 64 MiB device array of the library (one per device, held
+ * until the process ends) instead of the problem's grown work buffer.  No
+ * reference counterpart: for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops that forbid hipMalloc).  nrhs * N
+ * must fit the array (8 Mi complex64 or 4 Mi complex128 values), else
+ * HZ_ERR_ARG.  Calls that overlap on one device (several streams or host
+ * threads) share the array and race. */
+int hz_precond_apply_noalloc(hz_problem* p, int prec, int nrhs, const hz_c128* in, hz_c128* out);
 /* out = (L0^{-1})^H in, with the conjugate-transposed plane factors
  * (no reference counterpart: adjoint solves of the misfit gradient). */
 int hz_precond_apply_adjoint(hz_problem* p, int prec, int nrhs, const hz_c128* in, hz_c128* out);

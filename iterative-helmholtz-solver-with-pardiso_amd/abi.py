@@ -99,6 +99,7 @@ SIGNATURES = {
     "hz_fgmres_multi": (_i, [_i, C.POINTER(C.c_void_p), C.POINTER(C.c_int), C.POINTER(C.c_void_p),
                              C.POINTER(C.c_void_p), C.POINTER(KrylovOpts), C.POINTER(C.POINTER(KrylovStats))]),
     "hz_precond_apply": (_i, [_vp, _i, _i, _vp, _vp]),
+    "hz_precond_apply_noalloc": (_i, [_vp, _i, _i, _vp, _vp]),
     "hz_precond_apply_adjoint": (_i, [_vp, _i, _i, _vp, _vp]),
     "hz_L0_apply": (_i, [_vp, _i, _vp, _vp]),
     "hz_op_apply": (_i, [_vp, _i, _vp, _vp]),

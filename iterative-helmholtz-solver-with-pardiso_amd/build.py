@@ -24,6 +24,7 @@ SOURCES = [
     "k_stencil.hip",
     "k_blocktri.hip",
     "k_vec.hip",
+    "hz_noalloc.hip",
 ]
 
 FLAGS = [

@@ -196,6 +196,42 @@ int hz_precond_apply_noalloc(hz_problem* p, int prec, int nrhs, const hz_c128* i
  * (no reference counterpart: adjoint solves of the misfit gradient). */
 int hz_precond_apply_adjoint(hz_problem* p, int prec, int nrhs, const hz_c128* in, hz_c128* out);
 
+/* HODLR compression of the plane factors (the reference's HODLR build:
+ * SymRecCompressStruct / LowRankApproxStruct, HODLR/BinaryTreesHODLR.cpp:298-416,
+ * used by DirFactFastDiagStructOnline :2870-2969 and applied by
+ * DirSolveFastDiagStruct :2971-3018 inside Solve3DSparseUsingFT_HODLR,
+ * functions.cpp:6064-6244).  Every block inverse of precision `prec` (already
+ * factored) is compressed in its symmetric form K_j = G_j D_j: nodes of more
+ * than `smallsize` rows split into n - ceil(n/2) and ceil(n/2) rows, the lower
+ * off-diagonal block keeps the singular triplets with S[t] / S[0] >= eps
+ * (0 <= eps < 1; 0 keeps every nonzero one), the upper one is its transpose.
+ * Later forward preconditioner applies of that precision use the compressed
+ * records; adjoint applies use the dense factors, which flags =
+ * HZ_HODLR_FREE_DENSE releases.  nx <= 256, tree depth 3..6 (smallsize between
+ * about nx/64 and nx/8), at most rank 16 per block.  eps < 0 drops the
+ * compressed form (back to the dense factors if kept).  A new
+ * hz_precond_factor drops it too. */
+enum { HZ_HODLR_FREE_DENSE = 1 };
+int hz_precond_compress(hz_problem* p, int prec, double eps, int smallsize, int flags);
+typedef struct {
+  int32_t active;     /* 1 when the forward applies of prec use the compressed records */
+  int32_t smallsize;
+  int32_t levels;     /* depth of the row tree */
+  int32_t nodes;      /* compressed off-diagonal blocks per block inverse */
+  int32_t leaf_max;   /* rows of the largest leaf */
+  int32_t max_rank;
+  double eps;
+  double mean_rank;   /* over every (slot, y-line, node) */
+  double bytes;       /* device bytes of the compressed records */
+  double dense_bytes; /* device bytes of the dense factors still held */
+} hz_hodlr_info;
+int hz_precond_hodlr_info(const hz_problem* p, int prec, hz_hodlr_info* out);
+/* ranks of plane k's compressed blocks: out[j * nodes + q] for y-line j and
+ * node q (breadth-first order of the row tree); cap >= ny * nodes.  Planes
+ * that share a factor slot report the same ranks; a plane skipped by the
+ * SAVE_MEM rule gives HZ_ERR_ARG. */
+int hz_precond_hodlr_ranks(const hz_problem* p, int prec, int64_t k, int32_t* out, int64_t cap);
+
 /* ---------------------------------------------------------- operators */
 /* out = L0 in.  Replaces Multiply3DSparseUsingFT, functions.cpp:6247-6318. */
 int hz_L0_apply(hz_problem* p, int nrhs, const hz_c128* in, hz_c128* out);

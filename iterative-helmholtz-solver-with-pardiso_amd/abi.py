@@ -66,6 +66,24 @@ class TuneInfo(C.Structure):
     ]
 
 
+class HodlrInfo(C.Structure):
+    _fields_ = [
+        ("active", C.c_int32),
+        ("smallsize", C.c_int32),
+        ("levels", C.c_int32),
+        ("nodes", C.c_int32),
+        ("leaf_max", C.c_int32),
+        ("max_rank", C.c_int32),
+        ("eps", C.c_double),
+        ("mean_rank", C.c_double),
+        ("bytes", C.c_double),
+        ("dense_bytes", C.c_double),
+    ]
+
+
+HODLR_FREE_DENSE = 1
+
+
 class KrylovStats(C.Structure):
     _fields_ = [
         ("iters", C.c_int32),
@@ -101,6 +119,9 @@ SIGNATURES = {
     "hz_precond_apply": (_i, [_vp, _i, _i, _vp, _vp]),
     "hz_precond_apply_noalloc": (_i, [_vp, _i, _i, _vp, _vp]),
     "hz_precond_apply_adjoint": (_i, [_vp, _i, _i, _vp, _vp]),
+    "hz_precond_compress": (_i, [_vp, _i, _d, _i, _i]),
+    "hz_precond_hodlr_info": (_i, [_vp, _i, C.POINTER(HodlrInfo)]),
+    "hz_precond_hodlr_ranks": (_i, [_vp, _i, _i64, C.POINTER(C.c_int32), _i64]),
     "hz_L0_apply": (_i, [_vp, _i, _vp, _vp]),
     "hz_op_apply": (_i, [_vp, _i, _vp, _vp]),
     "hz_op_apply_adjoint": (_i, [_vp, _i, _vp, _vp]),

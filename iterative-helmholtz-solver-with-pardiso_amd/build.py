@@ -25,6 +25,7 @@ SOURCES = [
     "k_blocktri.hip",
     "k_vec.hip",
     "hz_noalloc.hip",
+    "k_hodlr.hip",
 ]
 
 FLAGS = [

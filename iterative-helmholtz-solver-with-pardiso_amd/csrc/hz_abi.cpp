@@ -504,7 +504,12 @@ int hz_problem_info(const hz_problem* p, int64_t* dims, double* factor_bytes) {
   HZ_API_BEGIN
   HZ_CHECK(p, HZ_ERR_ARG, "null problem");
   if (dims) { dims[0] = p->nx; dims[1] = p->ny; dims[2] = p->nz; dims[3] = p->N; }
-  if (factor_bytes) *factor_bytes = (double)(p->fac[0].G.bytes + p->fac[1].G.bytes);
+  if (factor_bytes) {
+    double b = 0;
+    for (int q = 0; q < 2; ++q)
+      b += (double)(p->fac[q].G.bytes + p->fac[q].Gsym.bytes + p->fac[q].hodlr.pool.bytes);
+    *factor_bytes = b;
+  }
   HZ_API_END
 }
 
@@ -527,6 +532,51 @@ int hz_precond_apply_adjoint(hz_problem* p, int prec, int nrhs, const hz_c128* i
   HZ_API_BEGIN
   HZ_CHECK(p && in && out && nrhs >= 1, HZ_ERR_ARG, "bad arguments");
   precond_apply(p, prec, nrhs, (const double2*)in, (double2*)out, true, false);
+  HZ_API_END
+}
+
+int hz_precond_compress(hz_problem* p, int prec, double eps, int smallsize, int flags) {
+  HZ_API_BEGIN
+  HZ_CHECK(p, HZ_ERR_ARG, "null problem");
+  HZ_CHECK(prec == HZ_PREC_FP64 || prec == HZ_PREC_C64, HZ_ERR_ARG, "bad precision %d", prec);
+  HZ_CHECK((flags & ~HZ_HODLR_FREE_DENSE) == 0, HZ_ERR_ARG, "unknown flags 0x%x", flags);
+  hodlr_compress(p, prec, eps, smallsize, flags);
+  HZ_API_END
+}
+
+int hz_precond_hodlr_info(const hz_problem* p, int prec, hz_hodlr_info* out) {
+  HZ_API_BEGIN
+  HZ_CHECK(p && out, HZ_ERR_ARG, "bad arguments");
+  HZ_CHECK(prec == HZ_PREC_FP64 || prec == HZ_PREC_C64, HZ_ERR_ARG, "bad precision %d", prec);
+  const Factors& f = p->fac[prec];
+  const Hodlr& h = f.hodlr;
+  memset(out, 0, sizeof(*out));
+  out->active = h.active ? 1 : 0;
+  out->smallsize = h.small;
+  out->levels = h.lev;
+  out->nodes = h.nn;
+  out->leaf_max = h.mleaf;
+  out->max_rank = h.pmax;
+  out->eps = h.eps;
+  out->mean_rank = h.mean_rank;
+  out->bytes = h.bytes;
+  out->dense_bytes = (double)(f.G.bytes + f.Gsym.bytes);
+  HZ_API_END
+}
+
+int hz_precond_hodlr_ranks(const hz_problem* p, int prec, int64_t k, int32_t* out, int64_t cap) {
+  HZ_API_BEGIN
+  HZ_CHECK(p && out, HZ_ERR_ARG, "bad arguments");
+  HZ_CHECK(prec == HZ_PREC_FP64 || prec == HZ_PREC_C64, HZ_ERR_ARG, "bad precision %d", prec);
+  const Hodlr& h = p->fac[prec].hodlr;
+  HZ_CHECK(h.active, HZ_ERR_STATE, "no compressed factors of precision %d", prec);
+  HZ_CHECK(k >= 0 && k < p->nz, HZ_ERR_ARG, "plane %lld outside 0..%lld", (long long)k, (long long)p->nz - 1);
+  const int slot = p->fmap[k];
+  HZ_CHECK(slot >= 0, HZ_ERR_ARG, "plane %lld is skipped (no factor)", (long long)k);
+  const int64_t cnt = p->ny * (int64_t)h.nn;
+  HZ_CHECK(cap >= cnt, HZ_ERR_ARG, "rank buffer of %lld entries, %lld needed", (long long)cap, (long long)cnt);
+  const unsigned char* r = h.ranks.data() + (size_t)slot * cnt;
+  for (int64_t e = 0; e < cnt; ++e) out[e] = r[e];
   HZ_API_END
 }
 

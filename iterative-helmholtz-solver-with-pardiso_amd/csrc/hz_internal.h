@@ -78,6 +78,19 @@ struct Tuning {
   int fft_lines = -1;    // z-FFT: 0 register-resident kernel, 8/16/32 lines per LDS workgroup, -1 automatic
 };
 
+// HODLR-compressed form of one precision's block inverses (k_hodlr.hip):
+// pool of records [leaf rows][level 0 rows]...[level L-1 rows], each row n
+// complex, one record per (factor slot, y-line) with its header in recs.
+struct Hodlr {
+  bool active = false;
+  double eps = 0.0;
+  int small = 0, lev = 0, mleaf = 0, nn = 0, pmax = 0;
+  double mean_rank = 0.0, bytes = 0.0;
+  DevBuf pool, recs;
+  std::vector<unsigned char> ranks;  // (slot, y-line, node) ranks, host copy
+  void release();
+};
+
 // Plane factors of one precision.
 struct Factors {
   bool ready = false;
@@ -87,6 +100,7 @@ struct Factors {
   // block columns paired J / T-1-J), for the widths it is built for
   bool sym_ready = false;
   DevBuf Gsym;
+  Hodlr hodlr;
 };
 
 }  // namespace hz
@@ -223,6 +237,11 @@ void factor_end(hz_problem* p, int prec, hipStream_t s);
 // `elem` bytes (the preconditioner's output there, FGMRES.cpp:196-206 and
 // functions.cpp:6023-6027)
 void launch_zero_skipped(hz_problem* p, int nrhs, void* x, size_t elem, hipStream_t s);
+// HODLR compression of precision prec's factors at eps with leaves of at most
+// `small` rows (eps < 0: drop the compressed form); flags HZ_HODLR_FREE_DENSE
+void hodlr_compress(hz_problem* p, int prec, double eps, int small, int flags);
+// forward plane solves on the compressed factors
+void launch_hodlr_solve(hz_problem* p, int prec, int nrhs, void* x, hipStream_t s);
 // in-place plane solves on z-transformed data (prec 0: double2, 1: float2).
 void launch_plane_solve(hz_problem* p, int prec, int nrhs, void* x, bool adjoint, hipStream_t s);
 // the same with explicit launch choices (autotuner)

@@ -2499,6 +2499,7 @@ void factor_begin(hz_problem* p, int prec, hipStream_t s) {
   Factors& f = p->fac[prec];
   f.ready = false;
   f.sym_ready = false;
+  f.hodlr.release();
   f.elem = prec == HZ_PREC_FP64 ? sizeof(double2) : sizeof(float2);
   const size_t bytes = (size_t)p->nslot * ny * (size_t)n * n * f.elem;
   if (f.G.bytes != bytes) {
@@ -2592,6 +2593,16 @@ namespace {
 void plane_solve_kernels(hz_problem* p, int prec, int nrhs, void* x, bool adjoint, const Tuning& t, hipStream_t s) {
   Factors& f = p->fac[prec];
   HZ_CHECK(f.ready, HZ_ERR_STATE, "preconditioner of precision %d not factored", prec);
+  if (f.hodlr.active) {
+    // compressed factors: forward sweeps on the HODLR records; adjoint sweeps
+    // on the dense factors when they were kept
+    if (!adjoint) {
+      launch_hodlr_solve(p, prec, nrhs, x, s);
+      return;
+    }
+    HZ_CHECK(f.G.ptr != nullptr, HZ_ERR_STATE,
+             "adjoint plane solve: the dense factors were released by HZ_HODLR_FREE_DENSE");
+  }
   const int n = (int)p->nx, ny = (int)p->ny;
   const int ng = std::max(1, APPLY_THREADS / n);
   PlaneTabs tb = plane_tabs(p);

@@ -100,6 +100,29 @@ class Solver:
     def factor(self, prec=PREC_FP64):
         self._run("hz_precond_factor", int(prec))
 
+    def compress(self, prec=PREC_C64, eps=1e-5, smallsize=16, free_dense=False):
+        """HODLR compression of the factors of `prec` (hz_precond_compress):
+        later forward preconditioner applies use the compressed records;
+        eps < 0 drops them."""
+        self._run("hz_precond_compress", int(prec), float(eps), int(smallsize),
+                  abi.HODLR_FREE_DENSE if free_dense else 0)
+        return self.hodlr_info(prec)
+
+    def hodlr_info(self, prec=PREC_C64):
+        info = abi.HodlrInfo()
+        abi.check(abi.lib().hz_precond_hodlr_info(self.h, int(prec), C.byref(info)), "hz_precond_hodlr_info")
+        return {f: getattr(info, f) for f, _ in abi.HodlrInfo._fields_}
+
+    def hodlr_ranks(self, k, prec=PREC_C64):
+        """(ny, nodes) ranks of plane k's compressed blocks."""
+        info = self.hodlr_info(prec)
+        ny = self.desc.n[1]
+        out = np.zeros(ny * info["nodes"], np.int32)
+        abi.check(abi.lib().hz_precond_hodlr_ranks(self.h, int(prec), int(k),
+                                                   out.ctypes.data_as(C.POINTER(C.c_int32)), out.size),
+                  "hz_precond_hodlr_ranks")
+        return out.reshape(ny, info["nodes"])
+
     def autotune(self, prec=PREC_C64, nrhs=1):
         """Time the preconditioner's launch variants for nrhs right-hand sides
         and keep the fastest; returns the choice."""

@@ -46,6 +46,10 @@ def parse():
     ap.add_argument("--max-iters", type=int, default=20000)
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--hodlr-eps", type=float, default=1e-4,
+                    help="also solve with the HODLR-compressed complex64 factors at this eps "
+                         "(the reference's HODLR threshold, main.cpp:161); < 0 skips")
+    ap.add_argument("--hodlr-small", type=int, default=7, help="HODLR leaf size (smallsize)")
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--config", choices=["all", "mixed200", "batched", "adjoint"], default="all",
                     help="all (default): the 200^3 mixed-precision solve as the headline, with the 128^3 FP64, "
@@ -197,6 +201,9 @@ def main():
         },
     }
 
+    if a.hodlr_eps >= 0:
+        out["hodlr"] = bench_hodlr(a, S, A, f, u32, t_mixed, stream, buf, torch, np)
+
     if not a.skip_fp64:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -226,6 +233,40 @@ def main():
                 out[key] = {"error": "%s: %s" % (type(e).__name__, e)}
             torch.cuda.empty_cache()
     print(json.dumps(out), flush=True)
+
+
+def bench_hodlr(a, S, A, f, u_dense, t_dense, stream, buf, torch, np):
+    """The same solve with the complex64 block inverses HODLR-compressed
+    (hz_precond_compress, the reference's HODLR build at its threshold):
+    factorisation, compression and FGMRES timed together; the compressed
+    plane solve timed with HIP events like the dense one."""
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    S.factor(A.PREC_C64)
+    info = S.compress(A.PREC_C64, eps=a.hodlr_eps, smallsize=a.hodlr_small)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    u, st = S.fgmres(f, m=a.m, restarts=max(1, a.max_iters // a.m), tol=a.tol, prec=A.PREC_C64, autotune=True)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    note("hodlr eps %g: %.2f s, %d iterations" % (a.hodlr_eps, t2 - t0, st.iters))
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for _ in range(3):
+        A.check(A.lib().hz_plane_solve(S.h, A.PREC_C64, 0, 1, buf.data_ptr()), "hz_plane_solve")
+    reps = 10
+    e0.record(stream)
+    for _ in range(reps):
+        A.check(A.lib().hz_plane_solve(S.h, A.PREC_C64, 0, 1, buf.data_ptr()), "hz_plane_solve")
+    e1.record(stream)
+    torch.cuda.synchronize()
+    err = (torch.linalg.vector_norm(u - u_dense) / torch.linalg.vector_norm(u_dense)).item()
+    S.compress(A.PREC_C64, eps=-1.0)
+    return {"eps": a.hodlr_eps, "smallsize": a.hodlr_small, "time_s": round(t2 - t0, 4), "iters": st.iters,
+            "rel_res": st.rel_res, "factor_compress_s": round(t1 - t0, 4),
+            "plane_solve_ms": round(e0.elapsed_time(e1) / reps, 4), "levels": info["levels"],
+            "max_rank": info["max_rank"], "mean_rank": round(info["mean_rank"], 3),
+            "record_bytes": info["bytes"], "dense_factor_bytes": info["dense_bytes"],
+            "err_vs_dense_mixed": err, "speedup_vs_dense_mixed": round(t_dense / (t2 - t0), 3)}
 
 
 def plane_solve_kernel(nx, tuning):

@@ -296,10 +296,18 @@ int hz_fgmres_multi(int nprob, hz_problem* const* probs, const int* nrhs, const 
 /* Receiver misfit J = 1/2 sum_s ||P u_s - d_s||^2 (S7 mode) for nsrc sources
  * f (device, nsrc*N), receivers rec (host, nrec flat indices) and data d
  * (host, nsrc*nrec).  Forward fields are kept on the device when
- * store_fields != 0, else recomputed by hz_misfit_backward.  No reference
+ * store_fields > 0, recomputed by hz_misfit_backward when it is 0, and
+ * chosen against the field budget with HZ_FIELDS_AUTO.  No reference
  * counterpart. */
 int hz_misfit_forward(hz_problem* p, int nsrc, const hz_c128* f, int nrec, const int64_t* rec,
                       const hz_c128* data, const hz_krylov_opts* opts, int store_fields, double* J);
+/* store_fields = HZ_FIELDS_AUTO: keep the nsrc forward fields (nsrc * N complex128) when they fit the
+ * field budget, else recompute them in hz_misfit_backward.  The budget is the value of
+ * hz_misfit_set_budget, or (0, the default) half of the device memory free when hz_misfit_forward
+ * starts.  hz_misfit_fields_stored reports the choice of the last forward call. */
+enum { HZ_FIELDS_AUTO = -1 };
+int hz_misfit_set_budget(hz_problem* p, double bytes);
+int hz_misfit_fields_stored(const hz_problem* p, int* stored);
 /* grad (device, N doubles) = dJ/dm, m = 1/c^2:
  *   A^H lambda_s = P^T (P u_s - d_s),  grad = -omega^2 sum_s Re(conj(u_s) lambda_s). */
 int hz_misfit_backward(hz_problem* p, double* grad);

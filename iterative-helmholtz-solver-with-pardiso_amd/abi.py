@@ -82,6 +82,7 @@ class HodlrInfo(C.Structure):
 
 
 HODLR_FREE_DENSE = 1
+FIELDS_AUTO = -1
 
 
 class KrylovStats(C.Structure):
@@ -133,6 +134,8 @@ SIGNATURES = {
     "hz_misfit_forward": (_i, [_vp, _i, _vp, _i, C.POINTER(C.c_int64), _vp, C.POINTER(KrylovOpts), _i,
                                C.POINTER(C.c_double)]),
     "hz_misfit_backward": (_i, [_vp, _vp]),
+    "hz_misfit_set_budget": (_i, [_vp, _d]),
+    "hz_misfit_fields_stored": (_i, [_vp, C.POINTER(C.c_int)]),
     "hz_fft_z": (_i, [_vp, _i, _i, _i, _vp, _vp]),
     "hz_plane_solve": (_i, [_vp, _i, _i, _i, _vp]),
     "hz_velocity_from_file": (_i, [C.c_char_p, C.POINTER(HzDesc), C.POINTER(C.c_double)]),

@@ -821,7 +821,21 @@ int hz_misfit_forward(hz_problem* p, int nsrc, const hz_c128* f, int nrec, const
                         p->stream));
   p->src_f.alloc(vb * nsrc);
   HZ_HIP(hipMemcpyAsync(p->src_f.ptr, f, vb * nsrc, hipMemcpyDeviceToDevice, p->stream));
-  p->fields_stored = store_fields != 0;
+  if (store_fields == HZ_FIELDS_AUTO) {
+    // keep the fields when they fit the budget (the caller's, or half of the
+    // device memory free now, after the problem's own buffers)
+    double budget = p->field_budget;
+    if (!(budget > 0.0)) {
+      size_t free_b = 0, total_b = 0;
+      HZ_HIP(hipMemGetInfo(&free_b, &total_b));
+      budget = 0.5 * (double)free_b;
+    }
+    p->fields_stored = (double)vb * nsrc <= budget;
+  } else {
+    HZ_CHECK(store_fields == 0 || store_fields == 1, HZ_ERR_ARG, "store_fields=%d (0, 1 or HZ_FIELDS_AUTO)",
+             store_fields);
+    p->fields_stored = store_fields != 0;
+  }
   if (p->fields_stored) p->fields_u.alloc(vb * nsrc);
   else grow(p->fields_u, vb * B);
   double Jt = 0.0;
@@ -839,6 +853,21 @@ int hz_misfit_forward(hz_problem* p, int nsrc, const hz_c128* f, int nrec, const
   }
   p->misfit_ready = true;
   *J = Jt;
+  HZ_API_END
+}
+
+int hz_misfit_set_budget(hz_problem* p, double bytes) {
+  HZ_API_BEGIN
+  HZ_CHECK(p && bytes >= 0.0, HZ_ERR_ARG, "bad arguments");
+  p->field_budget = bytes;
+  HZ_API_END
+}
+
+int hz_misfit_fields_stored(const hz_problem* p, int* stored) {
+  HZ_API_BEGIN
+  HZ_CHECK(p && stored, HZ_ERR_ARG, "bad arguments");
+  HZ_CHECK(p->misfit_ready, HZ_ERR_STATE, "hz_misfit_forward has not run");
+  *stored = p->fields_stored ? 1 : 0;
   HZ_API_END
 }
 

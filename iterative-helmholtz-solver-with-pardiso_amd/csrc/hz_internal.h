@@ -178,6 +178,7 @@ struct hz_problem {
   hz::DevBuf m_adj;     // adjoint right-hand side and multiplier (2 N)
   hz_krylov_opts mopts;
   bool fields_stored = false;
+  double field_budget = 0.0;  // bytes for stored forward fields (HZ_FIELDS_AUTO); 0: half the free memory
   bool misfit_ready = false;
 
   // offsets into d_tab

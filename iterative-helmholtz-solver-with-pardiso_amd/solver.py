@@ -219,6 +219,8 @@ class Solver:
     # ----------------------------------------------------------- adjoint
     def misfit_forward(self, f, rec, data, m=40, restarts=20, tol=1e-8, prec=PREC_C64, store_fields=True, batch=0,
                        graph=False):
+        """J = 1/2 sum_s ||P u_s - d_s||^2 (hz_misfit_forward); store_fields True,
+        False or "auto" (against the field budget)."""
         f, nsrc = self._vec(f)
         rec = np.ascontiguousarray(rec, dtype=np.int64)
         data = np.ascontiguousarray(data, dtype=np.complex128).reshape(nsrc, rec.size)
@@ -226,9 +228,19 @@ class Solver:
                            mask=MASK_FULL, monitor=0, tol=float(tol), breakdown=0.0, hist=None, hist_cap=0,
                            prec=int(prec), batch=int(batch), graph=int(bool(graph)))
         J = C.c_double()
+        store = abi.FIELDS_AUTO if store_fields in ("auto", None) else int(bool(store_fields))
         self._run("hz_misfit_forward", nsrc, _ptr(f), rec.size, rec.ctypes.data_as(C.POINTER(C.c_int64)),
-                  C.c_void_p(data.ctypes.data), C.byref(o), int(bool(store_fields)), C.byref(J))
+                  C.c_void_p(data.ctypes.data), C.byref(o), store, C.byref(J))
         return J.value
+
+    def misfit_budget(self, nbytes):
+        """Field budget of store_fields="auto" (0: half the free device memory)."""
+        abi.check(abi.lib().hz_misfit_set_budget(self.h, float(nbytes)), "hz_misfit_set_budget")
+
+    def misfit_fields_stored(self):
+        v = C.c_int()
+        abi.check(abi.lib().hz_misfit_fields_stored(self.h, C.byref(v)), "hz_misfit_fields_stored")
+        return bool(v.value)
 
     def misfit_backward(self):
         torch = _torch()

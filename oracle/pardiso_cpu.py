@@ -275,24 +275,31 @@ def sample(P, iters, m, threads, nplanes=None):
     b = np.random.default_rng(0).standard_normal(n2) + 0j
     list(ex.map(lambda q: q.solve(b), solvers))  # first solves carry one-time set-up
     t0 = time.perf_counter()
-    list(ex.map(lambda q: q.solve(b), solvers))
-    t_sol = time.perf_counter() - t0
+    for _ in range(3):
+        list(ex.map(lambda q: q.solve(b), solvers))
+    t_sol = (time.perf_counter() - t0) / 3
     for s in solvers:
         s.release()
+    # the vector stages: one untimed call each (FFT plans, thread start-up), then the mean of three
     x = np.random.default_rng(1).standard_normal(N) + 1j
-    t0 = time.perf_counter()
-    y = _fft_z(x, (nz, ny, nx), -1, threads)
-    _fft_z(y, (nz, ny, nx), +1, threads)
-    t_fft = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    P.op_apply(x)
-    t_op = time.perf_counter() - t0
+
+    def timed(fn, reps=3):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        return (time.perf_counter() - t0) / reps
+
+    t_fft = timed(lambda: _fft_z(_fft_z(x, (nz, ny, nx), -1, threads), (nz, ny, nx), +1, threads))
+    t_op = timed(lambda: P.op_apply(x))
     from scipy.linalg.blas import zaxpy, zdotc
     V = [np.random.default_rng(s).standard_normal(N) + 0j for s in range(2)]
-    t0 = time.perf_counter()
-    h = zdotc(V[0], V[1])
-    V[1] = zaxpy(V[0], V[1], a=-h)
-    t_gs1 = time.perf_counter() - t0
+
+    def gs1():
+        h = zdotc(V[0], V[1])
+        V[1] = zaxpy(V[0], V[1], a=-h)
+
+    t_gs1 = timed(gs1)
     batches = float(np.sum(P.plane_solved())) / kp
     t_factor = t_fac * batches
     t_iter = t_sol * batches + t_fft + t_op + t_gs1 * (m / 2.0 + 1)

@@ -142,7 +142,8 @@ def main():
     # ---- roofline of the dominant kernel: the complex64 plane solve, timed with
     # HIP events on the problem's stream.  Algorithmic bytes per launch: the
     # 2 ny - 1 block reads per plane (nx^2 complex64, or the packed upper block
-    # triangle for the symmetric kernel) plus four passes over the plane vectors
+    # triangle for the symmetric kernel; the twisted solve: 2 ny packed blocks
+    # per factor slot, both phases) plus four passes over the plane vectors
     # (read b, write t, read t, write x).
     nx, ny, nz = d.n
     N = d.N
@@ -161,7 +162,7 @@ def main():
     torch.cuda.synchronize()
     ms_kernel = e0.elapsed_time(e1) / reps
     fam = tuning.get("solve_kernel", 0)
-    bytes_kernel = plane_solve_bytes(d.n, fam)
+    bytes_kernel = plane_solve_bytes(d.n, fam, S.slots()[0])
     achieved = bytes_kernel / (ms_kernel * 1e-3) / 1e9
 
     out = {
@@ -272,6 +273,8 @@ def bench_hodlr(a, S, A, f, u_dense, t_dense, stream, buf, torch, np):
 def plane_solve_kernel(nx, tuning):
     """Name of the complex64 single-RHS plane-solve kernel the tuned choice launches."""
     fam = tuning.get("solve_kernel", 0)
+    if fam == 6:
+        return "k_solve_twist<%d, 0> + k_solve_twist<%d, 1>" % (nx, nx)
     if fam == 5:
         return "k_solve_sym2<%d, false>" % nx
     if fam == 3:
@@ -508,30 +511,35 @@ def bench_config2(a, hz, np, torch, steps, warmup):
 def traffic_source(n, fam):
     """Where roofline.traffic comes from: it is not measured in this run (the
     counter passes need rocprofv3), it is read from the committed summary."""
-    names = {3: "k_solve_dma_c64_200cube_pmc.md", 5: "k_solve_sym2_c64_200cube_pmc.md"}
-    if n != 200 or fam not in names:
+    names = {3: "k_solve_dma_c64_200cube_pmc.md", 5: "k_solve_sym2_c64_200cube_pmc.md",
+             6: "k_solve_twist_c64_200cube_pmc.md"}
+    if n != 200 or fam not in names or not os.path.exists(os.path.join(ROOT, "profiles", names[fam])):
         return None
     return "committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary profiles/%s (not measured in this run)" % names[fam]
 
 
-def plane_solve_bytes(dims, fam):
+def plane_solve_bytes(dims, fam, nslot):
     """Algorithmic HBM bytes of one complex64 single-RHS plane solve: 2 ny - 1
     block reads per plane (the full nx^2 block, or for the symmetric kernel the
-    upper block triangle of 8x8 tiles, 64 T (T+1)/2 entries with T = nx/8) and
-    four passes over the nx ny nz vector."""
+    upper block triangle of 8x8 tiles, 64 T (T+1)/2 entries with T = nx/8; the
+    twisted solve reads 2 ny packed blocks per factor slot for both of its
+    planes: ny - 1 in phase 0, ny + 1 in phase 1) and four passes over the
+    nx ny nz vector."""
     nx, ny, nz = dims
-    if fam == 5:
+    if fam in (5, 6):
         t = nx // 8
         blk = 64 * t * (t + 1) // 2
     else:
         blk = nx * nx
-    return (2 * ny - 1) * blk * 8 * nz + 4 * 8 * nx * ny * nz
+    blocks = 2 * ny * nslot if fam == 6 else (2 * ny - 1) * nz
+    return blocks * blk * 8 + 4 * 8 * nx * ny * nz
 
 
 def measured_traffic(n, fam):
     """HBM bytes per launch of the plane solve from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE passes (scaled by a 1 GiB copy), for the 200^3 grid."""
-    names = {3: "k_solve_dma_c64_200cube_pmc.md", 5: "k_solve_sym2_c64_200cube_pmc.md"}
+    names = {3: "k_solve_dma_c64_200cube_pmc.md", 5: "k_solve_sym2_c64_200cube_pmc.md",
+             6: "k_solve_twist_c64_200cube_pmc.md"}
     if n != 200 or fam not in names:
         return None
     path = os.path.join(ROOT, "profiles", names[fam])
@@ -551,12 +559,13 @@ def kernel_only(S, d, A, torch):
     dst.copy_(src)
     torch.cuda.synchronize()
     buf = torch.randn(d.N, dtype=torch.complex64, device="cuda")
-    # 5 launches of each single-RHS kernel that fits the grid: symmetric packed
-    # (family 5), then the full-block LDS ring (family 3, 200-wide planes)
-    fams = [5, 3] if d.n[0] == 200 else [5]
+    # 5 launches of each single-RHS kernel that fits the grid: twisted packed
+    # (family 6, two phase launches), symmetric packed (family 5), then the
+    # full-block LDS ring (family 3, 200-wide planes)
+    fams = [6, 5, 3] if d.n[0] == 200 else [6, 5]
     saved = os.environ.get("HZ_SOLVE_KERNEL")
     for fam in fams:
-        os.environ["HZ_SOLVE_KERNEL"] = {5: "sym", 3: "dma"}[fam]
+        os.environ["HZ_SOLVE_KERNEL"] = {6: "twist", 5: "sym", 3: "dma"}[fam]
         for _ in range(5):
             A.check(A.lib().hz_plane_solve(S.h, A.PREC_C64, 0, 1, buf.data_ptr()), "hz_plane_solve")
         A.check(A.lib().hz_problem_sync(S.h), "sync")
@@ -570,7 +579,7 @@ def kernel_only(S, d, A, torch):
         A.check(A.lib().hz_plane_solve(S.h, A.PREC_C64, 0, 1, buf.data_ptr()), "hz_plane_solve")
     A.check(A.lib().hz_problem_sync(S.h), "sync")
     print(json.dumps({"kernel_only": True, "copy_bytes": 2 * (1 << 30),
-                      "plane_solve_bytes": {str(f): plane_solve_bytes(d.n, f) for f in fams},
+                      "plane_solve_bytes": {str(f): plane_solve_bytes(d.n, f, S.slots()[0]) for f in fams},
                       "hodlr_record_bytes": info["bytes"]}), flush=True)
 
 

@@ -140,7 +140,9 @@ enum {
   HZ_SOLVE_CHUNKED = 2, /* chunked GEMV kernel, any width */
   HZ_SOLVE_STREAM_1 = 3, /* single-RHS ring with an LDS cross-group reduction (200-wide planes) */
   HZ_SOLVE_STREAM_MFMA = 4, /* streaming solve with the 8-RHS block product on FP32 MFMA (complex64) */
-  HZ_SOLVE_SYM = 5          /* symmetric packed factors (upper block triangle of G_j D_j), one RHS per pass */
+  HZ_SOLVE_SYM = 5,         /* symmetric packed factors (upper block triangle of G_j D_j), one RHS per pass */
+  HZ_SOLVE_TWIST = 6        /* twisted packed factors (chains from both ends meeting at y-line ny/2), the
+                               two planes of each factor slot per pass (complex64) */
 };
 typedef struct {
   int32_t solve_kernel; /* HZ_SOLVE_* */
@@ -171,6 +173,11 @@ int hz_problem_sync(hz_problem* p);
 void* hz_problem_stream(const hz_problem* p);
 /* dims[0..3] = nx, ny, nz, N; *factor_bytes = bytes held by the plane factors. */
 int hz_problem_info(const hz_problem* p, int64_t* dims, double* factor_bytes);
+/* *nslot = factor slots (distinct z symbols among the solved planes: planes k
+ * and nz - k share one), *nskip = planes that SAVE_MEM leaves unsolved.
+ * Replaces nothing in the reference (its plane loop factors every kept plane,
+ * FGMRES.cpp:196-206); reports what the plane map of hz_problem_create chose. */
+int hz_problem_slots(const hz_problem* p, int* nslot, int* nskip);
 
 /* ------------------------------------------------------- preconditioner */
 /* Factor every kz plane M_k = D_xy + kappa - kz_k^2 as a block-tridiagonal

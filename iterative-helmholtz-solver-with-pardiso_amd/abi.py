@@ -10,7 +10,7 @@ HZ_OK = 0
 ERRORS = {1: "HZ_ERR_ARG", 2: "HZ_ERR_HIP", 3: "HZ_ERR_STATE", 4: "HZ_ERR_NOMEM", 5: "HZ_ERR_NOT_CONVERGED",
           6: "HZ_ERR_BREAKDOWN", 7: "HZ_ERR_SINGULAR", 8: "HZ_ERR_NONFINITE"}
 PREC_FP64, PREC_C64 = 0, 1
-SOLVE_DEFAULT, SOLVE_STREAM, SOLVE_CHUNKED, SOLVE_STREAM_1, SOLVE_STREAM_MFMA, SOLVE_SYM = 0, 1, 2, 3, 4, 5
+SOLVE_DEFAULT, SOLVE_STREAM, SOLVE_CHUNKED, SOLVE_STREAM_1, SOLVE_STREAM_MFMA, SOLVE_SYM, SOLVE_TWIST = 0, 1, 2, 3, 4, 5, 6
 DRIVER_FGMRES, DRIVER_GMRES_REF = 0, 1
 ORTHO_REF_CONJ, ORTHO_MGS, ORTHO_MGS_SEQ = 0, 1, 2
 MASK_FULL, MASK_PHYS = 0, 1
@@ -112,6 +112,7 @@ SIGNATURES = {
     "hz_problem_sync": (_i, [_vp]),
     "hz_problem_stream": (_vp, [_vp]),
     "hz_problem_info": (_i, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
+    "hz_problem_slots": (_i, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "hz_precond_factor": (_i, [_vp, _i]),
     "hz_precond_factor_multi": (_i, [_i, C.POINTER(C.c_void_p), _i]),
     "hz_autotune": (_i, [_vp, _i, _i, C.POINTER(TuneInfo)]),

@@ -500,6 +500,14 @@ int hz_plane_solve(hz_problem* p, int prec, int adjoint, int nrhs, void* x) {
   HZ_API_END
 }
 
+int hz_problem_slots(const hz_problem* p, int* nslot, int* nskip) {
+  HZ_API_BEGIN
+  HZ_CHECK(p, HZ_ERR_ARG, "null problem");
+  if (nslot) *nslot = p->nslot;
+  if (nskip) *nskip = p->nskip;
+  HZ_API_END
+}
+
 int hz_problem_info(const hz_problem* p, int64_t* dims, double* factor_bytes) {
   HZ_API_BEGIN
   HZ_CHECK(p, HZ_ERR_ARG, "null problem");
@@ -507,7 +515,7 @@ int hz_problem_info(const hz_problem* p, int64_t* dims, double* factor_bytes) {
   if (factor_bytes) {
     double b = 0;
     for (int q = 0; q < 2; ++q)
-      b += (double)(p->fac[q].G.bytes + p->fac[q].Gsym.bytes + p->fac[q].hodlr.pool.bytes);
+      b += (double)(p->fac[q].G.bytes + p->fac[q].Gsym.bytes + p->fac[q].Gtw.bytes + p->fac[q].hodlr.pool.bytes);
     *factor_bytes = b;
   }
   HZ_API_END
@@ -560,7 +568,7 @@ int hz_precond_hodlr_info(const hz_problem* p, int prec, hz_hodlr_info* out) {
   out->eps = h.eps;
   out->mean_rank = h.mean_rank;
   out->bytes = h.bytes;
-  out->dense_bytes = (double)(f.G.bytes + f.Gsym.bytes);
+  out->dense_bytes = (double)(f.G.bytes + f.Gsym.bytes + f.Gtw.bytes);
   HZ_API_END
 }
 

@@ -100,8 +100,16 @@ struct Factors {
   // block columns paired J / T-1-J), for the widths it is built for
   bool sym_ready = false;
   DevBuf Gsym;
+  // twisted factors (complex64): blocks j < tw_m the top-down inverses, block
+  // tw_m the middle inverse, blocks j > tw_m the bottom-up inverses (Gb, dense,
+  // [slot][j - tw_m]), packed like Gsym into Gtw; tw_scr passes y_top, y_bot and
+  // f_m between the two phases (3 x TW_RHS x nz x nx complex64)
+  bool tw_ready = false;
+  int tw_m = 0;
+  DevBuf Gb, Gtw, tw_scr;
   Hodlr hodlr;
 };
+constexpr int TW_RHS = 64;  // right-hand sides per twisted solve launch
 
 }  // namespace hz
 
@@ -123,6 +131,10 @@ struct hz_problem {
   int nslot = 0, nskip = 0;
   hz::DevBuf d_fmap;     // int[nz]
   hz::DevBuf d_splane;   // int[nslot]
+  // the planes of each slot (int2[nslot]: the first plane, the second or -1);
+  // slot_pairs: no slot holds more than two planes
+  hz::DevBuf d_slotp;
+  bool slot_pairs = false;
   std::vector<double2> kappa;                  // (nx*ny) (w/cb)^2 (1 + i beta)
   std::vector<double2> c0;                     // (nx*ny) background speed
 
@@ -252,6 +264,8 @@ void launch_plane_solve_with(hz_problem* p, int prec, int nrhs, void* x, bool ad
 bool stream_solve_available(int prec, int nx);
 // whether the symmetric packed streaming kernel is instantiated for this width/precision
 bool sym_solve_available(int prec, int nx);
+// whether the twisted two-plane packed solve is built for this problem and precision
+bool twist_solve_available(const hz_problem* p, int prec);
 // whether the 8-RHS MFMA streaming kernel is instantiated for this width/precision
 bool mfma_solve_available(int prec, int nx);
 // whether the single-RHS ring kernel is instantiated for this width/precision

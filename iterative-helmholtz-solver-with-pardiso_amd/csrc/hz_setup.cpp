@@ -196,6 +196,19 @@ static void build_plane_map(hz_problem* p) {
   if (p->nslot > 0) {
     p->d_splane.alloc(p->nslot * sizeof(int));
     HZ_HIP(hipMemcpy(p->d_splane.ptr, p->splane.data(), p->nslot * sizeof(int), hipMemcpyHostToDevice));
+    std::vector<int2> sp(p->nslot, int2{-1, -1});
+    p->slot_pairs = true;
+    for (int64_t k = 0; k < p->nz; ++k) {
+      const int s = p->fmap[k];
+      if (s < 0) continue;
+      if (sp[s].x < 0) sp[s].x = (int)k;
+      else if (sp[s].y < 0) sp[s].y = (int)k;
+      else p->slot_pairs = false;
+    }
+    p->d_slotp.alloc(p->nslot * sizeof(int2));
+    HZ_HIP(hipMemcpy(p->d_slotp.ptr, sp.data(), p->nslot * sizeof(int2), hipMemcpyHostToDevice));
+  } else {
+    p->slot_pairs = false;
   }
 }
 

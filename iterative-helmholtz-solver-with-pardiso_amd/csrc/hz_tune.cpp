@@ -114,6 +114,11 @@ void autotune(hz_problem* p, int prec, int nrhs, hz_tune_info* info) {
       t.solve_family = HZ_SOLVE_SYM;
       cand.push_back(t);
     }
+    if (nrhs < 8 && twist_solve_available(p, prec)) {
+      Tuning t;
+      t.solve_family = HZ_SOLVE_TWIST;
+      cand.push_back(t);
+    }
     if (nrhs >= 8 && mfma_solve_available(prec, (int)p->nx)) {
       Tuning t;
       t.solve_family = HZ_SOLVE_STREAM_MFMA;

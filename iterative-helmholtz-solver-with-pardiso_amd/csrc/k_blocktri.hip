@@ -74,10 +74,19 @@ constexpr int TCM = 64;  // columns per trailing-update chunk on the matrix core
 constexpr int TC = 32;   // columns per trailing-update tile
 constexpr int CB = 4;    // columns per thread in the trailing update
 
+// mode 0: the top-down chain G_j = (D_j - sig_j G_{j-1})^{-1}, j = 0 .. ny-1,
+//         into G ([slot][j]), one workgroup per slot.
+// mode 1: the same in workgroups 0 .. nslot-1 and, in workgroups nslot ..
+//         2 nslot-1, the bottom-up chain of the twisted factors
+//         B_j = (D_j - sig_{j+1} B_{j+1})^{-1}, j = ny-1 .. m+1, into Gb
+//         ([slot][j - m], ny - m blocks a slot).
+// mode 2: the middle block T_m = (D_m - sig_m G_{m-1} - sig_{m+1} B_{m+1})^{-1}
+//         into Gb ([slot][0]) from both chains.
 template <typename T>
 __global__ __launch_bounds__(FACT_THREADS) void k_factor(typename Cx<T>::t* __restrict__ G, PlaneTabs tb, int n,
                                                          int ny, int nb, int* __restrict__ status,
-                                                         const int* __restrict__ splane) {
+                                                         const int* __restrict__ splane, int mode, int m,
+                                                         typename Cx<T>::t* __restrict__ Gb) {
   typedef typename Cx<T>::t C;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C* P = reinterpret_cast<C*>(smem_raw);     // nb * n, column-major panel
@@ -87,7 +96,11 @@ __global__ __launch_bounds__(FACT_THREADS) void k_factor(typename Cx<T>::t* __re
   int* piv = reinterpret_cast<int*>(pcol + n);  // n
   int* sh = piv + n;                         // scratch ints
 
-  const int k = splane[blockIdx.x];  // the plane whose z symbol this factor slot carries
+  const int nsl = mode == 1 ? (int)gridDim.x / 2 : (int)gridDim.x;
+  const bool up = mode == 1 && (int)blockIdx.x >= nsl;
+  const int slot = up ? (int)blockIdx.x - nsl : (int)blockIdx.x;
+  const bool rev = up || mode == 2;
+  const int k = splane[slot];  // the plane whose z symbol this factor slot carries
   const int tid = threadIdx.x;
   const int nt = blockDim.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -100,19 +113,32 @@ __global__ __launch_bounds__(FACT_THREADS) void k_factor(typename Cx<T>::t* __re
   }
   __syncthreads();
 
-  for (int j = 0; j < ny; ++j) {
-    C* Gj = G + ((size_t)blockIdx.x * ny + j) * nn;
-    const C* Gp = Gj - nn;
-    // ---- S_j = D_j - sig_j G_{j-1}
-    const double2 sg = tb.sig[j];
+  const int nblk = mode == 2 ? 1 : (up ? ny - 1 - m : ny);
+  C* const Gs = rev ? Gb + (size_t)slot * (ny - m) * nn : G + (size_t)slot * ny * nn;
+  for (int jj = 0; jj < nblk; ++jj) {
+    const int j = mode == 2 ? m : (up ? ny - 1 - jj : jj);
+    C* Gj = Gs + (size_t)(rev ? j - m : j) * nn;
+    // ---- S_j = D_j - sig_j G_{j-1} (bottom-up: - sig_{j+1} B_{j+1}; the
+    //      middle block both)
+    const bool hasp = rev ? j + 1 < ny : j > 0;
+    const C* Gp = rev ? Gj + nn : Gj - nn;
+    const double2 sg = hasp ? tb.sig[rev ? j + 1 : j] : make_double2(0, 0);
+    const bool mid = mode == 2 && m > 0;
+    const C* Gm = mid ? G + ((size_t)slot * ny + (m - 1)) * nn : nullptr;
+    const double2 sgm = mid ? tb.sig[m] : make_double2(0, 0);
     const double2 cdy = tb.cdy[j];
     for (size_t idx = tid; idx < nn; idx += nt) {
       const int r = (int)(idx % n), c = (int)(idx / n);
       double vx = 0.0, vy = 0.0;
-      if (j > 0) {
+      if (hasp) {
         const C g = Gp[idx];
         vx = -(sg.x * (double)g.x - sg.y * (double)g.y);
         vy = -(sg.x * (double)g.y + sg.y * (double)g.x);
+      }
+      if (mid) {
+        const C g = Gm[idx];
+        vx -= sgm.x * (double)g.x - sgm.y * (double)g.y;
+        vy -= sgm.x * (double)g.y + sgm.y * (double)g.x;
       }
       if (r == c) {
         const double2 kp = tb.kappa[r + (size_t)n * j];
@@ -1331,11 +1357,12 @@ struct SymLayout {
   static_assert(NX % TB == 0, "tile split");
 };
 
-// Gsym[(k ny + j) BS + ...] = packed G_j D_j of plane k
+// P[(s ny + j) BS + ...] = packed X_j D_j of factor slot s, X_j = G_j; with
+// G2 (the twisted factors) X_j = G2[s][j - m] for j >= m
 template <typename C, int NX, int TB>
 __global__ __launch_bounds__(256) void k_pack_sym(const C* __restrict__ G, C* __restrict__ P,
                                                   const double2* __restrict__ dx, const double2* __restrict__ dy,
-                                                  int ny) {
+                                                  int ny, const C* __restrict__ G2, int m) {
   typedef SymLayout<NX, TB> L;
   const int blk = blockIdx.y;
   const int j = blk % ny;
@@ -1344,7 +1371,7 @@ __global__ __launch_bounds__(256) void k_pack_sym(const C* __restrict__ G, C* __
   const bool two = J2 != J1;
   const int H1 = (J1 + 1) * TB, H2 = (J2 + 1) * TB;
   const int sz = H1 * TB + (two ? H2 * TB : 0);
-  const C* Gb = G + (size_t)blk * NX * NX;
+  const C* Gb = (G2 && j >= m) ? G2 + ((size_t)(blk / ny) * (ny - m) + (j - m)) * NX * NX : G + (size_t)blk * NX * NX;
   C* Pb = P + (size_t)blk * L::BS + (size_t)t * L::PAIR;
   const double2 dyj = dy[j];
   for (int e = threadIdx.x; e < sz; e += blockDim.x) {
@@ -1748,6 +1775,361 @@ __global__ __launch_bounds__(dma::NT) void k_solve_sym2(const float2* __restrict
         __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
       }
       slot = (slot + 1) & (NS - 1);
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(dma::wait_vm(0));
+}
+
+// ---- twisted packed solve of both planes of a factor slot
+// Planes k and nz - k share a factor slot.  The twisted factors split every
+// slot's chain at m: blocks j < m hold the top-down inverses G_j, block m
+// T_m = (D_m - sig_m G_{m-1} - sig_{m+1} B_{m+1})^{-1} and blocks j > m the
+// bottom-up inverses B_j = (D_j - sig_{j+1} B_{j+1})^{-1} (sig_j = cm_y(j)
+// cp_y(j-1)), all packed like the forward factors.  Phase 0 (forward):
+//   top:    t_0 = f_0, t_j = f_j - cm_y(j) G_{j-1} t_{j-1} (0 < j < m), y_top = G_{m-1} t_{m-1}
+//   bottom: b_{ny-1} = f_{ny-1}, b_j = f_j - cp_y(j) B_{j+1} b_{j+1} (m < j < ny-1), y_bot = B_{m+1} b_{m+1}
+// Phase 1 (backward):
+//   x_m = T_m (f_m - cm_y(m) y_top - cp_y(m) y_bot),
+//   top x_j = G_j (t_j - cp_y(j) x_{j+1}) down to j = 0, bottom x_j = B_j (b_j - cm_y(j) x_{j-1}) up to ny-1.
+// One workgroup per (slot, half) carries both planes of the slot: every packed
+// tile is read once for two right-hand sides and 2 nslot workgroups walk
+// chains of about ny/2 blocks (the one-plane kernel: nz workgroups, 2 ny - 1
+// blocks).  y_top, y_bot and a copy of f_m pass between the phases in scr
+// (phase 1 writes x_m into line m, which the other half reads as f_m).
+// Tiles, ring and products as in k_solve_sym2, with the two planes' values
+// side by side; the transposed sums are reduced over the whole wave (row
+// sums, then the row swaps), so one partial per 64-row block remains.
+__device__ __forceinline__ float xor16_f(float x, int lane) {
+  const unsigned u = __builtin_bit_cast(unsigned, x);
+  const auto r = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+  return __builtin_bit_cast(float, (lane & 16) ? r[0] : r[1]);
+}
+__device__ __forceinline__ float xor32_f(float x, int lane) {
+  const unsigned u = __builtin_bit_cast(unsigned, x);
+  const auto r = __builtin_amdgcn_permlane32_swap(u, u, false, false);
+  return __builtin_bit_cast(float, (lane & 32) ? r[0] : r[1]);
+}
+// 16-lane row sums of 16 values per lane: lane l holds the row sum of x[l & 15]
+__device__ __forceinline__ float row16_sum16(const float* x, int lane) {
+  const bool b0 = lane & 1, b1 = lane & 2, b2 = lane & 4, b3 = lane & 8;
+  float w[8], y[4];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) w[i] = pair_step<0xB1>(x[2 * i], x[2 * i + 1], b0);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) y[i] = pair_step<0x4E>(w[2 * i], w[2 * i + 1], b1);
+  const float z0 = pair_step<0x124>(y[0], y[1], b2), z1 = pair_step<0x124>(y[2], y[3], b2);
+  return pair_step<0x128>(z0, z1, b3);
+}
+
+template <int NX, int PH>
+__global__ __launch_bounds__(dma::NT) void k_solve_twist(const float2* __restrict__ P, float2* __restrict__ X,
+                                                         const double2* __restrict__ cmy,
+                                                         const double2* __restrict__ cpy,
+                                                         const double2* __restrict__ symd, int ny, int m, int nz,
+                                                         int64_t n2, int64_t N, const int2* __restrict__ slotp,
+                                                         float2* __restrict__ scr) {
+  typedef float T;
+  typedef float2 C;
+  typedef float4 U;
+  constexpr int RPL = 2, NCG = 4, TB = 8;
+  typedef SymShape<T, RPL, NCG, TB, NX> S_;
+  typedef SymLayout<NX, TB> L;
+  constexpr int n = S_::n, nru = S_::nru, CPB = S_::CPB, R = S_::R, PU = S_::PU;
+  constexpr int TPS = L::TPS, NPAIR = L::NPAIR, PAIR = L::PAIR, BS = L::BS, T_ = L::T;
+  constexpr int NS = S_::S;
+  constexpr int ZU = R * dma::NT;
+  constexpr int SU = ZU + 1;
+  constexpr int NQ2 = dma::NT / NCG / 64;  // one transposed partial per 64-row block
+  static_assert((NS & (NS - 1)) == 0, "ring slots: power of two");
+  static_assert(PU <= 128, "line DMA of two waves per plane");
+  static_assert((size_t)NS * SU * 16 + 2 * NCG * n * 8 + 2 * NQ2 * n * 8 + 3 * n * 8 + 5 * 64 * 16 <= 160 * 1024,
+                "LDS budget");
+  __shared__ U ring[NS * SU];
+  __shared__ C part[2][NCG * n];
+  __shared__ __align__(16) T tdot[2][NQ2 * n * 2];
+  __shared__ __align__(16) C v[2][n];
+  __shared__ U pbuf[5 * 64];  // lines of planes A (units 0..127) and B (128..255); coefficients at 256..319
+  __shared__ C sdxi[n];
+  const double2* pscal = reinterpret_cast<const double2*>(&pbuf[256]);
+
+  const int slot = blockIdx.x >> 1, half = blockIdx.x & 1;
+  const int2 sp = slotp[slot];
+  const bool twoB = sp.y >= 0;
+  const int64_t rhs = blockIdx.y;
+  const int nrhs = gridDim.y;
+  C* xs[2];
+  xs[0] = X + rhs * N + (int64_t)sp.x * n2;
+  xs[1] = X + rhs * N + (int64_t)(twoB ? sp.y : sp.x) * n2;
+  // scr: [what][rhs][plane][n], what = 0 y_top, 1 y_bot, 2 f_m
+  auto scr_at = [&](int what, int s) {
+    return scr + (((int64_t)what * nrhs + rhs) * nz + (s ? (twoB ? sp.y : sp.x) : sp.x)) * n;
+  };
+  const C* Pk = P + (size_t)slot * ny * BS;
+  const double2* dxi = symd + n + ny;
+  const double2* dyi = symd + 2 * n + ny;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int cg = wv % NCG, rb = (wv / NCG) * 64, ru = rb + lane;
+  const bool worker = ru < nru;
+  const int q2 = rb >> 6;
+  const int nsteps = PH == 0 ? (half == 0 ? m : ny - 1 - m) : (half == 0 ? m + 1 : ny - m);
+  auto blk_of = [&](int st) {
+    if (PH == 0) return half == 0 ? st : ny - 1 - st;
+    return half == 0 ? m - st : m + st;
+  };
+  // block whose vector the update after step st forms (-1: none; -2: y goes to scr)
+  auto next_of = [&](int st) {
+    const int jb = blk_of(st);
+    if (PH == 0) {
+      if (half == 0) return jb + 1 < m ? jb + 1 : -2;
+      return jb - 1 > m ? jb - 1 : -2;
+    }
+    if (half == 0) return jb > 0 ? jb - 1 : -1;
+    return jb < ny - 1 ? jb + 1 : -1;
+  };
+  const unsigned ring_lds = (unsigned)(size_t)(&ring[0]);
+  const unsigned pbuf_lds = (unsigned)(size_t)(&pbuf[0]);
+
+  if (PH == 0 && half == 0) {
+    // f_m for phase 1 (phase 1 overwrites line m with x_m)
+    for (int i = tid; i < 2 * n; i += dma::NT) {
+      const int s = i / n, c = i - s * n;
+      if (s == 0 || twoB) scr_at(2, s)[c] = xs[s][(size_t)m * n + c];
+    }
+  }
+  if (nsteps <= 0) return;
+  for (int i = tid; i < 2 * NQ2 * n * 2; i += dma::NT) (&tdot[0][0])[i] = 0;
+  if (tid < NS) ring[tid * SU + ZU] = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int i = tid; i < n; i += dma::NT) {
+    const double2 a = dxi[i];
+    sdxi[i] = make_float2((T)a.x, (T)a.y);
+  }
+  __syncthreads();
+  {
+    const int j0 = blk_of(0);
+    const double2 dy0 = dyi[j0];
+    for (int i = tid; i < 2 * n; i += dma::NT) {
+      const int s = i / n, c = i - s * n;
+      double fx, fy;
+      if (PH == 0) {
+        const C f = xs[s][(size_t)j0 * n + c];
+        fx = f.x;
+        fy = f.y;
+      } else {
+        // f_m - cm_y(m) y_top - cp_y(m) y_bot
+        const C f = scr_at(2, s)[c];
+        fx = f.x;
+        fy = f.y;
+        if (m > 0) {
+          const C yt = scr_at(0, s)[c];
+          const double2 cm = cmy[m];
+          fx -= cm.x * yt.x - cm.y * yt.y;
+          fy -= cm.x * yt.y + cm.y * yt.x;
+        }
+        if (m < ny - 1) {
+          const C yb = scr_at(1, s)[c];
+          const double2 cp = cpy[m];
+          fx -= cp.x * yb.x - cp.y * yb.y;
+          fy -= cp.x * yb.y + cp.y * yb.x;
+        }
+      }
+      const C a = sdxi[c];
+      const double dx = (double)a.x * dy0.x - (double)a.y * dy0.y, dy = (double)a.x * dy0.y + (double)a.y * dy0.x;
+      v[s][c] = make_float2((T)(fx * dx - fy * dy), (T)(fx * dy + fy * dx));
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
+
+  auto tile_dma = [&](const char* blk, int tt, int sl) {
+    const int tu = (tt < NPAIR ? PAIR : (NPAIR + 1) * TB * TB) / RPL;
+    const char* base = blk + (size_t)tt * PAIR * sizeof(C);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int u = min(r * dma::NT + tid, tu - 1);
+      dma16_untracked(base + (size_t)u * 16, ring_lds + (unsigned)((sl * SU + r * dma::NT + wv * 64) * 16));
+    }
+  };
+  auto blk_addr = [&](int st) {
+    return reinterpret_cast<const char*>(Pk + (size_t)blk_of(st < nsteps ? st : 0) * BS);
+  };
+#pragma unroll
+  for (int g = 0; g < NS - 1; ++g) tile_dma(blk_addr(0), g, g);
+
+  T acc[2][2 * RPL];
+  U vr[2];
+  int rslot = 0;
+  for (int st = 0; st < nsteps; ++st) {
+    const char* b0 = blk_addr(st);
+    const char* b1 = blk_addr(st + 1);
+    const int jb = blk_of(st), jn = next_of(st);
+#pragma unroll
+    for (int t = 0; t < TPS; ++t) {
+      __builtin_amdgcn_s_waitcnt(dma::wait_vm((NS - 2) * R));
+      __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
+      dma::raw_barrier();
+      const U* SL = ring + rslot * SU;
+      if (t == 0) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+#pragma unroll
+          for (int i = 0; i < 2 * RPL; ++i) acc[s][i] = 0;
+          vr[s] = reinterpret_cast<const U*>(v[s])[worker ? ru : 0];
+          if (!worker) vr[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        // the lines of this step's update (waves 0-1 plane A, 2-3 plane B)
+        // and its coefficients (wave 4: coupling, 1/dy of the next block)
+        if (jn >= 0) {
+          if (wv < 4) {
+            const int s = wv >> 1;
+            const char* lb = reinterpret_cast<const char*>(xs[s] + (size_t)jn * n);
+            dma16_untracked(lb + (size_t)min((wv & 1) * 64 + lane, PU - 1) * 16,
+                            pbuf_lds + (unsigned)(wv * 64 * 16));
+          } else if (wv == 4) {
+            const double2* src;
+            if (lane == 1) src = dyi + jn;
+            else src = (PH == 0 ? (half == 0) : (half != 0)) ? cmy + jn : cpy + jn;
+            dma16_untracked(src, pbuf_lds + 256 * 16);
+          }
+        }
+      }
+      {
+        const int J1 = t, J2 = T_ - 1 - t;
+        const int Hu1 = (J1 + 1) * TB / RPL, Hu2 = (J2 + 1) * TB / RPL;
+        const int nu1 = J1 * TB / RPL, nu2 = J2 * TB / RPL;
+        const int ub2 = (J1 + 1) * TB * TB / RPL;
+        const bool two = J2 != J1;
+        const C zc = make_float2(0.f, 0.f);
+        if (rb < Hu1) {
+          const bool ok1 = worker && ru < Hu1;
+          const bool ok2 = worker && two && ru < Hu2;
+          const int g1 = ok1 ? cg * CPB * Hu1 + ru : ZU;
+          const int g2 = ok2 ? ub2 + cg * CPB * Hu2 + ru : ZU;
+          const int st1 = ok1 ? Hu1 : 0, st2 = ok2 ? Hu2 : 0;
+          U a1[CPB], a2[CPB];
+#pragma unroll
+          for (int cc = 0; cc < CPB; ++cc) {
+            a1[cc] = SL[g1 + cc * st1];
+            a2[cc] = SL[g2 + cc * st2];
+          }
+          T tv[16];
+#pragma unroll
+          for (int i = 0; i < 16; ++i) tv[i] = 0;
+#pragma unroll
+          for (int s = 0; s < 2; ++s) {
+            const C* vrow = reinterpret_cast<const C*>(&vr[s]);
+            const C* v1 = &v[s][J1 * TB + cg * CPB];
+            const C* v2 = &v[s][J2 * TB + cg * CPB];
+            C w1[RPL], w2[RPL];
+#pragma unroll
+            for (int i = 0; i < RPL; ++i) {
+              w1[i] = ru < nu1 ? vrow[i] : zc;
+              w2[i] = ru < nu2 ? vrow[i] : zc;
+            }
+#pragma unroll
+            for (int cc = 0; cc < CPB; ++cc) {
+              unit_fma_fused<T, RPL>(a1[cc], v1[cc], acc[s]);
+              unit_fma_fused<T, RPL>(a2[cc], v2[cc], acc[s]);
+              HZ_SY_UNIT_DOT(a1[cc], w1, &tv[8 * s + 2 * cc])
+              HZ_SY_UNIT_DOT(a2[cc], w2, &tv[8 * s + 4 + 2 * cc])
+            }
+          }
+          T sm = row16_sum16(tv, lane);
+          sm += xor16_f(sm, lane);
+          sm += xor32_f(sm, lane);
+          const int kv = lane & 15, s = kv >> 3, kk = kv & 7;
+          if (lane < 16 && (two || kk < 4)) {
+            const int col = ((kk & 4) ? J2 : J1) * TB + cg * CPB + ((kk >> 1) & 1);
+            tdot[s][(q2 * n + col) * 2 + (kk & 1)] = sm;
+          }
+        } else if (rb < Hu2 && two) {
+          const bool ok2 = worker && ru < Hu2;
+          const int g2 = ok2 ? ub2 + cg * CPB * Hu2 + ru : ZU;
+          const int st2 = ok2 ? Hu2 : 0;
+          U a2[CPB];
+#pragma unroll
+          for (int cc = 0; cc < CPB; ++cc) a2[cc] = SL[g2 + cc * st2];
+          T tv[8];
+#pragma unroll
+          for (int i = 0; i < 8; ++i) tv[i] = 0;
+#pragma unroll
+          for (int s = 0; s < 2; ++s) {
+            const C* vrow = reinterpret_cast<const C*>(&vr[s]);
+            const C* v2 = &v[s][J2 * TB + cg * CPB];
+            C w2[RPL];
+#pragma unroll
+            for (int i = 0; i < RPL; ++i) w2[i] = ru < nu2 ? vrow[i] : zc;
+#pragma unroll
+            for (int cc = 0; cc < CPB; ++cc) {
+              unit_fma_fused<T, RPL>(a2[cc], v2[cc], acc[s]);
+              HZ_SY_UNIT_DOT(a2[cc], w2, &tv[4 * s + 2 * cc])
+            }
+          }
+          T sm = row16_sum8(tv, lane);
+          sm += xor16_f(sm, lane);
+          sm += xor32_f(sm, lane);
+          const int kv = lane & 7, s = kv >> 2, kk = kv & 3;
+          if (lane < 8) {
+            const int col = J2 * TB + cg * CPB + (kk >> 1);
+            tdot[s][(q2 * n + col) * 2 + (kk & 1)] = sm;
+          }
+        }
+      }
+      {
+        // the tile NS - 1 stages ahead into the slot the previous stage read
+        const int ta = t + NS - 1;
+        const int tt = ta % TPS;
+        const char* blk = (ta / TPS) == 0 ? b0 : b1;
+        const bool past = (ta / TPS) != 0 && st + 1 >= nsteps;
+        tile_dma(past ? reinterpret_cast<const char*>(Pk) : blk, past ? 0 : tt, (rslot + NS - 1) & (NS - 1));
+      }
+      if (t == TPS - 1) {
+        if (worker) {
+#pragma unroll
+          for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int r = 0; r < RPL; ++r)
+              part[s][cg * n + ru * RPL + r] = make_float2(acc[s][2 * r], acc[s][2 * r + 1]);
+        }
+        __builtin_amdgcn_s_waitcnt(dma::wait_vm((NS - 1) * R));
+        __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
+        dma::raw_barrier();
+        if (tid < n) {
+          const C a = sdxi[tid];
+#pragma unroll
+          for (int s = 0; s < 2; ++s) {
+            if (s == 1 && !twoB) break;
+            C y = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int q = 0; q < NCG; ++q) {
+              const C u = part[s][q * n + tid];
+              y.x += u.x; y.y += u.y;
+            }
+#pragma unroll
+            for (int q = 0; q < NQ2; ++q) {
+              const C u = reinterpret_cast<const C*>(tdot[s])[q * n + tid];
+              y.x += u.x; y.y += u.y;
+            }
+            C* x = xs[s];
+            if (PH == 1 && (half == 0 || st > 0)) x[(size_t)jb * n + tid] = y;  // x_jb
+            if (jn == -2) {
+              scr_at(half, s)[tid] = y;  // y_top / y_bot
+            } else if (jn >= 0) {
+              const C pre = reinterpret_cast<const C*>(pbuf + 128 * s)[tid];
+              const double2 cf = pscal[0], b = pscal[1];
+              C nv;
+              nv.x = pre.x - (T)(cf.x * y.x - cf.y * y.y);
+              nv.y = pre.y - (T)(cf.x * y.y + cf.y * y.x);
+              if (PH == 0) x[(size_t)jn * n + tid] = nv;  // t_jn / b_jn
+              const double2 di =
+                  make_double2((double)a.x * b.x - (double)a.y * b.y, (double)a.x * b.y + (double)a.y * b.x);
+              v[s][tid] = make_float2((T)((double)nv.x * di.x - (double)nv.y * di.y),
+                                      (T)((double)nv.x * di.y + (double)nv.y * di.x));
+            }
+          }
+        }
+        __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
+      }
+      rslot = (rslot + 1) & (NS - 1);
     }
   }
   __builtin_amdgcn_s_waitcnt(dma::wait_vm(0));
@@ -2321,9 +2703,58 @@ bool try_pack_sym(hz_problem* p, int prec, hipStream_t s) {
   const double2* dy = dx + NX;
   if (p->nslot == 0) return true;
   hipLaunchKernelGGL((k_pack_sym<C, NX, TB>), dim3(L::TPS, (unsigned)(p->nslot * p->ny)), dim3(256), 0, s, f.G.as<C>(),
-                     f.Gsym.as<C>(), dx, dy, (int)p->ny);
+                     f.Gsym.as<C>(), dx, dy, (int)p->ny, (const C*)nullptr, 0);
   HZ_HIP(hipGetLastError());
   return true;
+}
+
+// the twisted factors (G for j < m, Gb from m on) packed like Gsym into Gtw
+template <int NX>
+bool try_pack_twist(hz_problem* p, hipStream_t s) {
+  if (p->nx != NX) return false;
+  typedef SymLayout<NX, 8> L;
+  Factors& f = p->fac[HZ_PREC_C64];
+  const size_t bytes = (size_t)p->nslot * p->ny * L::BS * sizeof(float2);
+  if (f.Gtw.bytes != bytes) {
+    f.Gtw.release();
+    f.Gtw.alloc(bytes);
+  }
+  const double2* dx = p->d_symd.as<double2>();
+  hipLaunchKernelGGL((k_pack_sym<float2, NX, 8>), dim3(L::TPS, (unsigned)(p->nslot * p->ny)), dim3(256), 0, s,
+                     f.G.as<float2>(), f.Gtw.as<float2>(), dx, dx + NX, (int)p->ny, f.Gb.as<float2>(), f.tw_m);
+  HZ_HIP(hipGetLastError());
+  return true;
+}
+
+bool pack_twist_any(hz_problem* p, hipStream_t s) {
+  return try_pack_twist<200>(p, s) || try_pack_twist<192>(p, s) || try_pack_twist<160>(p, s) ||
+         try_pack_twist<128>(p, s);
+}
+
+// both phases of the twisted solve, TW_RHS right-hand sides per launch pair
+template <int NX>
+bool try_solve_twist(hz_problem* p, int nrhs, void* x, hipStream_t s) {
+  if (p->nx != NX) return false;
+  const Factors& f = p->fac[HZ_PREC_C64];
+  const float2* P = f.Gtw.as<float2>();
+  const int2* sp = p->d_slotp.as<int2>();
+  float2* scr = f.tw_scr.as<float2>();
+  for (int r0 = 0; r0 < nrhs; r0 += TW_RHS) {
+    const int nr = std::min(TW_RHS, nrhs - r0);
+    float2* xr = reinterpret_cast<float2*>(x) + (size_t)r0 * p->N;
+    const dim3 grid((unsigned)(2 * p->nslot), (unsigned)nr);
+    hipLaunchKernelGGL((k_solve_twist<NX, 0>), grid, dim3(dma::NT), 0, s, P, xr, p->tab(1, 0), p->tab(1, 1),
+                       p->d_symd.as<double2>(), (int)p->ny, f.tw_m, (int)p->nz, p->n2, p->N, sp, scr);
+    hipLaunchKernelGGL((k_solve_twist<NX, 1>), grid, dim3(dma::NT), 0, s, P, xr, p->tab(1, 0), p->tab(1, 1),
+                       p->d_symd.as<double2>(), (int)p->ny, f.tw_m, (int)p->nz, p->n2, p->N, sp, scr);
+    HZ_HIP(hipGetLastError());
+  }
+  return true;
+}
+
+bool solve_twist_any(hz_problem* p, int nrhs, void* x, hipStream_t s) {
+  return try_solve_twist<200>(p, nrhs, x, s) || try_solve_twist<192>(p, nrhs, x, s) ||
+         try_solve_twist<160>(p, nrhs, x, s) || try_solve_twist<128>(p, nrhs, x, s);
 }
 
 template <int NX>
@@ -2407,9 +2838,9 @@ bool try_launch_mr(int n, int ny, int nz, int nrhs, int cap, hipStream_t s, cons
   return true;
 }
 
-// Plane-solve kernel family: HZ_SOLVE_KERNEL=mr|dma|fast|mfma|sym forces one for
+// Plane-solve kernel family: HZ_SOLVE_KERNEL=mr|dma|fast|mfma|mfma_sym|sym|twist forces one for
 // timing; otherwise the tuned (or default) choice applies.
-enum SolveKernel { SK_DEFAULT, SK_MR, SK_DMA, SK_FAST, SK_MFMA, SK_SYM, SK_MFMA_SYM };
+enum SolveKernel { SK_DEFAULT, SK_MR, SK_DMA, SK_FAST, SK_MFMA, SK_SYM, SK_MFMA_SYM, SK_TWIST };
 SolveKernel solve_kernel_choice() {
   const char* v = getenv("HZ_SOLVE_KERNEL");
   if (!v || !v[0]) return SK_DEFAULT;
@@ -2419,6 +2850,7 @@ SolveKernel solve_kernel_choice() {
   if (!strcmp(v, "mfma")) return SK_MFMA;
   if (!strcmp(v, "mfma_sym")) return SK_MFMA_SYM;
   if (!strcmp(v, "sym")) return SK_SYM;
+  if (!strcmp(v, "twist")) return SK_TWIST;
   return SK_DEFAULT;
 }
 
@@ -2452,6 +2884,10 @@ bool launch_mr_any(int prec, int n, int ny, int nz, int nrhs, int cap, hipStream
 
 bool sym_solve_available(int prec, int nx) {
   return (prec == HZ_PREC_C64 || prec == HZ_PREC_FP64) && (nx == 200 || nx == 192 || nx == 160 || nx == 128);
+}
+
+bool twist_solve_available(const hz_problem* p, int prec) {
+  return prec == HZ_PREC_C64 && p->fac[prec].tw_ready;
 }
 
 bool mfma_solve_available(int prec, int nx) {
@@ -2499,7 +2935,24 @@ void factor_begin(hz_problem* p, int prec, hipStream_t s) {
   Factors& f = p->fac[prec];
   f.ready = false;
   f.sym_ready = false;
+  f.tw_ready = false;
   f.hodlr.release();
+  // twisted factors (complex64, the packed widths, at most two planes a slot):
+  // the bottom-up chain runs beside the top-down one in the same launch
+  const bool twist = prec == HZ_PREC_C64 && ny >= 2 && p->slot_pairs && sym_solve_available(prec, n) &&
+                     !getenv("HZ_NO_TWIST") && !getenv("HZ_NO_SYM_PACK");
+  f.tw_m = twist ? ny / 2 : 0;
+  if (twist) {
+    const size_t gb = (size_t)p->nslot * (ny - f.tw_m) * (size_t)n * n * sizeof(float2);
+    if (f.Gb.bytes != gb) {
+      f.Gb.release();
+      f.Gb.alloc(gb);
+    }
+  } else {
+    f.Gb.release();
+    f.Gtw.release();
+    f.tw_scr.release();
+  }
   f.elem = prec == HZ_PREC_FP64 ? sizeof(double2) : sizeof(float2);
   const size_t bytes = (size_t)p->nslot * ny * (size_t)n * n * f.elem;
   if (f.G.bytes != bytes) {
@@ -2522,15 +2975,20 @@ void factor_begin(hz_problem* p, int prec, hipStream_t s) {
     set_max_lds((const void*)&k_factor<double>, lds);
     if (p->nslot > 0)
       hipLaunchKernelGGL(k_factor<double>, dim3((unsigned)p->nslot), dim3(FACT_THREADS), lds, s, f.G.as<double2>(),
-                         tb, n, ny, nb, st, p->d_splane.as<int>());
+                         tb, n, ny, nb, st, p->d_splane.as<int>(), 0, 0, (double2*)nullptr);
   } else {
     while (nb > 4 && factor_lds<float>(n, nb) > 160 * 1024) nb /= 2;
     const size_t lds = factor_lds<float>(n, nb);
     HZ_CHECK(lds <= 160 * 1024, HZ_ERR_ARG, "nx=%d too large for the LDS panel", n);
     set_max_lds((const void*)&k_factor<float>, lds);
-    if (p->nslot > 0)
-      hipLaunchKernelGGL(k_factor<float>, dim3((unsigned)p->nslot), dim3(FACT_THREADS), lds, s, f.G.as<float2>(), tb,
-                         n, ny, nb, st, p->d_splane.as<int>());
+    if (p->nslot > 0) {
+      hipLaunchKernelGGL(k_factor<float>, dim3((unsigned)(twist ? 2 * p->nslot : p->nslot)), dim3(FACT_THREADS), lds,
+                         s, f.G.as<float2>(), tb, n, ny, nb, st, p->d_splane.as<int>(), twist ? 1 : 0, f.tw_m,
+                         f.Gb.as<float2>());
+      if (twist)
+        hipLaunchKernelGGL(k_factor<float>, dim3((unsigned)p->nslot), dim3(FACT_THREADS), lds, s, f.G.as<float2>(), tb,
+                           n, ny, nb, st, p->d_splane.as<int>(), 2, f.tw_m, f.Gb.as<float2>());
+    }
   }
   HZ_HIP(hipGetLastError());
   HZ_HIP(hipMemcpyAsync(p->pin_misc.ptr, st, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -2545,6 +3003,18 @@ void factor_end(hz_problem* p, int prec, hipStream_t s) {
   if (sym_solve_available(prec, (int)p->nx) && !getenv("HZ_NO_SYM_PACK")) {
     f.sym_ready = pack_sym_any(p, prec, s);
     HZ_HIP(hipStreamSynchronize(s));
+  }
+  // twisted packed copy; the unpacked bottom-up blocks are not kept
+  f.tw_ready = false;
+  if (f.Gb.ptr && p->nslot > 0) {
+    f.tw_ready = pack_twist_any(p, s);
+    const size_t sb = (size_t)3 * TW_RHS * p->nz * p->nx * sizeof(float2);
+    if (f.tw_scr.bytes != sb) {
+      f.tw_scr.release();
+      f.tw_scr.alloc(sb);
+    }
+    HZ_HIP(hipStreamSynchronize(s));
+    f.Gb.release();
   }
   f.ready = true;
 }
@@ -2615,7 +3085,7 @@ void plane_solve_kernels(hz_problem* p, int prec, int nrhs, void* x, bool adjoin
   // packed symmetric factors with G_j^H = conj(D_j^-1 P_j conj(.)), unless the
   // environment asks for another kernel family
   if (adjoint && ny >= 1 && prec == HZ_PREC_C64 && reinterpret_cast<uintptr_t>(x) % 16 == 0 &&
-      (sk == SK_DEFAULT || sk == SK_SYM || sk == SK_MFMA)) {
+      (sk == SK_DEFAULT || sk == SK_SYM || sk == SK_MFMA || sk == SK_TWIST)) {
     if (sk != SK_SYM && nrhs >= 8 && mfma_solve_available(prec, n)) {
       const int g = nrhs / 8;
       launch_mfma8_any(n, ny, (int)p->nz, g, s, f.G.ptr, x, cmy, cpy, p->d_symd.as<double2>(), true, p->n2, p->N,
@@ -2632,8 +3102,15 @@ void plane_solve_kernels(hz_problem* p, int prec, int nrhs, void* x, bool adjoin
   if (sk == SK_DEFAULT && t.solve_family == HZ_SOLVE_STREAM_1) sk = SK_DMA;
   if (sk == SK_DEFAULT && t.solve_family == HZ_SOLVE_STREAM_MFMA) sk = SK_MFMA;
   if (sk == SK_DEFAULT && t.solve_family == HZ_SOLVE_SYM) sk = SK_SYM;
-  // untuned default for one right-hand side: the symmetric packed stream where it is built
+  if (sk == SK_DEFAULT && t.solve_family == HZ_SOLVE_TWIST) sk = SK_TWIST;
+  // untuned default for one right-hand side: the twisted two-plane packed
+  // solve, else the symmetric packed stream, where they are built
+  if (sk == SK_DEFAULT && t.solve_family == HZ_SOLVE_DEFAULT && nrhs == 1 && f.tw_ready) sk = SK_TWIST;
   if (sk == SK_DEFAULT && t.solve_family == HZ_SOLVE_DEFAULT && nrhs == 1 && f.sym_ready) sk = SK_SYM;
+  if (!adjoint && sk == SK_TWIST && f.tw_ready && reinterpret_cast<uintptr_t>(x) % 16 == 0 &&
+      solve_twist_any(p, nrhs, x, s))
+    return;
+  if (sk == SK_TWIST) sk = SK_SYM;
   if (!adjoint && ny >= 1 && sk == SK_SYM && f.sym_ready && reinterpret_cast<uintptr_t>(x) % 16 == 0 &&
       solve_sym_any(p, prec, nrhs, x, false, s))
     return;

@@ -886,6 +886,9 @@ void hodlr_compress(hz_problem* p, int prec, double eps, int small, int flags) {
     f.G.release();
     f.Gsym.release();
     f.sym_ready = false;
+    f.Gtw.release();
+    f.tw_scr.release();
+    f.tw_ready = false;
   }
 }
 

@@ -96,6 +96,12 @@ class Solver:
         abi.check(abi.lib().hz_problem_info(self.h, dims, C.byref(b)), "hz_problem_info")
         return b.value
 
+    def slots(self):
+        """(factor slots, planes skipped by SAVE_MEM) of the plane map."""
+        ns, nk = C.c_int(), C.c_int()
+        abi.check(abi.lib().hz_problem_slots(self.h, C.byref(ns), C.byref(nk)), "hz_problem_slots")
+        return ns.value, nk.value
+
     # ---------------------------------------------------- preconditioner
     def factor(self, prec=PREC_FP64):
         self._run("hz_precond_factor", int(prec))

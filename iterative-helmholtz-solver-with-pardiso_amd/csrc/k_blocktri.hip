@@ -1780,6 +1780,17 @@ __global__ __launch_bounds__(dma::NT) void k_solve_sym2(const float2* __restrict
   __builtin_amdgcn_s_waitcnt(dma::wait_vm(0));
 }
 
+// lane l ^ 16 and lane l ^ 32 of a wave (gfx950 permlane swaps)
+__device__ __forceinline__ float xor16_f(float x, int lane) {
+  const unsigned u = __builtin_bit_cast(unsigned, x);
+  const auto r = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+  return __builtin_bit_cast(float, (lane & 16) ? r[0] : r[1]);
+}
+__device__ __forceinline__ float xor32_f(float x, int lane) {
+  const unsigned u = __builtin_bit_cast(unsigned, x);
+  const auto r = __builtin_amdgcn_permlane32_swap(u, u, false, false);
+  return __builtin_bit_cast(float, (lane & 32) ? r[0] : r[1]);
+}
 // ---- twisted packed solve of both planes of a factor slot
 // Planes k and nz - k share a factor slot.  The twisted factors split every
 // slot's chain at m: blocks j < m hold the top-down inverses G_j, block m
@@ -1796,61 +1807,55 @@ __global__ __launch_bounds__(dma::NT) void k_solve_sym2(const float2* __restrict
 // chains of about ny/2 blocks (the one-plane kernel: nz workgroups, 2 ny - 1
 // blocks).  y_top, y_bot and a copy of f_m pass between the phases in scr
 // (phase 1 writes x_m into line m, which the other half reads as f_m).
-// Tiles, ring and products as in k_solve_sym2, with the two planes' values
-// side by side; the transposed sums are reduced over the whole wave (row
-// sums, then the row swaps), so one partial per 64-row block remains.
-__device__ __forceinline__ float xor16_f(float x, int lane) {
-  const unsigned u = __builtin_bit_cast(unsigned, x);
-  const auto r = __builtin_amdgcn_permlane16_swap(u, u, false, false);
-  return __builtin_bit_cast(float, (lane & 16) ? r[0] : r[1]);
-}
-__device__ __forceinline__ float xor32_f(float x, int lane) {
-  const unsigned u = __builtin_bit_cast(unsigned, x);
-  const auto r = __builtin_amdgcn_permlane32_swap(u, u, false, false);
-  return __builtin_bit_cast(float, (lane & 32) ? r[0] : r[1]);
-}
-// 16-lane row sums of 16 values per lane: lane l holds the row sum of x[l & 15]
-__device__ __forceinline__ float row16_sum16(const float* x, int lane) {
-  const bool b0 = lane & 1, b1 = lane & 2, b2 = lane & 4, b3 = lane & 8;
-  float w[8], y[4];
-#pragma unroll
-  for (int i = 0; i < 8; ++i) w[i] = pair_step<0xB1>(x[2 * i], x[2 * i + 1], b0);
-#pragma unroll
-  for (int i = 0; i < 4; ++i) y[i] = pair_step<0x4E>(w[2 * i], w[2 * i + 1], b1);
-  const float z0 = pair_step<0x124>(y[0], y[1], b2), z1 = pair_step<0x124>(y[2], y[3], b2);
-  return pair_step<0x128>(z0, z1, b3);
-}
-
+// Tiles and ring as in k_solve_sym2.  Both products run on the matrix cores
+// (v_mfma_f32_4x4x1f32: 16 blocks of 4x4x1; D(lane 4b+j, reg i) +=
+// A(lane 4b+i) B(lane 4b+j)).  The four real columns of every product are the
+// two planes' real and imaginary parts, so no block column idles:
+//   direct, y_r += K(r, c) v'_c: lane l holds row unit rb + l (rows 2(rb+l)
+//     and 2(rb+l)+1); A = Re or Im K of one of those rows, B = the components
+//     of v'_c, so D(lane 4b+j, reg i) is row 2(rb+4b+i) (+1), component j,
+//     accumulated over the step's tiles;
+//   transposed, y_c += K(r, c) v'_r (rows above the diagonal tile): lane 4b+j
+//     holds (row unit rb+16k+b, column c_j) of the wave's four columns;
+//     A = the components of v'_r, B = Re or Im K, so D(lane 4b+j, reg i) sums
+//     block b's rows for column c_j, component i, and is reduced over the 16
+//     blocks at the end of the tile.
+// Components: 0, 1 = Re, Im of plane A; 2, 3 = Re, Im of plane B.  Im K
+// multiplies the rotated components (-Im A, Re A, -Im B, Re B).
 template <int NX, int PH>
-__global__ __launch_bounds__(dma::NT) void k_solve_twist(const float2* __restrict__ P, float2* __restrict__ X,
-                                                         const double2* __restrict__ cmy,
-                                                         const double2* __restrict__ cpy,
-                                                         const double2* __restrict__ symd, int ny, int m, int nz,
-                                                         int64_t n2, int64_t N, const int2* __restrict__ slotp,
-                                                         float2* __restrict__ scr) {
+__global__ __launch_bounds__(dma::NT) void k_solve_twist_mx(const float2* __restrict__ P, float2* __restrict__ X,
+                                                            const double2* __restrict__ cmy,
+                                                            const double2* __restrict__ cpy,
+                                                            const double2* __restrict__ symd, int ny, int m, int nz,
+                                                            int64_t n2, int64_t N, const int2* __restrict__ slotp,
+                                                            float2* __restrict__ scr) {
   typedef float T;
   typedef float2 C;
   typedef float4 U;
   constexpr int RPL = 2, NCG = 4, TB = 8;
   typedef SymShape<T, RPL, NCG, TB, NX> S_;
   typedef SymLayout<NX, TB> L;
-  constexpr int n = S_::n, nru = S_::nru, CPB = S_::CPB, R = S_::R, PU = S_::PU;
+  constexpr int n = S_::n, CPB = S_::CPB, R = S_::R, PU = S_::PU;
   constexpr int TPS = L::TPS, NPAIR = L::NPAIR, PAIR = L::PAIR, BS = L::BS, T_ = L::T;
   constexpr int NS = S_::S;
   constexpr int ZU = R * dma::NT;
   constexpr int SU = ZU + 1;
-  constexpr int NQ2 = dma::NT / NCG / 64;  // one transposed partial per 64-row block
+  constexpr int NQ2 = dma::NT / NCG / 64;  // row halves (one transposed partial each)
   static_assert((NS & (NS - 1)) == 0, "ring slots: power of two");
   static_assert(PU <= 128, "line DMA of two waves per plane");
-  static_assert((size_t)NS * SU * 16 + 2 * NCG * n * 8 + 2 * NQ2 * n * 8 + 3 * n * 8 + 5 * 64 * 16 <= 160 * 1024,
-                "LDS budget");
+  static_assert(CPB == 2 && NQ2 == 2, "four columns and 64 row units per wave");
+  static_assert((size_t)NS * SU * 16 + (NCG + NQ2 + 2) * n * 16 + 5 * 64 * 16 + n * 8 <= 160 * 1024, "LDS budget");
   __shared__ U ring[NS * SU];
-  __shared__ C part[2][NCG * n];
-  __shared__ __align__(16) T tdot[2][NQ2 * n * 2];
-  __shared__ __align__(16) C v[2][n];
-  __shared__ U pbuf[5 * 64];  // lines of planes A (units 0..127) and B (128..255); coefficients at 256..319
+  __shared__ U part[NCG * n];   // direct products per column group
+  __shared__ U tdot[NQ2 * n];   // transposed products per row half
+  __shared__ U vq[n], vqr[n];   // v' components and rotated components
+  __shared__ U pbuf[5 * 64];    // lines of planes A (units 0..127) and B (128..255); coefficients at 256..319
   __shared__ C sdxi[n];
   const double2* pscal = reinterpret_cast<const double2*>(&pbuf[256]);
+  const T* vqf = reinterpret_cast<const T*>(vq);
+  const T* vqrf = reinterpret_cast<const T*>(vqr);
+  T* tdf = reinterpret_cast<T*>(tdot);
+  T* pf = reinterpret_cast<T*>(part);
 
   const int slot = blockIdx.x >> 1, half = blockIdx.x & 1;
   const int2 sp = slotp[slot];
@@ -1870,8 +1875,8 @@ __global__ __launch_bounds__(dma::NT) void k_solve_twist(const float2* __restric
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int cg = wv % NCG, rb = (wv / NCG) * 64, ru = rb + lane;
-  const bool worker = ru < nru;
   const int q2 = rb >> 6;
+  const int bq = lane >> 2, jq = lane & 3;  // the lane's MFMA block and index in the block
   const int nsteps = PH == 0 ? (half == 0 ? m : ny - 1 - m) : (half == 0 ? m + 1 : ny - m);
   auto blk_of = [&](int st) {
     if (PH == 0) return half == 0 ? st : ny - 1 - st;
@@ -1898,7 +1903,7 @@ __global__ __launch_bounds__(dma::NT) void k_solve_twist(const float2* __restric
     }
   }
   if (nsteps <= 0) return;
-  for (int i = tid; i < 2 * NQ2 * n * 2; i += dma::NT) (&tdot[0][0])[i] = 0;
+  for (int i = tid; i < NQ2 * n; i += dma::NT) tdot[i] = make_float4(0.f, 0.f, 0.f, 0.f);
   if (tid < NS) ring[tid * SU + ZU] = make_float4(0.f, 0.f, 0.f, 0.f);
   for (int i = tid; i < n; i += dma::NT) {
     const double2 a = dxi[i];
@@ -1908,34 +1913,40 @@ __global__ __launch_bounds__(dma::NT) void k_solve_twist(const float2* __restric
   {
     const int j0 = blk_of(0);
     const double2 dy0 = dyi[j0];
-    for (int i = tid; i < 2 * n; i += dma::NT) {
-      const int s = i / n, c = i - s * n;
-      double fx, fy;
-      if (PH == 0) {
-        const C f = xs[s][(size_t)j0 * n + c];
-        fx = f.x;
-        fy = f.y;
-      } else {
-        // f_m - cm_y(m) y_top - cp_y(m) y_bot
-        const C f = scr_at(2, s)[c];
-        fx = f.x;
-        fy = f.y;
-        if (m > 0) {
-          const C yt = scr_at(0, s)[c];
-          const double2 cm = cmy[m];
-          fx -= cm.x * yt.x - cm.y * yt.y;
-          fy -= cm.x * yt.y + cm.y * yt.x;
+    for (int c = tid; c < n; c += dma::NT) {
+      T w[4];
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        double fx, fy;
+        if (PH == 0) {
+          const C f = xs[s][(size_t)j0 * n + c];
+          fx = f.x;
+          fy = f.y;
+        } else {
+          // f_m - cm_y(m) y_top - cp_y(m) y_bot
+          const C f = scr_at(2, s)[c];
+          fx = f.x;
+          fy = f.y;
+          if (m > 0) {
+            const C yt = scr_at(0, s)[c];
+            const double2 cm = cmy[m];
+            fx -= cm.x * yt.x - cm.y * yt.y;
+            fy -= cm.x * yt.y + cm.y * yt.x;
+          }
+          if (m < ny - 1) {
+            const C yb = scr_at(1, s)[c];
+            const double2 cp = cpy[m];
+            fx -= cp.x * yb.x - cp.y * yb.y;
+            fy -= cp.x * yb.y + cp.y * yb.x;
+          }
         }
-        if (m < ny - 1) {
-          const C yb = scr_at(1, s)[c];
-          const double2 cp = cpy[m];
-          fx -= cp.x * yb.x - cp.y * yb.y;
-          fy -= cp.x * yb.y + cp.y * yb.x;
-        }
+        const C a = sdxi[c];
+        const double dx = (double)a.x * dy0.x - (double)a.y * dy0.y, dy = (double)a.x * dy0.y + (double)a.y * dy0.x;
+        w[2 * s] = (T)(fx * dx - fy * dy);
+        w[2 * s + 1] = (T)(fx * dy + fy * dx);
       }
-      const C a = sdxi[c];
-      const double dx = (double)a.x * dy0.x - (double)a.y * dy0.y, dy = (double)a.x * dy0.y + (double)a.y * dy0.x;
-      v[s][c] = make_float2((T)(fx * dx - fy * dy), (T)(fx * dy + fy * dx));
+      vq[c] = make_float4(w[0], w[1], w[2], w[3]);
+      vqr[c] = make_float4(-w[1], w[0], -w[3], w[2]);
     }
   }
   __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
@@ -1955,8 +1966,9 @@ __global__ __launch_bounds__(dma::NT) void k_solve_twist(const float2* __restric
 #pragma unroll
   for (int g = 0; g < NS - 1; ++g) tile_dma(blk_addr(0), g, g);
 
-  T acc[2][2 * RPL];
-  U vr[2];
+  const hz_v4f z4 = {0.f, 0.f, 0.f, 0.f};
+  hz_v4f dE1 = z4, dO1 = z4, dE2 = z4, dO2 = z4;  // direct: even / odd rows, block columns J1 / J2
+  T va[4][2], vr[4][2];                          // transposed A operands of the step
   int rslot = 0;
   for (int st = 0; st < nsteps; ++st) {
     const char* b0 = blk_addr(st);
@@ -1969,13 +1981,20 @@ __global__ __launch_bounds__(dma::NT) void k_solve_twist(const float2* __restric
       dma::raw_barrier();
       const U* SL = ring + rslot * SU;
       if (t == 0) {
+        dE1 = z4;
+        dO1 = z4;
+        dE2 = z4;
+        dO2 = z4;
+        // rows 2 (rb + 16 k + b) + e of the transposed products, component i
 #pragma unroll
-        for (int s = 0; s < 2; ++s) {
+        for (int k = 0; k < 4; ++k)
 #pragma unroll
-          for (int i = 0; i < 2 * RPL; ++i) acc[s][i] = 0;
-          vr[s] = reinterpret_cast<const U*>(v[s])[worker ? ru : 0];
-          if (!worker) vr[s] = make_float4(0.f, 0.f, 0.f, 0.f);
-        }
+          for (int e = 0; e < 2; ++e) {
+            const int row = 2 * (rb + 16 * k + bq) + e;
+            const int ix = (row < n ? row : 0) * 4 + jq;
+            va[k][e] = row < n ? vqf[ix] : 0.f;
+            vr[k][e] = row < n ? vqrf[ix] : 0.f;
+          }
         // the lines of this step's update (waves 0-1 plane A, 2-3 plane B)
         // and its coefficients (wave 4: coupling, 1/dy of the next block)
         if (jn >= 0) {
@@ -1998,79 +2017,60 @@ __global__ __launch_bounds__(dma::NT) void k_solve_twist(const float2* __restric
         const int nu1 = J1 * TB / RPL, nu2 = J2 * TB / RPL;
         const int ub2 = (J1 + 1) * TB * TB / RPL;
         const bool two = J2 != J1;
-        const C zc = make_float2(0.f, 0.f);
-        if (rb < Hu1) {
-          const bool ok1 = worker && ru < Hu1;
-          const bool ok2 = worker && two && ru < Hu2;
-          const int g1 = ok1 ? cg * CPB * Hu1 + ru : ZU;
-          const int g2 = ok2 ? ub2 + cg * CPB * Hu2 + ru : ZU;
-          const int st1 = ok1 ? Hu1 : 0, st2 = ok2 ? Hu2 : 0;
+        if (rb < Hu2) {
+          // ---- direct products
+          const bool ok1 = ru < Hu1, ok2 = two && ru < Hu2;
           U a1[CPB], a2[CPB];
 #pragma unroll
           for (int cc = 0; cc < CPB; ++cc) {
-            a1[cc] = SL[g1 + cc * st1];
-            a2[cc] = SL[g2 + cc * st2];
+            a1[cc] = SL[ok1 ? (cg * CPB + cc) * Hu1 + ru : ZU];
+            a2[cc] = SL[ok2 ? ub2 + (cg * CPB + cc) * Hu2 + ru : ZU];
           }
-          T tv[16];
 #pragma unroll
-          for (int i = 0; i < 16; ++i) tv[i] = 0;
-#pragma unroll
-          for (int s = 0; s < 2; ++s) {
-            const C* vrow = reinterpret_cast<const C*>(&vr[s]);
-            const C* v1 = &v[s][J1 * TB + cg * CPB];
-            const C* v2 = &v[s][J2 * TB + cg * CPB];
-            C w1[RPL], w2[RPL];
-#pragma unroll
-            for (int i = 0; i < RPL; ++i) {
-              w1[i] = ru < nu1 ? vrow[i] : zc;
-              w2[i] = ru < nu2 ? vrow[i] : zc;
-            }
+          for (int cc = 0; cc < CPB; ++cc) {
+            const int c1 = (J1 * TB + cg * CPB + cc) * 4 + jq;
+            const T bv = vqf[c1], br = vqrf[c1];
+            dE1 = __builtin_amdgcn_mfma_f32_4x4x1f32(a1[cc].x, bv, dE1, 0, 0, 0);
+            dO1 = __builtin_amdgcn_mfma_f32_4x4x1f32(a1[cc].z, bv, dO1, 0, 0, 0);
+            dE1 = __builtin_amdgcn_mfma_f32_4x4x1f32(a1[cc].y, br, dE1, 0, 0, 0);
+            dO1 = __builtin_amdgcn_mfma_f32_4x4x1f32(a1[cc].w, br, dO1, 0, 0, 0);
+          }
+          if (two) {
 #pragma unroll
             for (int cc = 0; cc < CPB; ++cc) {
-              unit_fma_fused<T, RPL>(a1[cc], v1[cc], acc[s]);
-              unit_fma_fused<T, RPL>(a2[cc], v2[cc], acc[s]);
-              HZ_SY_UNIT_DOT(a1[cc], w1, &tv[8 * s + 2 * cc])
-              HZ_SY_UNIT_DOT(a2[cc], w2, &tv[8 * s + 4 + 2 * cc])
+              const int c2 = (J2 * TB + cg * CPB + cc) * 4 + jq;
+              const T bv = vqf[c2], br = vqrf[c2];
+              dE2 = __builtin_amdgcn_mfma_f32_4x4x1f32(a2[cc].x, bv, dE2, 0, 0, 0);
+              dO2 = __builtin_amdgcn_mfma_f32_4x4x1f32(a2[cc].z, bv, dO2, 0, 0, 0);
+              dE2 = __builtin_amdgcn_mfma_f32_4x4x1f32(a2[cc].y, br, dE2, 0, 0, 0);
+              dO2 = __builtin_amdgcn_mfma_f32_4x4x1f32(a2[cc].w, br, dO2, 0, 0, 0);
             }
           }
-          T sm = row16_sum16(tv, lane);
+          // ---- transposed products: lane (b, j) <-> (row unit rb + 16 k + b, column c_j)
+          const bool lo = jq < 2;
+          const int nuj = lo ? nu1 : nu2;
+          const int basej = lo ? (cg * CPB + jq) * Hu1 : ub2 + (cg * CPB + jq - 2) * Hu2;
+          const bool okj = lo || two;
+          hz_v4f tA = z4, tB = z4;
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const int rk = rb + 16 * k + bq;
+            const U u = SL[okj && rk < nuj ? basej + rk : ZU];
+            tA = __builtin_amdgcn_mfma_f32_4x4x1f32(va[k][0], u.x, tA, 0, 0, 0);
+            tB = __builtin_amdgcn_mfma_f32_4x4x1f32(vr[k][0], u.y, tB, 0, 0, 0);
+            tA = __builtin_amdgcn_mfma_f32_4x4x1f32(va[k][1], u.z, tA, 0, 0, 0);
+            tB = __builtin_amdgcn_mfma_f32_4x4x1f32(vr[k][1], u.w, tB, 0, 0, 0);
+          }
+          // sum over the 16 blocks (lane bits 2..5): afterwards lane l holds
+          // component (l >> 2) & 3 of column c_(l & 3)
+          const T w0 = pair_step<0x124>(tA[0] + tB[0], tA[1] + tB[1], lane & 4);
+          const T w1 = pair_step<0x124>(tA[2] + tB[2], tA[3] + tB[3], lane & 4);
+          T sm = pair_step<0x128>(w0, w1, lane & 8);
           sm += xor16_f(sm, lane);
           sm += xor32_f(sm, lane);
-          const int kv = lane & 15, s = kv >> 3, kk = kv & 7;
-          if (lane < 16 && (two || kk < 4)) {
-            const int col = ((kk & 4) ? J2 : J1) * TB + cg * CPB + ((kk >> 1) & 1);
-            tdot[s][(q2 * n + col) * 2 + (kk & 1)] = sm;
-          }
-        } else if (rb < Hu2 && two) {
-          const bool ok2 = worker && ru < Hu2;
-          const int g2 = ok2 ? ub2 + cg * CPB * Hu2 + ru : ZU;
-          const int st2 = ok2 ? Hu2 : 0;
-          U a2[CPB];
-#pragma unroll
-          for (int cc = 0; cc < CPB; ++cc) a2[cc] = SL[g2 + cc * st2];
-          T tv[8];
-#pragma unroll
-          for (int i = 0; i < 8; ++i) tv[i] = 0;
-#pragma unroll
-          for (int s = 0; s < 2; ++s) {
-            const C* vrow = reinterpret_cast<const C*>(&vr[s]);
-            const C* v2 = &v[s][J2 * TB + cg * CPB];
-            C w2[RPL];
-#pragma unroll
-            for (int i = 0; i < RPL; ++i) w2[i] = ru < nu2 ? vrow[i] : zc;
-#pragma unroll
-            for (int cc = 0; cc < CPB; ++cc) {
-              unit_fma_fused<T, RPL>(a2[cc], v2[cc], acc[s]);
-              HZ_SY_UNIT_DOT(a2[cc], w2, &tv[4 * s + 2 * cc])
-            }
-          }
-          T sm = row16_sum8(tv, lane);
-          sm += xor16_f(sm, lane);
-          sm += xor32_f(sm, lane);
-          const int kv = lane & 7, s = kv >> 2, kk = kv & 3;
-          if (lane < 8) {
-            const int col = J2 * TB + cg * CPB + (kk >> 1);
-            tdot[s][(q2 * n + col) * 2 + (kk & 1)] = sm;
+          if (lane < 16 && (jq < 2 || two)) {
+            const int col = (jq < 2 ? J1 : J2) * TB + cg * CPB + (jq & 1);
+            tdf[(q2 * n + col) * 4 + ((lane >> 2) & 3)] = sm;
           }
         }
       }
@@ -2083,48 +2083,58 @@ __global__ __launch_bounds__(dma::NT) void k_solve_twist(const float2* __restric
         tile_dma(past ? reinterpret_cast<const char*>(Pk) : blk, past ? 0 : tt, (rslot + NS - 1) & (NS - 1));
       }
       if (t == TPS - 1) {
-        if (worker) {
+        {
+          // direct partials: lane 4b+j, reg i = rows 2(rb+4b+i), +1, component j
+          const hz_v4f dE = dE1 + dE2, dO = dO1 + dO2;
 #pragma unroll
-          for (int s = 0; s < 2; ++s)
-#pragma unroll
-            for (int r = 0; r < RPL; ++r)
-              part[s][cg * n + ru * RPL + r] = make_float2(acc[s][2 * r], acc[s][2 * r + 1]);
+          for (int i = 0; i < 4; ++i) {
+            const int row = 2 * (rb + 4 * bq + i);
+            if (row < n) {
+              pf[(cg * n + row) * 4 + jq] = dE[i];
+              pf[(cg * n + row + 1) * 4 + jq] = dO[i];
+            }
+          }
         }
         __builtin_amdgcn_s_waitcnt(dma::wait_vm((NS - 1) * R));
         __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
         dma::raw_barrier();
         if (tid < n) {
+          U y = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+          for (int q = 0; q < NCG; ++q) {
+            const U u = part[q * n + tid];
+            y.x += u.x; y.y += u.y; y.z += u.z; y.w += u.w;
+          }
+#pragma unroll
+          for (int q = 0; q < NQ2; ++q) {
+            const U u = tdot[q * n + tid];
+            y.x += u.x; y.y += u.y; y.z += u.z; y.w += u.w;
+          }
           const C a = sdxi[tid];
+          const double2 cf = pscal[0], b = pscal[1];
+          const double2 di =
+              make_double2((double)a.x * b.x - (double)a.y * b.y, (double)a.x * b.y + (double)a.y * b.x);
+          T w[4];
 #pragma unroll
           for (int s = 0; s < 2; ++s) {
-            if (s == 1 && !twoB) break;
-            C y = make_float2(0.f, 0.f);
-#pragma unroll
-            for (int q = 0; q < NCG; ++q) {
-              const C u = part[s][q * n + tid];
-              y.x += u.x; y.y += u.y;
-            }
-#pragma unroll
-            for (int q = 0; q < NQ2; ++q) {
-              const C u = reinterpret_cast<const C*>(tdot[s])[q * n + tid];
-              y.x += u.x; y.y += u.y;
-            }
+            const C ys = s == 0 ? make_float2(y.x, y.y) : make_float2(y.z, y.w);
+            const bool wr = s == 0 || twoB;
             C* x = xs[s];
-            if (PH == 1 && (half == 0 || st > 0)) x[(size_t)jb * n + tid] = y;  // x_jb
-            if (jn == -2) {
-              scr_at(half, s)[tid] = y;  // y_top / y_bot
-            } else if (jn >= 0) {
+            if (wr && PH == 1 && (half == 0 || st > 0)) x[(size_t)jb * n + tid] = ys;  // x_jb
+            if (wr && jn == -2) scr_at(half, s)[tid] = ys;                             // y_top / y_bot
+            C nv = make_float2(0.f, 0.f);
+            if (jn >= 0) {
               const C pre = reinterpret_cast<const C*>(pbuf + 128 * s)[tid];
-              const double2 cf = pscal[0], b = pscal[1];
-              C nv;
-              nv.x = pre.x - (T)(cf.x * y.x - cf.y * y.y);
-              nv.y = pre.y - (T)(cf.x * y.y + cf.y * y.x);
-              if (PH == 0) x[(size_t)jn * n + tid] = nv;  // t_jn / b_jn
-              const double2 di =
-                  make_double2((double)a.x * b.x - (double)a.y * b.y, (double)a.x * b.y + (double)a.y * b.x);
-              v[s][tid] = make_float2((T)((double)nv.x * di.x - (double)nv.y * di.y),
-                                      (T)((double)nv.x * di.y + (double)nv.y * di.x));
+              nv.x = pre.x - (T)(cf.x * ys.x - cf.y * ys.y);
+              nv.y = pre.y - (T)(cf.x * ys.y + cf.y * ys.x);
+              if (wr && PH == 0) x[(size_t)jn * n + tid] = nv;  // t_jn / b_jn
             }
+            w[2 * s] = (T)((double)nv.x * di.x - (double)nv.y * di.y);
+            w[2 * s + 1] = (T)((double)nv.x * di.y + (double)nv.y * di.x);
+          }
+          if (jn >= 0) {
+            vq[tid] = make_float4(w[0], w[1], w[2], w[3]);
+            vqr[tid] = make_float4(-w[1], w[0], -w[3], w[2]);
           }
         }
         __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
@@ -2743,9 +2753,9 @@ bool try_solve_twist(hz_problem* p, int nrhs, void* x, hipStream_t s) {
     const int nr = std::min(TW_RHS, nrhs - r0);
     float2* xr = reinterpret_cast<float2*>(x) + (size_t)r0 * p->N;
     const dim3 grid((unsigned)(2 * p->nslot), (unsigned)nr);
-    hipLaunchKernelGGL((k_solve_twist<NX, 0>), grid, dim3(dma::NT), 0, s, P, xr, p->tab(1, 0), p->tab(1, 1),
+    hipLaunchKernelGGL((k_solve_twist_mx<NX, 0>), grid, dim3(dma::NT), 0, s, P, xr, p->tab(1, 0), p->tab(1, 1),
                        p->d_symd.as<double2>(), (int)p->ny, f.tw_m, (int)p->nz, p->n2, p->N, sp, scr);
-    hipLaunchKernelGGL((k_solve_twist<NX, 1>), grid, dim3(dma::NT), 0, s, P, xr, p->tab(1, 0), p->tab(1, 1),
+    hipLaunchKernelGGL((k_solve_twist_mx<NX, 1>), grid, dim3(dma::NT), 0, s, P, xr, p->tab(1, 0), p->tab(1, 1),
                        p->d_symd.as<double2>(), (int)p->ny, f.tw_m, (int)p->nz, p->n2, p->N, sp, scr);
     HZ_HIP(hipGetLastError());
   }

@@ -12,7 +12,6 @@ along z, the oracle's 7-point operator and a right-preconditioned FGMRES
 with modified Gram-Schmidt in numpy."""
 import ctypes as C
 import os
-import time
 
 import numpy as np
 
@@ -255,53 +254,3 @@ def fgmres(P, M, f, m=35, tol=1e-8, max_iters=20000):
             u = u + y[i] * Z[i]
     r = f - P.op_apply(u)
     return u, it, bool(np.linalg.norm(r) <= tol * nf)
-
-
-def sample(P, iters, m, threads, nplanes=None):
-    """Extrapolated time-to-solution of the PARDISO path on `threads` host
-    threads: the factorisation and one solve of one plane per thread (run
-    concurrently), one z-DFT pair, one 7-point operator apply and one
-    Gram-Schmidt step, timed on the full-size problem and scaled to every plane
-    and `iters` FGMRES(m) iterations."""
-    nx, ny, nz = P.n
-    n2, N = nx * ny, P.N
-    kp = min(nplanes or threads, nz)
-    ks = list(range(kp))
-    ex = pool(threads)
-    solvers = [PlaneSolver(*plane_csr(P, k)) for k in ks]
-    t0 = time.perf_counter()
-    list(ex.map(lambda q: q.factor(), solvers))
-    t_fac = time.perf_counter() - t0  # one concurrent batch of kp planes
-    b = np.random.default_rng(0).standard_normal(n2) + 0j
-    list(ex.map(lambda q: q.solve(b), solvers))  # first solves carry one-time set-up
-    t0 = time.perf_counter()
-    for _ in range(3):
-        list(ex.map(lambda q: q.solve(b), solvers))
-    t_sol = (time.perf_counter() - t0) / 3
-    for s in solvers:
-        s.release()
-    # the vector stages: one untimed call each (FFT plans, thread start-up), then the mean of three
-    x = np.random.default_rng(1).standard_normal(N) + 1j
-
-    def timed(fn, reps=3):
-        fn()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            fn()
-        return (time.perf_counter() - t0) / reps
-
-    t_fft = timed(lambda: _fft_z(_fft_z(x, (nz, ny, nx), -1, threads), (nz, ny, nx), +1, threads))
-    t_op = timed(lambda: P.op_apply(x))
-    from scipy.linalg.blas import zaxpy, zdotc
-    V = [np.random.default_rng(s).standard_normal(N) + 0j for s in range(2)]
-
-    def gs1():
-        h = zdotc(V[0], V[1])
-        V[1] = zaxpy(V[0], V[1], a=-h)
-
-    t_gs1 = timed(gs1)
-    batches = float(np.sum(P.plane_solved())) / kp
-    t_factor = t_fac * batches
-    t_iter = t_sol * batches + t_fft + t_op + t_gs1 * (m / 2.0 + 1)
-    return {"total": t_factor + iters * t_iter, "factor": t_factor, "iter": t_iter, "nplanes": kp,
-            "t_fac_plane": t_fac / kp, "t_sol_plane": t_sol / kp}

@@ -106,6 +106,9 @@ struct Factors {
   // f_m between the two phases (3 x TW_RHS x nz x nx complex64)
   bool tw_ready = false;
   int tw_m = 0;
+  // the top-down chain G_j is complete for every j (with the twisted factors
+  // it stops at tw_m - 1 until a one-directional kernel first needs it)
+  bool top_full = true;
   DevBuf Gb, Gtw, tw_scr;
   Hodlr hodlr;
 };
@@ -266,6 +269,10 @@ bool stream_solve_available(int prec, int nx);
 bool sym_solve_available(int prec, int nx);
 // whether the twisted two-plane packed solve is built for this problem and precision
 bool twist_solve_available(const hz_problem* p, int prec);
+// completes the top-down factor chain (and its packed copy) where the
+// factorisation stopped it at the twisted factors' middle; synchronous, and an
+// error under stream capture
+void ensure_dense(hz_problem* p, int prec, hipStream_t s);
 // whether the 8-RHS MFMA streaming kernel is instantiated for this width/precision
 bool mfma_solve_available(int prec, int nx);
 // whether the single-RHS ring kernel is instantiated for this width/precision

@@ -82,6 +82,7 @@ void autotune(hz_problem* p, int prec, int nrhs, hz_tune_info* info) {
   HZ_CHECK(prec == HZ_PREC_FP64 || prec == HZ_PREC_C64, HZ_ERR_ARG, "bad precision %d", prec);
   HZ_CHECK(nrhs >= 1, HZ_ERR_ARG, "bad right-hand-side count %d", nrhs);
   HZ_CHECK(p->fac[prec].ready, HZ_ERR_STATE, "hz_precond_factor(prec=%d) has not run", prec);
+  ensure_dense(p, prec, p->stream);  // every candidate timed on complete factors
   const char* cpath = getenv("HZ_TUNE_CACHE");
   const std::string key = cache_key(p, prec, nrhs);
   Tuning best;

@@ -74,8 +74,8 @@ constexpr int TCM = 64;  // columns per trailing-update chunk on the matrix core
 constexpr int TC = 32;   // columns per trailing-update tile
 constexpr int CB = 4;    // columns per thread in the trailing update
 
-// mode 0: the top-down chain G_j = (D_j - sig_j G_{j-1})^{-1}, j = 0 .. ny-1,
-//         into G ([slot][j]), one workgroup per slot.
+// mode 0: the top-down chain G_j = (D_j - sig_j G_{j-1})^{-1}, j = jbeg .. jend-1
+//         (from G_{jbeg-1} when jbeg > 0), into G ([slot][j]), one workgroup per slot.
 // mode 1: the same in workgroups 0 .. nslot-1 and, in workgroups nslot ..
 //         2 nslot-1, the bottom-up chain of the twisted factors
 //         B_j = (D_j - sig_{j+1} B_{j+1})^{-1}, j = ny-1 .. m+1, into Gb
@@ -86,7 +86,7 @@ template <typename T>
 __global__ __launch_bounds__(FACT_THREADS) void k_factor(typename Cx<T>::t* __restrict__ G, PlaneTabs tb, int n,
                                                          int ny, int nb, int* __restrict__ status,
                                                          const int* __restrict__ splane, int mode, int m,
-                                                         typename Cx<T>::t* __restrict__ Gb) {
+                                                         typename Cx<T>::t* __restrict__ Gb, int jbeg, int jend) {
   typedef typename Cx<T>::t C;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C* P = reinterpret_cast<C*>(smem_raw);     // nb * n, column-major panel
@@ -113,10 +113,10 @@ __global__ __launch_bounds__(FACT_THREADS) void k_factor(typename Cx<T>::t* __re
   }
   __syncthreads();
 
-  const int nblk = mode == 2 ? 1 : (up ? ny - 1 - m : ny);
+  const int nblk = mode == 2 ? 1 : (up ? ny - 1 - m : jend - jbeg);
   C* const Gs = rev ? Gb + (size_t)slot * (ny - m) * nn : G + (size_t)slot * ny * nn;
   for (int jj = 0; jj < nblk; ++jj) {
-    const int j = mode == 2 ? m : (up ? ny - 1 - jj : jj);
+    const int j = mode == 2 ? m : (up ? ny - 1 - jj : jbeg + jj);
     C* Gj = Gs + (size_t)(rev ? j - m : j) * nn;
     // ---- S_j = D_j - sig_j G_{j-1} (bottom-up: - sig_{j+1} B_{j+1}; the
     //      middle block both)
@@ -2699,7 +2699,7 @@ bool solve_sym_mfma8_any(hz_problem* p, int groups, void* x, hipStream_t s) {
 }
 
 template <typename T, int RPL, int NCG, int TB, int NX>
-bool try_pack_sym(hz_problem* p, int prec, hipStream_t s) {
+bool try_pack_sym(hz_problem* p, int prec, hipStream_t s, bool alloc_only) {
   if (p->nx != NX) return false;
   typedef typename Cx<T>::t C;
   typedef SymLayout<NX, TB> L;
@@ -2711,7 +2711,7 @@ bool try_pack_sym(hz_problem* p, int prec, hipStream_t s) {
   }
   const double2* dx = p->d_symd.as<double2>();
   const double2* dy = dx + NX;
-  if (p->nslot == 0) return true;
+  if (p->nslot == 0 || alloc_only) return true;
   hipLaunchKernelGGL((k_pack_sym<C, NX, TB>), dim3(L::TPS, (unsigned)(p->nslot * p->ny)), dim3(256), 0, s, f.G.as<C>(),
                      f.Gsym.as<C>(), dx, dy, (int)p->ny, (const C*)nullptr, 0);
   HZ_HIP(hipGetLastError());
@@ -2788,14 +2788,16 @@ bool try_solve_sym2(hz_problem* p, int prec, int nrhs, void* x, bool adjoint, hi
   X(HZ_PREC_C64, float, 2, 4, 8, 160)    \
   X(HZ_PREC_C64, float, 2, 4, 8, 128)
 
-bool pack_sym_any(hz_problem* p, int prec, hipStream_t s) {
+bool pack_sym_any(hz_problem* p, int prec, hipStream_t s, bool alloc_only = false) {
 #define HZ_SYM_PACK(P, T, RPL, NCG, TB, NX) \
-  if (prec == P && try_pack_sym<T, RPL, NCG, TB, NX>(p, prec, s)) return true;
+  if (prec == P && try_pack_sym<T, RPL, NCG, TB, NX>(p, prec, s, alloc_only)) return true;
   HZ_SYM_WIDTHS(HZ_SYM_PACK)
 #undef HZ_SYM_PACK
   if (prec == HZ_PREC_FP64)
-    return try_pack_sym<double, 1, 2, 8, 200>(p, prec, s) || try_pack_sym<double, 1, 2, 8, 192>(p, prec, s) ||
-           try_pack_sym<double, 1, 2, 8, 160>(p, prec, s) || try_pack_sym<double, 1, 2, 8, 128>(p, prec, s);
+    return try_pack_sym<double, 1, 2, 8, 200>(p, prec, s, alloc_only) ||
+           try_pack_sym<double, 1, 2, 8, 192>(p, prec, s, alloc_only) ||
+           try_pack_sym<double, 1, 2, 8, 160>(p, prec, s, alloc_only) ||
+           try_pack_sym<double, 1, 2, 8, 128>(p, prec, s, alloc_only);
   return false;
 }
 
@@ -2900,6 +2902,19 @@ bool twist_solve_available(const hz_problem* p, int prec) {
   return prec == HZ_PREC_C64 && p->fac[prec].tw_ready;
 }
 
+namespace {
+// whether plane_solve_kernels takes the twisted solve for this call (the only
+// kernel that runs on an incomplete top-down chain)
+bool uses_twist(const hz_problem* p, int prec, int nrhs, bool adjoint, const Tuning& t, const void* x) {
+  const Factors& f = p->fac[prec];
+  if (adjoint || prec != HZ_PREC_C64 || !f.tw_ready || reinterpret_cast<uintptr_t>(x) % 16 != 0) return false;
+  const SolveKernel sk = solve_kernel_choice();
+  if (sk == SK_TWIST) return true;
+  if (sk != SK_DEFAULT) return false;
+  return t.solve_family == HZ_SOLVE_TWIST || (t.solve_family == HZ_SOLVE_DEFAULT && nrhs == 1);
+}
+}  // namespace
+
 bool mfma_solve_available(int prec, int nx) {
   return prec == HZ_PREC_C64 && (nx == 200 || nx == 192 || nx == 160 || nx == 128 || nx == 96 || nx == 64);
 }
@@ -2952,6 +2967,7 @@ void factor_begin(hz_problem* p, int prec, hipStream_t s) {
   const bool twist = prec == HZ_PREC_C64 && ny >= 2 && p->slot_pairs && sym_solve_available(prec, n) &&
                      !getenv("HZ_NO_TWIST") && !getenv("HZ_NO_SYM_PACK");
   f.tw_m = twist ? ny / 2 : 0;
+  f.top_full = !twist || getenv("HZ_FACTOR_FULL") != nullptr;
   if (twist) {
     const size_t gb = (size_t)p->nslot * (ny - f.tw_m) * (size_t)n * n * sizeof(float2);
     if (f.Gb.bytes != gb) {
@@ -2985,19 +3001,21 @@ void factor_begin(hz_problem* p, int prec, hipStream_t s) {
     set_max_lds((const void*)&k_factor<double>, lds);
     if (p->nslot > 0)
       hipLaunchKernelGGL(k_factor<double>, dim3((unsigned)p->nslot), dim3(FACT_THREADS), lds, s, f.G.as<double2>(),
-                         tb, n, ny, nb, st, p->d_splane.as<int>(), 0, 0, (double2*)nullptr);
+                         tb, n, ny, nb, st, p->d_splane.as<int>(), 0, 0, (double2*)nullptr, 0, ny);
   } else {
     while (nb > 4 && factor_lds<float>(n, nb) > 160 * 1024) nb /= 2;
     const size_t lds = factor_lds<float>(n, nb);
     HZ_CHECK(lds <= 160 * 1024, HZ_ERR_ARG, "nx=%d too large for the LDS panel", n);
     set_max_lds((const void*)&k_factor<float>, lds);
     if (p->nslot > 0) {
+      // with the twisted factors the top-down chain stops at tw_m - 1 (the
+      // rest is completed on the first use of a one-directional kernel)
       hipLaunchKernelGGL(k_factor<float>, dim3((unsigned)(twist ? 2 * p->nslot : p->nslot)), dim3(FACT_THREADS), lds,
                          s, f.G.as<float2>(), tb, n, ny, nb, st, p->d_splane.as<int>(), twist ? 1 : 0, f.tw_m,
-                         f.Gb.as<float2>());
+                         f.Gb.as<float2>(), 0, f.top_full ? ny : f.tw_m);
       if (twist)
         hipLaunchKernelGGL(k_factor<float>, dim3((unsigned)p->nslot), dim3(FACT_THREADS), lds, s, f.G.as<float2>(), tb,
-                           n, ny, nb, st, p->d_splane.as<int>(), 2, f.tw_m, f.Gb.as<float2>());
+                           n, ny, nb, st, p->d_splane.as<int>(), 2, f.tw_m, f.Gb.as<float2>(), 0, 0);
     }
   }
   HZ_HIP(hipGetLastError());
@@ -3011,7 +3029,9 @@ void factor_end(hz_problem* p, int prec, hipStream_t s) {
   // symmetric packed copy for the widths the packed solve is instantiated for
   f.sym_ready = false;
   if (sym_solve_available(prec, (int)p->nx) && !getenv("HZ_NO_SYM_PACK")) {
-    f.sym_ready = pack_sym_any(p, prec, s);
+    // with an incomplete top-down chain only the storage is set up here
+    const bool ok = pack_sym_any(p, prec, s, !f.top_full);
+    f.sym_ready = ok && f.top_full;
     HZ_HIP(hipStreamSynchronize(s));
   }
   // twisted packed copy; the unpacked bottom-up blocks are not kept
@@ -3032,6 +3052,35 @@ void factor_end(hz_problem* p, int prec, hipStream_t s) {
 void launch_factor(hz_problem* p, int prec, hipStream_t s) {
   factor_begin(p, prec, s);
   factor_end(p, prec, s);
+}
+
+void ensure_dense(hz_problem* p, int prec, hipStream_t s) {
+  Factors& f = p->fac[prec];
+  if (!f.ready || f.top_full) return;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  HZ_HIP(hipStreamIsCapturing(s, &cs));
+  HZ_CHECK(cs == hipStreamCaptureStatusNone, HZ_ERR_STATE,
+           "the one-directional plane factors are completed on their first use, which cannot run under stream "
+           "capture: run one solve of this kind before capturing");
+  const int n = (int)p->nx, ny = (int)p->ny;
+  int* st = reinterpret_cast<int*>(dscal(p, 63));
+  *p->pin_misc.as<int>() = -1;
+  launch_fill(1, make_double2(0, 0), dscal(p, 63), s);
+  int nb = 32;
+  while (nb > 4 && factor_lds<float>(n, nb) > 160 * 1024) nb /= 2;
+  const size_t lds = factor_lds<float>(n, nb);
+  if (p->nslot > 0)
+    hipLaunchKernelGGL(k_factor<float>, dim3((unsigned)p->nslot), dim3(FACT_THREADS), lds, s, f.G.as<float2>(),
+                       plane_tabs(p), n, ny, nb, st, p->d_splane.as<int>(), 0, f.tw_m, (float2*)nullptr, f.tw_m, ny);
+  HZ_HIP(hipGetLastError());
+  HZ_HIP(hipMemcpyAsync(p->pin_misc.ptr, st, sizeof(int), hipMemcpyDeviceToHost, s));
+  HZ_HIP(hipStreamSynchronize(s));
+  HZ_CHECK(*p->pin_misc.as<int>() == 0, HZ_ERR_SINGULAR, "zero pivot while factoring a plane");
+  f.top_full = true;
+  if (sym_solve_available(prec, n) && !getenv("HZ_NO_SYM_PACK")) {
+    f.sym_ready = pack_sym_any(p, prec, s);
+    HZ_HIP(hipStreamSynchronize(s));
+  }
 }
 
 void launch_plane_solve(hz_problem* p, int prec, int nrhs, void* x, bool adjoint, hipStream_t s) {
@@ -3083,6 +3132,7 @@ void plane_solve_kernels(hz_problem* p, int prec, int nrhs, void* x, bool adjoin
     HZ_CHECK(f.G.ptr != nullptr, HZ_ERR_STATE,
              "adjoint plane solve: the dense factors were released by HZ_HODLR_FREE_DENSE");
   }
+  if (!f.top_full && !uses_twist(p, prec, nrhs, adjoint, t, x)) ensure_dense(p, prec, s);
   const int n = (int)p->nx, ny = (int)p->ny;
   const int ng = std::max(1, APPLY_THREADS / n);
   PlaneTabs tb = plane_tabs(p);

@@ -778,6 +778,7 @@ void hodlr_compress(hz_problem* p, int prec, double eps, int small, int flags) {
     return;
   }
   HZ_CHECK(f.ready && f.G.ptr, HZ_ERR_STATE, "compression needs the dense factors of precision %d", prec);
+  ensure_dense(p, prec, p->stream);
   HZ_CHECK(eps < 1.0, HZ_ERR_ARG, "eps=%g: the truncation rule needs 0 <= eps < 1", eps);
   const int n = (int)p->nx, ny = (int)p->ny;
   HZ_CHECK(n >= 2 && n <= HT, HZ_ERR_ARG, "HODLR plane solve: nx=%d outside 2..%d", n, HT);

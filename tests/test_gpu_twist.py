@@ -155,3 +155,49 @@ def test_twist_fgmres_iterations(hz, oracle, monkeypatch):
         assert st.converged
         assert abs(st.iters - out["sym"][1].iters) <= 1
         assert _rel(u, u_o) < 1e-7, (k, _rel(u, u_o))
+
+
+def test_twist_lazy_top_chain(hz, monkeypatch):
+    """With the twisted factors the factorisation stops the top-down chain at
+    ny/2 and completes it on the first use of a one-directional kernel (packed,
+    full-block, adjoint, HODLR): every such result equals the one from a
+    factorisation that ran the whole chain at once (HZ_FACTOR_FULL), bit for
+    bit, and the twisted solve is the same either way."""
+    print("case twist lazy", flush=True)
+    d, c = _case(hz, "ref", 160, 20, 9, 5)
+    monkeypatch.delenv("HZ_SOLVE_KERNEL", raising=False)
+    monkeypatch.delenv("HZ_FACTOR_FULL", raising=False)
+    lazy = hz.Solver(d, c)
+    lazy.factor(hz.abi.PREC_C64)
+    monkeypatch.setenv("HZ_FACTOR_FULL", "1")
+    full = hz.Solver(d, c)
+    full.factor(hz.abi.PREC_C64)
+    monkeypatch.delenv("HZ_FACTOR_FULL")
+    assert lazy.factor_bytes() == full.factor_bytes()
+    x = torch.from_numpy(_rand((3, d.N), 17)).to(torch.complex64).cuda()
+
+    def run(S, kernel, nrhs=1, adjoint=False):
+        if kernel:
+            monkeypatch.setenv("HZ_SOLVE_KERNEL", kernel)
+        else:
+            monkeypatch.delenv("HZ_SOLVE_KERNEL", raising=False)
+        y = S.plane_solve(x[:nrhs].clone(), hz.abi.PREC_C64, adjoint=adjoint)
+        torch.cuda.synchronize()
+        return y.cpu().numpy()
+
+    # the twisted solve first (the lazy factors stay incomplete), then every
+    # one-directional family
+    assert np.array_equal(run(lazy, ""), run(full, ""))
+    for kernel, nrhs, adjoint in (("sym", 1, False), ("", 3, False), ("mfma", 3, False), ("", 1, True)):
+        a, b = run(lazy, kernel, nrhs, adjoint), run(full, kernel, nrhs, adjoint)
+        print("  %s nrhs=%d adjoint=%s: max |diff| %.3g" % (kernel or "default", nrhs, adjoint, np.abs(a - b).max()))
+        assert np.array_equal(a, b), (kernel, nrhs, adjoint)
+    # HODLR compression of a lazily factored problem
+    monkeypatch.delenv("HZ_SOLVE_KERNEL", raising=False)
+    lazy2 = hz.Solver(d, c)
+    lazy2.factor(hz.abi.PREC_C64)
+    i1 = lazy2.compress(hz.abi.PREC_C64, eps=1e-4, smallsize=7)
+    i2 = full.compress(hz.abi.PREC_C64, eps=1e-4, smallsize=7)
+    assert i1["max_rank"] == i2["max_rank"] and i1["bytes"] == i2["bytes"]
+    for k in (0, 3):
+        assert np.array_equal(lazy2.hodlr_ranks(k, hz.abi.PREC_C64), full.hodlr_ranks(k, hz.abi.PREC_C64))

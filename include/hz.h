@@ -218,7 +218,12 @@ int hz_precond_apply_adjoint(hz_problem* p, int prec, int nrhs, const hz_c128* i
  * about nx/64 and nx/8), at most rank 16 per block.  eps < 0 drops the
  * compressed form (back to the dense factors if kept).  A new
  * hz_precond_factor drops it too. */
-enum { HZ_HODLR_FREE_DENSE = 1 };
+/* HZ_HODLR_TWISTED (complex64, where the twisted factors exist): compress the
+ * twisted factors (chains from both ends of each y-line chain meeting at
+ * ny/2) instead of the one-directional ones; the forward applies then run two
+ * workgroups per plane over half chains.  No counterpart in the reference,
+ * whose HODLR recursion is one-directional (BinaryTreesHODLR.cpp). */
+enum { HZ_HODLR_FREE_DENSE = 1, HZ_HODLR_TWISTED = 2 };
 int hz_precond_compress(hz_problem* p, int prec, double eps, int smallsize, int flags);
 typedef struct {
   int32_t active;     /* 1 when the forward applies of prec use the compressed records */

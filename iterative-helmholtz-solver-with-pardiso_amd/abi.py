@@ -82,6 +82,7 @@ class HodlrInfo(C.Structure):
 
 
 HODLR_FREE_DENSE = 1
+HODLR_TWISTED = 2
 FIELDS_AUTO = -1
 
 

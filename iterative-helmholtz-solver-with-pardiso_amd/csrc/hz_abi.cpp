@@ -547,7 +547,7 @@ int hz_precond_compress(hz_problem* p, int prec, double eps, int smallsize, int 
   HZ_API_BEGIN
   HZ_CHECK(p, HZ_ERR_ARG, "null problem");
   HZ_CHECK(prec == HZ_PREC_FP64 || prec == HZ_PREC_C64, HZ_ERR_ARG, "bad precision %d", prec);
-  HZ_CHECK((flags & ~HZ_HODLR_FREE_DENSE) == 0, HZ_ERR_ARG, "unknown flags 0x%x", flags);
+  HZ_CHECK((flags & ~(HZ_HODLR_FREE_DENSE | HZ_HODLR_TWISTED)) == 0, HZ_ERR_ARG, "unknown flags 0x%x", flags);
   hodlr_compress(p, prec, eps, smallsize, flags);
   HZ_API_END
 }

@@ -83,6 +83,7 @@ struct Tuning {
 // complex, one record per (factor slot, y-line) with its header in recs.
 struct Hodlr {
   bool active = false;
+  bool twisted = false;  // records of the twisted factors (applied by the two-phase sweep)
   double eps = 0.0;
   int small = 0, lev = 0, mleaf = 0, nn = 0, pmax = 0;
   double mean_rank = 0.0, bytes = 0.0;

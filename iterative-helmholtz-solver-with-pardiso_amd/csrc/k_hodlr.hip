@@ -73,8 +73,24 @@ __device__ __forceinline__ double2 to_z(C v) {
 // Circle-method round robin over rr (even) columns: pair s of round t.
 __device__ __forceinline__ int rr_pos(int x, int t, int rr) { return x == 0 ? 0 : ((x - 1 + t) % (rr - 1)) + 1; }
 
+// K(r, c) of block b of the packed symmetric layout (the upper block triangle of
+// K = X D in 8 x 8 tiles, block columns J and T-1-J side by side, each holding
+// rows [0, 8 (J+1)) column-major; below that, K(r, c) = K(c, r))
+__device__ __forceinline__ size_t packed_at(int r, int c, int T) {
+  if (r / 8 > c / 8) {
+    const int t = r;
+    r = c;
+    c = t;
+  }
+  const int J = c / 8, H = 8 * (J + 1);
+  const size_t base = J <= T - 1 - J ? (size_t)J * (T + 1) * 64 : (size_t)(T - 1 - J) * (T + 1) * 64 + (size_t)(T - J) * 64;
+  return base + (size_t)(c - 8 * J) * H + r;
+}
+
+// G: dense blocks ([rec] n x n, column-major; K = G D); with packed != 0 the
+// blocks are read from the packed symmetric form of K itself (n = 8 T)
 template <typename CIN>
-__global__ __launch_bounds__(HT) void k_hodlr_compress(const CIN* __restrict__ G, const double2* __restrict__ symd,
+__global__ __launch_bounds__(HT) void k_hodlr_compress(const CIN* __restrict__ G, int packed, const double2* __restrict__ symd,
                                                        int n, int ny, int nrec, const int4* __restrict__ nodes, int nn,
                                                        const int2* __restrict__ leaves, int nl, int mleaf, int lev,
                                                        double eps, double floor_rel, CIN* __restrict__ stage,
@@ -106,10 +122,14 @@ __global__ __launch_bounds__(HT) void k_hodlr_compress(const CIN* __restrict__ G
 
   for (int rec = blockIdx.x; rec < nrec; rec += gridDim.x) {
     const int j = rec % ny;
-    const CIN* Gb = G + (size_t)rec * n * n;
+    const int T = n / 8;
+    const CIN* Gb = G + (size_t)rec * (packed ? (size_t)T * (T + 1) * 32 : (size_t)n * n);
     CIN* Sb = stage + (size_t)rec * n * WS;
     const double2 dyj = dy[j];
-    auto K = [&](int r, int c) { return zmul(to_z(Gb[(size_t)c * n + r]), zmul(dx[c], dyj)); };
+    auto K = [&](int r, int c) {
+      if (packed) return to_z(Gb[packed_at(r, c, T)]);
+      return zmul(to_z(Gb[(size_t)c * n + r]), zmul(dx[c], dyj));
+    };
     // leaves: [c][i] = K(i, leaf start + c), zero past the leaf
     for (int e = tid; e < mleaf * n; e += HT) {
       const int c = e / n, i = e - c * n;
@@ -701,6 +721,263 @@ __global__ __launch_bounds__(HT) void k_solve_hodlr(const C* __restrict__ pool, 
   }
 }
 
+// The twisted sweep (records of the twisted factors: X_j = G_j below m, T_m,
+// B_j above m), one half of one plane per workgroup.  Phase 0: the top half
+// forms t_j downward (t_0 = f_0, t_j = f_j - cm_y(j) X_{j-1} t_{j-1}) and the
+// bottom half b_j upward (b_j = f_j - cp_y(j) X_{j+1} b_{j+1}), leaving
+// y_top = X_{m-1} t_{m-1}, y_bot = X_{m+1} b_{m+1} and a copy of f_m in scr.
+// Phase 1: both halves form x_m = T_m (f_m - cm_y(m) y_top - cp_y(m) y_bot) and
+// sweep outward (x_j = X_j (t_j - cp_y(j) x_{j+1}) above, x_j = X_j (b_j -
+// cm_y(j) x_{j-1}) below).  Steps, tree products and prefetch as in
+// k_solve_hodlr.
+template <typename C, int LEV, int PMAX, int NB, int PH>
+__global__ __launch_bounds__(HT) void k_solve_hodlr_tw(const C* __restrict__ pool, const HRec* __restrict__ recs,
+                                                       C* __restrict__ X, const double2* __restrict__ cmy,
+                                                       const double2* __restrict__ cpy,
+                                                       const double2* __restrict__ symd, int n, int ny, int small,
+                                                       int mleaf, int64_t n2, int64_t N, const int* __restrict__ fmap,
+                                                       int m, int nz, C* __restrict__ scr) {
+  typedef typename CT<C>::T T;
+  typedef HBuf<C, LEV, PMAX> B;
+  typedef CPair<C> P2;
+  constexpr int ML = B::ML;
+  static_assert(ML % 2 == 0 && PMAX % 2 == 0, "values in pairs");
+  __shared__ C v[HT];
+  __shared__ C ex[2][PMAX][HT / 64];
+  const int k = blockIdx.x >> 1, half = blockIdx.x & 1;
+  const int kf = __builtin_amdgcn_readfirstlane(fmap[k]);
+  if (kf < 0) return;
+  C* x = X + (int64_t)blockIdx.y * N + (int64_t)k * n2;
+  // scr: [what][rhs][plane][n], what = 0 y_top, 1 y_bot, 2 f_m
+  auto scr_at = [&](int what) { return scr + (((int64_t)what * gridDim.y + blockIdx.y) * nz + k) * n; };
+  const HRec* rk = recs + (size_t)kf * ny;
+  const double2* dxi = symd + n + ny;
+  const double2* dyi = symd + 2 * n + ny;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  // this slot's leaf: walk the tree
+  int la = 0, lm = n, sa = 0, S = HT;
+  for (int l = 0; l < LEV; ++l) {
+    if (lm <= small) break;
+    const int m2 = (lm + 1) / 2, m1 = lm - m2;
+    S >>= 1;
+    if (tid < sa + S) {
+      lm = m1;
+    } else {
+      la += m1;
+      lm = m2;
+      sa += S;
+    }
+  }
+  const int row = tid - sa < lm ? la + (tid - sa) : -1;
+  const int r0 = row >= 0 ? row : 0;
+  // 1/dx of this row in a register (no per-lane load inside the sweep)
+  const double2 dxr = row >= 0 ? dxi[row] : make_double2(0, 0);
+  const int nsteps = PH == 0 ? (half == 0 ? m : ny - 1 - m) : (half == 0 ? m + 1 : ny - m);
+  const int mlp = (mleaf + 1) / 2;  // leaf pairs
+  auto blk_of = [&](int st) {
+    if (PH == 0) return half == 0 ? st : ny - 1 - st;
+    return half == 0 ? m - st : m + st;
+  };
+  // block whose vector the update of step st forms (-1: none; -2: y goes to scr)
+  auto next_blk = [&](int st) {
+    const int jb = blk_of(st);
+    if (PH == 0) {
+      if (half == 0) return jb + 1 < m ? jb + 1 : -2;
+      return jb - 1 > m ? jb - 1 : -2;
+    }
+    if (half == 0) return jb > 0 ? jb - 1 : -1;
+    return jb < ny - 1 ? jb + 1 : -1;
+  };
+  // the update toward larger j couples through cm_y, toward smaller j through cp_y
+  const bool usecm = (PH == 0) == (half == 0);
+  if (PH == 0 && half == 0) {
+    // f_m for phase 1 (phase 1 overwrites line m with x_m)
+    for (int i = tid; i < n; i += HT) scr_at(2)[i] = x[(size_t)m * n + i];
+  }
+  if (nsteps <= 0) return;
+  const C zc = from_z<C>(make_double2(0, 0));
+
+  // Branch-free loads: every pair slot loads (past the record's sizes from a
+  // clamped address in the same lines), so that the wait before a step
+  // covers exactly what was issued one step earlier; the step masks the
+  // slots past the record's sizes.
+  auto load = [&](B& b, int st) {
+    const int sc = min(st, nsteps - 1);
+    const HRec h = rk[blk_of(sc)];
+    const P2* base = reinterpret_cast<const P2*>(pool + h.off);
+#pragma unroll
+    for (int c = 0; c < ML / 2; ++c) {
+      const P2 q = base[(size_t)(c < mlp ? c : 0) * n + r0];
+      b.L[2 * c] = q.a;
+      b.L[2 * c + 1] = q.b;
+    }
+    size_t o = (size_t)mlp * n;
+#pragma unroll
+    for (int l = 0; l < LEV; ++l) {
+      const int pl = h.p[l], pp = (pl + 1) / 2;
+      b.p[l] = pl;
+#pragma unroll
+      for (int t = 0; t < PMAX / 2; ++t) {
+        const P2 q = base[o + (size_t)(t < pp ? t : 0) * n + r0];
+        b.X[l][2 * t] = q.a;
+        b.X[l][2 * t + 1] = q.b;
+      }
+      o += (size_t)pp * n;
+    }
+    const int jn = next_blk(sc);
+    // the update reads the line of jn: f (phase 0) or t / b (phase 1)
+    b.pre = x[(size_t)max(jn, 0) * n + r0];
+    b.cf = jn >= 0 ? (usecm ? cmy[jn] : cpy[jn]) : make_double2(0, 0);
+    b.dyn = dyi[max(jn, 0)];
+  };
+
+  // rank slots in pairs (one uniform branch per pair) so that the two
+  // reductions interleave
+  auto level_pass = [&](const B& b, const C vr, C& y, auto lc) {
+    constexpr int l = decltype(lc)::value;
+    constexpr int G = HT >> (l + 1);
+    const int pl = b.p[l];
+#pragma unroll
+    for (int t0 = 0; t0 < PMAX; t0 += 2) {
+      if (t0 < pl) {
+#pragma unroll
+        for (int t = t0; t < t0 + 2; ++t) {
+          const C xv = (row >= 0 && t < pl) ? b.X[l][t] : zc;
+          C pr;
+          pr.x = xv.x * vr.x - xv.y * vr.y;
+          pr.y = xv.x * vr.y + xv.y * vr.x;
+          if constexpr (G <= 32) {
+            C s;
+            s.x = group_sum<G>(pr.x, lane);
+            s.y = group_sum<G>(pr.y, lane);
+            C q;
+            q.x = group_partner<G>(s.x, lane);
+            q.y = group_partner<G>(s.y, lane);
+            y = cfma(xv, q, y);
+          } else {
+            C s;
+            s.x = group_sum<64>(pr.x, lane);
+            s.y = group_sum<64>(pr.y, lane);
+            if (lane == 0) ex[l][t][wv] = s;
+          }
+        }
+      }
+    }
+  };
+  auto level_finish = [&](const B& b, C& y, auto lc) {
+    constexpr int l = decltype(lc)::value;
+    const int pl = b.p[l];
+#pragma unroll
+    for (int t = 0; t < PMAX; ++t) {
+      if (t < ((pl + 1) & ~1)) {
+        const C xv = (row >= 0 && t < pl) ? b.X[l][t] : zc;
+        C q;
+        if constexpr (l == 1) {
+          q = ex[1][t][wv ^ 1];
+        } else {
+          const C q0 = ex[0][t][wv ^ 2], q1 = ex[0][t][wv ^ 3];
+          q.x = q0.x + q1.x;
+          q.y = q0.y + q1.y;
+        }
+        y = cfma(xv, q, y);
+      }
+    }
+  };
+
+  auto step = [&](int st, const B& b) {
+    C y = zc;
+    if (row >= 0) {
+#pragma unroll
+      for (int c = 0; c < ML; ++c)
+        if (c < mleaf) y = cfma(b.L[c], v[min(la + c, n - 1)], y);
+    }
+    const C vr = row >= 0 ? v[row] : zc;
+    if constexpr (LEV > 0) level_pass(b, vr, y, std::integral_constant<int, 0>{});
+    if constexpr (LEV > 1) level_pass(b, vr, y, std::integral_constant<int, 1>{});
+    if constexpr (LEV > 2) level_pass(b, vr, y, std::integral_constant<int, 2>{});
+    if constexpr (LEV > 3) level_pass(b, vr, y, std::integral_constant<int, 3>{});
+    if constexpr (LEV > 4) level_pass(b, vr, y, std::integral_constant<int, 4>{});
+    if constexpr (LEV > 5) level_pass(b, vr, y, std::integral_constant<int, 5>{});
+    __syncthreads();
+    if constexpr (LEV >= 1) level_finish(b, y, std::integral_constant<int, 0>{});
+    if constexpr (LEV >= 2) level_finish(b, y, std::integral_constant<int, 1>{});
+    if (row >= 0) {
+      const int i = row;
+      const int jb = blk_of(st), jn = next_blk(st);
+      if (PH == 1 && (half == 0 || st > 0)) x[(size_t)jb * n + i] = y;  // x_jb
+      if (jn == -2) scr_at(half)[i] = y;                               // y_top / y_bot
+      if (jn >= 0) {
+        const double2 cf = b.cf;
+        C nv;
+        nv.x = b.pre.x - (T)(cf.x * (double)y.x - cf.y * (double)y.y);
+        nv.y = b.pre.y - (T)(cf.x * (double)y.y + cf.y * (double)y.x);
+        if (PH == 0) x[(size_t)jn * n + i] = nv;  // t_jn / b_jn
+        v[i] = from_z<C>(zmul(to_z(nv), zmul(dxr, b.dyn)));
+      }
+    }
+    __syncthreads();
+  };
+
+  // v' of the first block: D^{-1} f (phase 0), D_m^{-1} (f_m - cm_y(m) y_top -
+  // cp_y(m) y_bot) (phase 1)
+  if (row >= 0) {
+    const int j0 = blk_of(0);
+    double2 f;
+    if (PH == 0) {
+      f = to_z(x[(size_t)j0 * n + row]);
+    } else {
+      f = to_z(scr_at(2)[row]);
+      if (m > 0) {
+        const double2 yt = zmul(cmy[m], to_z(scr_at(0)[row]));
+        f.x -= yt.x;
+        f.y -= yt.y;
+      }
+      if (m < ny - 1) {
+        const double2 yb = zmul(cpy[m], to_z(scr_at(1)[row]));
+        f.x -= yb.x;
+        f.y -= yb.y;
+      }
+    }
+    v[row] = from_z<C>(zmul(f, zmul(dxr, dyi[j0])));
+  }
+  // named buffers, no copies between them
+  B b0, b1, b2;
+  if constexpr (NB == 1) {
+    __syncthreads();
+    for (int st = 0; st < nsteps; ++st) {
+      load(b0, st);
+      step(st, b0);
+    }
+  } else if constexpr (NB == 2) {
+    load(b0, 0);
+    __syncthreads();
+    for (int st = 0; st < nsteps; st += 2) {
+      load(b1, st + 1);
+      step(st, b0);
+      if (st + 1 < nsteps) {
+        load(b0, st + 2);
+        step(st + 1, b1);
+      }
+    }
+  } else {
+    load(b0, 0);
+    load(b1, 1);
+    __syncthreads();
+    for (int st = 0; st < nsteps; st += 3) {
+      load(b2, st + 2);
+      step(st, b0);
+      if (st + 1 < nsteps) {
+        load(b0, st + 3);
+        step(st + 1, b1);
+      }
+      if (st + 2 < nsteps) {
+        load(b1, st + 4);
+        step(st + 2, b2);
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------- host
 struct Tree {
   int lev = 0, mleaf = 0;
@@ -743,8 +1020,39 @@ void launch_solve_lev(hz_problem* p, const Hodlr& h, int nrhs, void* x, hipStrea
                      h.mleaf, p->n2, p->N, p->d_fmap.as<int>());
 }
 
+// the twisted sweep: phase 0 then phase 1, TW_RHS right-hand sides per pair
+template <typename C, int LEV, int PMAX>
+void launch_solve_lev_tw(hz_problem* p, const Hodlr& h, int nrhs, void* x, hipStream_t s) {
+  constexpr int NB = sizeof(C) == 8 ? (PMAX <= 4 ? 3 : 2) : 1;
+  const Factors& f = p->fac[HZ_PREC_C64];
+  for (int r0 = 0; r0 < nrhs; r0 += TW_RHS) {
+    const int nr = std::min(TW_RHS, nrhs - r0);
+    C* xr = reinterpret_cast<C*>(x) + (size_t)r0 * p->N;
+    const dim3 grid((unsigned)(2 * p->nz), (unsigned)nr);
+    hipLaunchKernelGGL((k_solve_hodlr_tw<C, LEV, PMAX, NB, 0>), grid, dim3(HT), 0, s, h.pool.as<C>(),
+                       h.recs.as<HRec>(), xr, p->tab(1, 0), p->tab(1, 1), p->d_symd.as<double2>(), (int)p->nx,
+                       (int)p->ny, h.small, h.mleaf, p->n2, p->N, p->d_fmap.as<int>(), f.tw_m, (int)p->nz,
+                       f.tw_scr.as<C>());
+    hipLaunchKernelGGL((k_solve_hodlr_tw<C, LEV, PMAX, NB, 1>), grid, dim3(HT), 0, s, h.pool.as<C>(),
+                       h.recs.as<HRec>(), xr, p->tab(1, 0), p->tab(1, 1), p->d_symd.as<double2>(), (int)p->nx,
+                       (int)p->ny, h.small, h.mleaf, p->n2, p->N, p->d_fmap.as<int>(), f.tw_m, (int)p->nz,
+                       f.tw_scr.as<C>());
+  }
+}
+
 template <typename C, int PMAX>
 void launch_solve_pmax(hz_problem* p, const Hodlr& h, int nrhs, void* x, hipStream_t s) {
+  if constexpr (sizeof(C) == 8) {
+    if (h.twisted) {
+      switch (h.lev) {
+        case 3: launch_solve_lev_tw<C, 3, PMAX>(p, h, nrhs, x, s); return;
+        case 4: launch_solve_lev_tw<C, 4, PMAX>(p, h, nrhs, x, s); return;
+        case 5: launch_solve_lev_tw<C, 5, PMAX>(p, h, nrhs, x, s); return;
+        case 6: launch_solve_lev_tw<C, 6, PMAX>(p, h, nrhs, x, s); return;
+        default: HZ_CHECK(false, HZ_ERR_ARG, "HODLR tree depth %d outside 3..6", h.lev);
+      }
+    }
+  }
   switch (h.lev) {
     case 3: launch_solve_lev<C, 3, PMAX>(p, h, nrhs, x, s); break;
     case 4: launch_solve_lev<C, 4, PMAX>(p, h, nrhs, x, s); break;
@@ -777,8 +1085,14 @@ void hodlr_compress(hz_problem* p, int prec, double eps, int small, int flags) {
     f.hodlr.release();
     return;
   }
-  HZ_CHECK(f.ready && f.G.ptr, HZ_ERR_STATE, "compression needs the dense factors of precision %d", prec);
-  ensure_dense(p, prec, p->stream);
+  const bool twisted = (flags & HZ_HODLR_TWISTED) != 0;
+  if (twisted) {
+    HZ_CHECK(prec == HZ_PREC_C64 && f.ready && f.tw_ready && f.Gtw.ptr, HZ_ERR_STATE,
+             "twisted compression needs the twisted complex64 factors (widths 128, 160, 192, 200, ny >= 2)");
+  } else {
+    HZ_CHECK(f.ready && f.G.ptr, HZ_ERR_STATE, "compression needs the dense factors of precision %d", prec);
+    ensure_dense(p, prec, p->stream);
+  }
   HZ_CHECK(eps < 1.0, HZ_ERR_ARG, "eps=%g: the truncation rule needs 0 <= eps < 1", eps);
   const int n = (int)p->nx, ny = (int)p->ny;
   HZ_CHECK(n >= 2 && n <= HT, HZ_ERR_ARG, "HODLR plane solve: nx=%d outside 2..%d", n, HT);
@@ -823,12 +1137,13 @@ void hodlr_compress(hz_problem* p, int prec, double eps, int small, int flags) {
   // pivoted QR stops there when 0.05 eps lies below it)
   const double floor_rel = prec == HZ_PREC_FP64 ? 1e-15 : 1e-7;
   if (prec == HZ_PREC_FP64)
-    hipLaunchKernelGGL(k_hodlr_compress<double2>, dim3(gw), dim3(HT), 0, s, f.G.as<double2>(),
+    hipLaunchKernelGGL(k_hodlr_compress<double2>, dim3(gw), dim3(HT), 0, s, f.G.as<double2>(), 0,
                        p->d_symd.as<double2>(), n, ny, nrec, dnodes.as<int4>(), nn, dleaves.as<int2>(),
                        (int)t.leaves.size(), t.mleaf, t.lev, eps, floor_rel, stage.as<double2>(),
                        dranks.as<unsigned char>(), dover.as<int>(), wsA.as<double2>(), wsR.as<double2>());
   else
-    hipLaunchKernelGGL(k_hodlr_compress<float2>, dim3(gw), dim3(HT), 0, s, f.G.as<float2>(),
+    hipLaunchKernelGGL(k_hodlr_compress<float2>, dim3(gw), dim3(HT), 0, s,
+                       twisted ? f.Gtw.as<float2>() : f.G.as<float2>(), twisted ? 1 : 0,
                        p->d_symd.as<double2>(), n, ny, nrec, dnodes.as<int4>(), nn, dleaves.as<int2>(),
                        (int)t.leaves.size(), t.mleaf, t.lev, eps, floor_rel, stage.as<float2>(),
                        dranks.as<unsigned char>(), dover.as<int>(), wsA.as<double2>(), wsR.as<double2>());
@@ -874,6 +1189,7 @@ void hodlr_compress(hz_problem* p, int prec, double eps, int small, int flags) {
   HZ_HIP(hipGetLastError());
   HZ_HIP(hipStreamSynchronize(s));
   H.active = true;
+  H.twisted = twisted;
   H.eps = eps;
   H.small = small;
   H.lev = t.lev;
@@ -888,7 +1204,7 @@ void hodlr_compress(hz_problem* p, int prec, double eps, int small, int flags) {
     f.Gsym.release();
     f.sym_ready = false;
     f.Gtw.release();
-    f.tw_scr.release();
+    if (!twisted) f.tw_scr.release();  // the twisted sweep keeps its workspace
     f.tw_ready = false;
   }
 }

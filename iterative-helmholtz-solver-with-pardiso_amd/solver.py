@@ -106,12 +106,13 @@ class Solver:
     def factor(self, prec=PREC_FP64):
         self._run("hz_precond_factor", int(prec))
 
-    def compress(self, prec=PREC_C64, eps=1e-5, smallsize=16, free_dense=False):
+    def compress(self, prec=PREC_C64, eps=1e-5, smallsize=16, free_dense=False, twisted=False):
         """HODLR compression of the factors of `prec` (hz_precond_compress):
         later forward preconditioner applies use the compressed records;
-        eps < 0 drops them."""
-        self._run("hz_precond_compress", int(prec), float(eps), int(smallsize),
-                  abi.HODLR_FREE_DENSE if free_dense else 0)
+        eps < 0 drops them.  twisted: compress the twisted complex64 factors
+        (two-phase half-chain sweeps)."""
+        flags = (abi.HODLR_FREE_DENSE if free_dense else 0) | (abi.HODLR_TWISTED if twisted else 0)
+        self._run("hz_precond_compress", int(prec), float(eps), int(smallsize), flags)
         return self.hodlr_info(prec)
 
     def hodlr_info(self, prec=PREC_C64):

@@ -50,6 +50,9 @@ def parse():
                     help="also solve with the HODLR-compressed complex64 factors at this eps "
                          "(the reference's HODLR threshold, main.cpp:161); < 0 skips")
     ap.add_argument("--hodlr-small", type=int, default=7, help="HODLR leaf size (smallsize)")
+    ap.add_argument("--hodlr-one-way", action="store_true",
+                    help="compress the one-directional factors (the reference's recursion) instead of the "
+                         "twisted ones (two workgroups per plane over half chains)")
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--config", choices=["all", "mixed200", "batched", "adjoint"], default="all",
                     help="all (default): the 200^3 mixed-precision solve as the headline, with the 128^3 FP64, "
@@ -238,13 +241,22 @@ def main():
 
 def bench_hodlr(a, S, A, f, u_dense, t_dense, stream, buf, torch, np):
     """The same solve with the complex64 block inverses HODLR-compressed
-    (hz_precond_compress, the reference's HODLR build at its threshold):
-    factorisation, compression and FGMRES timed together; the compressed
-    plane solve timed with HIP events like the dense one."""
+    (hz_precond_compress, the reference's HODLR build at its threshold; by
+    default the twisted factors' records): factorisation, compression and
+    FGMRES timed together; the compressed plane solve timed with HIP events
+    like the dense one."""
+    twisted = not a.hodlr_one_way
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     S.factor(A.PREC_C64)
-    info = S.compress(A.PREC_C64, eps=a.hodlr_eps, smallsize=a.hodlr_small)
+    try:
+        info = S.compress(A.PREC_C64, eps=a.hodlr_eps, smallsize=a.hodlr_small, twisted=twisted)
+    except A.HzError:
+        if not twisted:
+            raise
+        # no twisted factors for this width: the one-directional records
+        twisted = False
+        info = S.compress(A.PREC_C64, eps=a.hodlr_eps, smallsize=a.hodlr_small)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     u, st = S.fgmres(f, m=a.m, restarts=max(1, a.max_iters // a.m), tol=a.tol, prec=A.PREC_C64, autotune=True)
@@ -262,7 +274,8 @@ def bench_hodlr(a, S, A, f, u_dense, t_dense, stream, buf, torch, np):
     torch.cuda.synchronize()
     err = (torch.linalg.vector_norm(u - u_dense) / torch.linalg.vector_norm(u_dense)).item()
     S.compress(A.PREC_C64, eps=-1.0)
-    return {"eps": a.hodlr_eps, "smallsize": a.hodlr_small, "time_s": round(t2 - t0, 4), "iters": st.iters,
+    return {"eps": a.hodlr_eps, "smallsize": a.hodlr_small, "twisted": twisted, "time_s": round(t2 - t0, 4),
+            "iters": st.iters,
             "rel_res": st.rel_res, "factor_compress_s": round(t1 - t0, 4),
             "plane_solve_ms": round(e0.elapsed_time(e1) / reps, 4), "levels": info["levels"],
             "max_rank": info["max_rank"], "mean_rank": round(info["mean_rank"], 3),
@@ -573,11 +586,13 @@ def kernel_only(S, d, A, torch):
         os.environ.pop("HZ_SOLVE_KERNEL", None)
     else:
         os.environ["HZ_SOLVE_KERNEL"] = saved
-    # then 5 launches of the HODLR-compressed solve (eps 1e-4, smallsize 7)
-    info = S.compress(A.PREC_C64, eps=1e-4, smallsize=7)
-    for _ in range(5):
-        A.check(A.lib().hz_plane_solve(S.h, A.PREC_C64, 0, 1, buf.data_ptr()), "hz_plane_solve")
-    A.check(A.lib().hz_problem_sync(S.h), "sync")
+    # then 5 launches of the HODLR-compressed solve (eps 1e-4, smallsize 7):
+    # one-directional records, then the twisted ones
+    for tw in (False, True):
+        info = S.compress(A.PREC_C64, eps=1e-4, smallsize=7, twisted=tw)
+        for _ in range(5):
+            A.check(A.lib().hz_plane_solve(S.h, A.PREC_C64, 0, 1, buf.data_ptr()), "hz_plane_solve")
+        A.check(A.lib().hz_problem_sync(S.h), "sync")
     print(json.dumps({"kernel_only": True, "copy_bytes": 2 * (1 << 30),
                       "plane_solve_bytes": {str(f): plane_solve_bytes(d.n, f, S.slots()[0]) for f in fams},
                       "hodlr_record_bytes": info["bytes"]}), flush=True)

@@ -287,7 +287,7 @@ def plane_solve_kernel(nx, tuning):
     """Name of the complex64 single-RHS plane-solve kernel the tuned choice launches."""
     fam = tuning.get("solve_kernel", 0)
     if fam == 6:
-        return "k_solve_twist_mx<%d, 0> + k_solve_twist_mx<%d, 1>" % (nx, nx)
+        return "k_solve_twist_pw<%d, 0> + k_solve_twist_pw<%d, 1>" % (nx, nx)
     if fam == 5:
         return "k_solve_sym2<%d, false>" % nx
     if fam == 3:

@@ -26,6 +26,7 @@
 #include <cstdlib>
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 
 #include "hz_internal.h"
 
@@ -1473,7 +1474,7 @@ struct SymShape {
 #pragma clang diagnostic push
 #pragma clang diagnostic ignored "-Winline-asm"
 __device__ __forceinline__ void dma16_untracked(const void* g, unsigned lds_wave) {
-  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(g), "s"(lds_wave)
+  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(g), "s"(__builtin_amdgcn_readfirstlane(lds_wave))
                : "memory", "m0");
 }
 #pragma clang diagnostic pop
@@ -1807,10 +1808,14 @@ __device__ __forceinline__ float xor32_f(float x, int lane) {
 // chains of about ny/2 blocks (the one-plane kernel: nz workgroups, 2 ny - 1
 // blocks).  y_top, y_bot and a copy of f_m pass between the phases in scr
 // (phase 1 writes x_m into line m, which the other half reads as f_m).
-// Tiles and ring as in k_solve_sym2.  Both products run on the matrix cores
-// (v_mfma_f32_4x4x1f32: 16 blocks of 4x4x1; D(lane 4b+j, reg i) +=
-// A(lane 4b+i) B(lane 4b+j)).  The four real columns of every product are the
-// two planes' real and imaginary parts, so no block column idles:
+//
+// Waves: wave (cg, rb), cg = wave % 4, rb = 64 (wave / 4), takes columns
+// 2cg, 2cg + 1 of both block columns of a tile pair (J1 = t, J2 = T - 1 - t)
+// and row units rb .. rb + 63 (a unit is two rows of one column).  Both
+// products run on the matrix cores (v_mfma_f32_4x4x1f32: 16 blocks of 4x4x1;
+// D(lane 4b+j, reg i) += A(lane 4b+i) B(lane 4b+j)).  The four real columns
+// of every product are the two planes' real and imaginary parts, so no block
+// column idles:
 //   direct, y_r += K(r, c) v'_c: lane l holds row unit rb + l (rows 2(rb+l)
 //     and 2(rb+l)+1); A = Re or Im K of one of those rows, B = the components
 //     of v'_c, so D(lane 4b+j, reg i) is row 2(rb+4b+i) (+1), component j,
@@ -1822,8 +1827,26 @@ __device__ __forceinline__ float xor32_f(float x, int lane) {
 //     blocks at the end of the tile.
 // Components: 0, 1 = Re, Im of plane A; 2, 3 = Re, Im of plane B.  Im K
 // multiplies the rotated components (-Im A, Re A, -Im B, Re B).
+//
+// Tile rings: each wave moves exactly the part of every tile pair that it
+// reads into a ring of its own, so tiles need no barrier: a wave waits for its
+// own transfers only, and the workgroup meets twice per block step (partial
+// sums, then the new right-hand side).  Slot layouts (16-byte units; segments
+// 68 units apart, so the transposed reads of four columns fall in distinct
+// 4-bank groups):
+//   rb = 0:  [0, Hu1), [68, 68 + Hu1)   columns 2cg, 2cg + 1 of J1
+//            [136, 200), [204, 268)    columns 2cg, 2cg + 1 of J2, row units
+//                                      0 .. min(64, Hu2); unit 272 holds zeros
+//   rb = 64: [0, 64), [68, 132)        columns 2cg, 2cg + 1 of J2, row units
+//                                      64 .. Hu2; unit 136 holds zeros
+// (Hu = 4 (J + 1) stored row units of block column J.)  Every tile costs a
+// fixed number of transfers per wave (4, or 2 for rb = 64; lanes past a
+// segment repeat its last unit, tiles without rows for the wave repeat the
+// tile's first unit), so the vmcnt waits count tiles exactly.  The waves of
+// rb = 64 hold no row of J1 (Hu1 <= 64) and skip the tiles whose J2 rows end
+// before unit 64.
 template <int NX, int PH>
-__global__ __launch_bounds__(dma::NT) void k_solve_twist_mx(const float2* __restrict__ P, float2* __restrict__ X,
+__global__ __launch_bounds__(dma::NT) void k_solve_twist_pw(const float2* __restrict__ P, float2* __restrict__ X,
                                                             const double2* __restrict__ cmy,
                                                             const double2* __restrict__ cpy,
                                                             const double2* __restrict__ symd, int ny, int m, int nz,
@@ -1833,21 +1856,29 @@ __global__ __launch_bounds__(dma::NT) void k_solve_twist_mx(const float2* __rest
   typedef float2 C;
   typedef float4 U;
   constexpr int RPL = 2, NCG = 4, TB = 8;
-  typedef SymShape<T, RPL, NCG, TB, NX> S_;
   typedef SymLayout<NX, TB> L;
-  constexpr int n = S_::n, CPB = S_::CPB, R = S_::R, PU = S_::PU;
-  constexpr int TPS = L::TPS, NPAIR = L::NPAIR, PAIR = L::PAIR, BS = L::BS, T_ = L::T;
-  constexpr int NS = S_::S;
-  constexpr int ZU = R * dma::NT;
-  constexpr int SU = ZU + 1;
-  constexpr int NQ2 = dma::NT / NCG / 64;  // row halves (one transposed partial each)
-  static_assert((NS & (NS - 1)) == 0, "ring slots: power of two");
+  constexpr int n = NX, T_ = L::T, TPS = L::TPS, PAIR = L::PAIR, BS = L::BS;
+  constexpr int PU = n * (int)sizeof(C) / 16;  // units of one y-line
+  constexpr int NW = 5;                        // ring slots per wave
+  // column segments of a slot start at units 0, 68, 136, 204 (distinct
+  // 4-bank groups: the transposed reads of four columns are conflict-free);
+  // the zero unit of a slot is its last unit
+  constexpr int SEG = 68;
+  constexpr int SLO = 273, SHI = 137;          // units per slot (rb = 0, rb = 64)
+  constexpr int LLO = 4, LHI = 2;              // transfers per tile
+  constexpr bool HAS_HI = n / RPL > 64;
+  // transposed blocks of the second row half that can hold a stored row
+  constexpr int KT64 = 4 * (T_ - 1) <= 64 ? 0 : (4 * (T_ - 1) - 64 + 15) / 16;
+  static_assert(n / RPL <= 128 && KT64 <= 4, "two row halves of 64 units");
+  static_assert(TPS * TB / RPL <= 64, "block column J1 lies in the first row half");
+  static_assert(NW <= TPS && TPS * LLO <= 63, "vmcnt range");
   static_assert(PU <= 128, "line DMA of two waves per plane");
-  static_assert(CPB == 2 && NQ2 == 2, "four columns and 64 row units per wave");
-  static_assert((size_t)NS * SU * 16 + (NCG + NQ2 + 2) * n * 16 + 5 * 64 * 16 + n * 8 <= 160 * 1024, "LDS budget");
-  __shared__ U ring[NS * SU];
+  static_assert(2 * n <= dma::NT, "one thread per plane and row in the step epilogue");
+  static_assert((size_t)(4 * NW * (SLO + SHI) + (NCG + 2 + 2) * n + 5 * 64) * 16 + n * 8 <= 160 * 1024,
+                "LDS budget");
+  __shared__ U ring[4 * NW * SLO + 4 * NW * SHI];
   __shared__ U part[NCG * n];   // direct products per column group
-  __shared__ U tdot[NQ2 * n];   // transposed products per row half
+  __shared__ U tdot[2 * n];     // transposed products per row half
   __shared__ U vq[n], vqr[n];   // v' components and rotated components
   __shared__ U pbuf[5 * 64];    // lines of planes A (units 0..127) and B (128..255); coefficients at 256..319
   __shared__ C sdxi[n];
@@ -1860,22 +1891,21 @@ __global__ __launch_bounds__(dma::NT) void k_solve_twist_mx(const float2* __rest
   const int slot = blockIdx.x >> 1, half = blockIdx.x & 1;
   const int2 sp = slotp[slot];
   const bool twoB = sp.y >= 0;
+  const int pl0 = __builtin_amdgcn_readfirstlane(sp.x);
+  const int pl1 = __builtin_amdgcn_readfirstlane(twoB ? sp.y : sp.x);
   const int64_t rhs = blockIdx.y;
   const int nrhs = gridDim.y;
-  C* xs[2];
-  xs[0] = X + rhs * N + (int64_t)sp.x * n2;
-  xs[1] = X + rhs * N + (int64_t)(twoB ? sp.y : sp.x) * n2;
+  C* const xs0 = X + rhs * N + (int64_t)pl0 * n2;
+  C* const xs1 = X + rhs * N + (int64_t)pl1 * n2;
+  auto xs = [&](int s) { return s ? xs1 : xs0; };
   // scr: [what][rhs][plane][n], what = 0 y_top, 1 y_bot, 2 f_m
-  auto scr_at = [&](int what, int s) {
-    return scr + (((int64_t)what * nrhs + rhs) * nz + (s ? (twoB ? sp.y : sp.x) : sp.x)) * n;
-  };
+  auto scr_at = [&](int what, int s) { return scr + (((int64_t)what * nrhs + rhs) * nz + (s ? pl1 : pl0)) * n; };
   const C* Pk = P + (size_t)slot * ny * BS;
   const double2* dxi = symd + n + ny;
   const double2* dyi = symd + 2 * n + ny;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int cg = wv % NCG, rb = (wv / NCG) * 64, ru = rb + lane;
-  const int q2 = rb >> 6;
+  const int cg = wv % NCG;
   const int bq = lane >> 2, jq = lane & 3;  // the lane's MFMA block and index in the block
   const int nsteps = PH == 0 ? (half == 0 ? m : ny - 1 - m) : (half == 0 ? m + 1 : ny - m);
   auto blk_of = [&](int st) {
@@ -1899,12 +1929,16 @@ __global__ __launch_bounds__(dma::NT) void k_solve_twist_mx(const float2* __rest
     // f_m for phase 1 (phase 1 overwrites line m with x_m)
     for (int i = tid; i < 2 * n; i += dma::NT) {
       const int s = i / n, c = i - s * n;
-      if (s == 0 || twoB) scr_at(2, s)[c] = xs[s][(size_t)m * n + c];
+      if (s == 0 || twoB) scr_at(2, s)[c] = xs(s)[(size_t)m * n + c];
     }
   }
   if (nsteps <= 0) return;
-  for (int i = tid; i < NQ2 * n; i += dma::NT) tdot[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (tid < NS) ring[tid * SU + ZU] = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int i = tid; i < 2 * n; i += dma::NT) tdot[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (tid < 8 * NW) {
+    const int w = tid / NW, s = tid - w * NW;
+    const int z = w < 4 ? (w * NW + s) * SLO + SLO - 1 : 4 * NW * SLO + ((w - 4) * NW + s) * SHI + SHI - 1;
+    ring[z] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
   for (int i = tid; i < n; i += dma::NT) {
     const double2 a = dxi[i];
     sdxi[i] = make_float2((T)a.x, (T)a.y);
@@ -1919,7 +1953,7 @@ __global__ __launch_bounds__(dma::NT) void k_solve_twist_mx(const float2* __rest
       for (int s = 0; s < 2; ++s) {
         double fx, fy;
         if (PH == 0) {
-          const C f = xs[s][(size_t)j0 * n + c];
+          const C f = xs(s)[(size_t)j0 * n + c];
           fx = f.x;
           fy = f.y;
         } else {
@@ -1949,200 +1983,256 @@ __global__ __launch_bounds__(dma::NT) void k_solve_twist_mx(const float2* __rest
       vqr[c] = make_float4(-w[1], w[0], -w[3], w[2]);
     }
   }
-  __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
+  __syncthreads();
 
-  auto tile_dma = [&](const char* blk, int tt, int sl) {
-    const int tu = (tt < NPAIR ? PAIR : (NPAIR + 1) * TB * TB) / RPL;
-    const char* base = blk + (size_t)tt * PAIR * sizeof(C);
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int u = min(r * dma::NT + tid, tu - 1);
-      dma16_untracked(base + (size_t)u * 16, ring_lds + (unsigned)((sl * SU + r * dma::NT + wv * 64) * 16));
-    }
-  };
   auto blk_addr = [&](int st) {
     return reinterpret_cast<const char*>(Pk + (size_t)blk_of(st < nsteps ? st : 0) * BS);
   };
-#pragma unroll
-  for (int g = 0; g < NS - 1; ++g) tile_dma(blk_addr(0), g, g);
-
   const hz_v4f z4 = {0.f, 0.f, 0.f, 0.f};
-  hz_v4f dE1 = z4, dO1 = z4, dE2 = z4, dO2 = z4;  // direct: even / odd rows, block columns J1 / J2
-  T va[4][2], vr[4][2];                          // transposed A operands of the step
-  int rslot = 0;
-  for (int st = 0; st < nsteps; ++st) {
-    const char* b0 = blk_addr(st);
-    const char* b1 = blk_addr(st + 1);
-    const int jb = blk_of(st), jn = next_of(st);
-#pragma unroll
-    for (int t = 0; t < TPS; ++t) {
-      __builtin_amdgcn_s_waitcnt(dma::wait_vm((NS - 2) * R));
-      __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
-      dma::raw_barrier();
-      const U* SL = ring + rslot * SU;
-      if (t == 0) {
-        dE1 = z4;
-        dO1 = z4;
-        dE2 = z4;
-        dO2 = z4;
-        // rows 2 (rb + 16 k + b) + e of the transposed products, component i
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-#pragma unroll
-          for (int e = 0; e < 2; ++e) {
-            const int row = 2 * (rb + 16 * k + bq) + e;
-            const int ix = (row < n ? row : 0) * 4 + jq;
-            va[k][e] = row < n ? vqf[ix] : 0.f;
-            vr[k][e] = row < n ? vqrf[ix] : 0.f;
-          }
-        // the lines of this step's update (waves 0-1 plane A, 2-3 plane B)
-        // and its coefficients (wave 4: coupling, 1/dy of the next block)
-        if (jn >= 0) {
-          if (wv < 4) {
-            const int s = wv >> 1;
-            const char* lb = reinterpret_cast<const char*>(xs[s] + (size_t)jn * n);
-            dma16_untracked(lb + (size_t)min((wv & 1) * 64 + lane, PU - 1) * 16,
-                            pbuf_lds + (unsigned)(wv * 64 * 16));
-          } else if (wv == 4) {
-            const double2* src;
-            if (lane == 1) src = dyi + jn;
-            else src = (PH == 0 ? (half == 0) : (half != 0)) ? cmy + jn : cpy + jn;
-            dma16_untracked(src, pbuf_lds + 256 * 16);
-          }
-        }
+
+  auto run = [&](auto hi_tag) {
+    constexpr bool HI = decltype(hi_tag)::value;
+    constexpr int SZ = HI ? SHI : SLO, ZU = SZ - 1, LD = HI ? LHI : LLO;
+    constexpr int rb = HI ? 64 : 0, q2 = HI ? 1 : 0;
+    const int wq = HI ? wv - 4 : wv;
+    const int wbase = HI ? 4 * NW * SLO + wq * NW * SHI : wq * NW * SLO;
+    const U* WR = ring + wbase;
+    const unsigned wr_lds = ring_lds + (unsigned)wbase * 16u;
+    const int ca = cg * 2;
+    // the wave's transfers of tile tt of a block into ring slot sl
+    auto tile_dma = [&](const char* blk, int tt, int sl, int lane) {
+      const int J1 = tt, J2 = T_ - 1 - tt;
+      const bool two = J2 != J1;
+      const int Hu1 = (J1 + 1) * TB / RPL, Hu2 = (J2 + 1) * TB / RPL;
+      const int ub2 = (J1 + 1) * TB * TB / RPL;
+      const char* base = blk + (size_t)tt * PAIR * sizeof(C);
+      const unsigned dst = wr_lds + (unsigned)(sl * SZ) * 16u;
+      int u[LD];
+      if (!HI) {
+        const int L2 = Hu2 < 64 ? Hu2 : 64;
+        u[0] = ca * Hu1 + min(lane, Hu1 - 1);
+        u[1] = (ca + 1) * Hu1 + min(lane, Hu1 - 1);
+        u[2] = two ? ub2 + ca * Hu2 + min(lane, L2 - 1) : 0;
+        u[3] = two ? ub2 + (ca + 1) * Hu2 + min(lane, L2 - 1) : 0;
+      } else {
+        const bool has = two && Hu2 > 64;
+        u[0] = has ? ub2 + ca * Hu2 + 64 + min(lane, Hu2 - 65) : 0;
+        u[1] = has ? ub2 + (ca + 1) * Hu2 + 64 + min(lane, Hu2 - 65) : 0;
       }
-      {
-        const int J1 = t, J2 = T_ - 1 - t;
-        const int Hu1 = (J1 + 1) * TB / RPL, Hu2 = (J2 + 1) * TB / RPL;
-        const int nu1 = J1 * TB / RPL, nu2 = J2 * TB / RPL;
-        const int ub2 = (J1 + 1) * TB * TB / RPL;
-        const bool two = J2 != J1;
-        if (rb < Hu2) {
-          // ---- direct products
-          const bool ok1 = ru < Hu1, ok2 = two && ru < Hu2;
-          U a1[CPB], a2[CPB];
 #pragma unroll
-          for (int cc = 0; cc < CPB; ++cc) {
-            a1[cc] = SL[ok1 ? (cg * CPB + cc) * Hu1 + ru : ZU];
-            a2[cc] = SL[ok2 ? ub2 + (cg * CPB + cc) * Hu2 + ru : ZU];
-          }
+      for (int i = 0; i < LD; ++i) dma16_untracked(base + (size_t)u[i] * 16, dst + (unsigned)(i * SEG * 16));
+    };
 #pragma unroll
-          for (int cc = 0; cc < CPB; ++cc) {
-            const int c1 = (J1 * TB + cg * CPB + cc) * 4 + jq;
-            const T bv = vqf[c1], br = vqrf[c1];
-            dE1 = __builtin_amdgcn_mfma_f32_4x4x1f32(a1[cc].x, bv, dE1, 0, 0, 0);
-            dO1 = __builtin_amdgcn_mfma_f32_4x4x1f32(a1[cc].z, bv, dO1, 0, 0, 0);
-            dE1 = __builtin_amdgcn_mfma_f32_4x4x1f32(a1[cc].y, br, dE1, 0, 0, 0);
-            dO1 = __builtin_amdgcn_mfma_f32_4x4x1f32(a1[cc].w, br, dO1, 0, 0, 0);
-          }
-          if (two) {
+    for (int g = 0; g < NW; ++g) tile_dma(blk_addr(0), g, g, lane);
+
+    hz_v4f dE1 = z4, dO1 = z4, dE2 = z4, dO2 = z4;  // direct: even / odd rows, block columns J1 / J2
+    T va[4][2], vr[4][2];                          // transposed A operands of the step
+    int rslot = 0;
+    for (int st = 0; st < nsteps; ++st) {
+      const char* b0 = blk_addr(st);
+      const char* b1 = blk_addr(st + 1);
+      const int jb = blk_of(st), jn = next_of(st);
 #pragma unroll
-            for (int cc = 0; cc < CPB; ++cc) {
-              const int c2 = (J2 * TB + cg * CPB + cc) * 4 + jq;
-              const T bv = vqf[c2], br = vqrf[c2];
-              dE2 = __builtin_amdgcn_mfma_f32_4x4x1f32(a2[cc].x, bv, dE2, 0, 0, 0);
-              dO2 = __builtin_amdgcn_mfma_f32_4x4x1f32(a2[cc].z, bv, dO2, 0, 0, 0);
-              dE2 = __builtin_amdgcn_mfma_f32_4x4x1f32(a2[cc].y, br, dE2, 0, 0, 0);
-              dO2 = __builtin_amdgcn_mfma_f32_4x4x1f32(a2[cc].w, br, dO2, 0, 0, 0);
+      for (int t = 0; t < TPS; ++t) {
+        // the lane index, opaque to the compiler at every tile: the tile's
+        // lane-dependent offsets are formed here instead of being kept in
+        // registers across the whole step, which leaves registers for the
+        // tile's operands to be read ahead of the products
+        const int lane = tid & 63;
+        const int bq = lane >> 2, jq = lane & 3;
+        // this tile's transfers are complete once at most the NW - 1 younger
+        // tiles' transfers are outstanding
+        __builtin_amdgcn_s_waitcnt(dma::wait_vm((NW - 1) * LD));
+        asm volatile("" ::: "memory");
+        const U* SL = WR + rslot * SZ;
+        if (t == 0) {
+          dE1 = z4;
+          dO1 = z4;
+          dE2 = z4;
+          dO2 = z4;
+          // rows 2 (rb + 16 k + b) + e of the transposed products, component i
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+              const int row = 2 * (rb + 16 * k + bq) + e;
+              const int ix = (row < n ? row : 0) * 4 + jq;
+              va[k][e] = row < n ? vqf[ix] : 0.f;
+              vr[k][e] = row < n ? vqrf[ix] : 0.f;
+            }
+          // the lines of this step's update (waves 0-1 plane A, 2-3 plane B)
+          // and its coefficients (wave 4: coupling, 1/dy of the next block)
+          if (jn >= 0) {
+            if (!HI) {
+              const int s = wv >> 1;
+              const char* lb = reinterpret_cast<const char*>(xs(s) + (size_t)jn * n);
+              dma16_untracked(lb + (size_t)min((wv & 1) * 64 + lane, PU - 1) * 16,
+                              pbuf_lds + (unsigned)(wv * 64 * 16));
+            } else if (wv == 4) {
+              const double2* src;
+              if (lane == 1) src = dyi + jn;
+              else src = (PH == 0 ? (half == 0) : (half != 0)) ? cmy + jn : cpy + jn;
+              dma16_untracked(src, pbuf_lds + 256 * 16);
             }
           }
-          // ---- transposed products: lane (b, j) <-> (row unit rb + 16 k + b, column c_j)
-          const bool lo = jq < 2;
-          const int nuj = lo ? nu1 : nu2;
-          const int basej = lo ? (cg * CPB + jq) * Hu1 : ub2 + (cg * CPB + jq - 2) * Hu2;
-          const bool okj = lo || two;
-          hz_v4f tA = z4, tB = z4;
-#pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            const int rk = rb + 16 * k + bq;
-            const U u = SL[okj && rk < nuj ? basej + rk : ZU];
-            tA = __builtin_amdgcn_mfma_f32_4x4x1f32(va[k][0], u.x, tA, 0, 0, 0);
-            tB = __builtin_amdgcn_mfma_f32_4x4x1f32(vr[k][0], u.y, tB, 0, 0, 0);
-            tA = __builtin_amdgcn_mfma_f32_4x4x1f32(va[k][1], u.z, tA, 0, 0, 0);
-            tB = __builtin_amdgcn_mfma_f32_4x4x1f32(vr[k][1], u.w, tB, 0, 0, 0);
-          }
-          // sum over the 16 blocks (lane bits 2..5): afterwards lane l holds
-          // component (l >> 2) & 3 of column c_(l & 3)
-          const T w0 = pair_step<0x124>(tA[0] + tB[0], tA[1] + tB[1], lane & 4);
-          const T w1 = pair_step<0x124>(tA[2] + tB[2], tA[3] + tB[3], lane & 4);
-          T sm = pair_step<0x128>(w0, w1, lane & 8);
-          sm += xor16_f(sm, lane);
-          sm += xor32_f(sm, lane);
-          if (lane < 16 && (jq < 2 || two)) {
-            const int col = (jq < 2 ? J1 : J2) * TB + cg * CPB + (jq & 1);
-            tdf[(q2 * n + col) * 4 + ((lane >> 2) & 3)] = sm;
-          }
         }
-      }
-      {
-        // the tile NS - 1 stages ahead into the slot the previous stage read
-        const int ta = t + NS - 1;
-        const int tt = ta % TPS;
-        const char* blk = (ta / TPS) == 0 ? b0 : b1;
-        const bool past = (ta / TPS) != 0 && st + 1 >= nsteps;
-        tile_dma(past ? reinterpret_cast<const char*>(Pk) : blk, past ? 0 : tt, (rslot + NS - 1) & (NS - 1));
-      }
-      if (t == TPS - 1) {
         {
-          // direct partials: lane 4b+j, reg i = rows 2(rb+4b+i), +1, component j
-          const hz_v4f dE = dE1 + dE2, dO = dO1 + dO2;
+          const int J1 = t, J2 = T_ - 1 - t;
+          const int Hu1 = (J1 + 1) * TB / RPL, Hu2 = (J2 + 1) * TB / RPL;
+          const int nu1 = J1 * TB / RPL, nu2 = J2 * TB / RPL;
+          const bool two = J2 != J1;
+          if (!HI || (HAS_HI && two && Hu2 > 64)) {
+            constexpr int KT = HI ? KT64 : 4;
+            // ---- every operand of the tile first (one LDS round trip per tile)
+            // direct: lane = row unit rb + lane; transposed: lane (b, j) <->
+            // (row unit rb + 16 k + b, column c_j)
+            U a1[2], a2[2], uu[4];
+            T bv1[2], br1[2], bv2[2], br2[2];
+            const bool lo = jq < 2;
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const int row = 2 * (rb + 4 * bq + i);
-            if (row < n) {
-              pf[(cg * n + row) * 4 + jq] = dE[i];
-              pf[(cg * n + row + 1) * 4 + jq] = dO[i];
+            for (int cc = 0; cc < 2; ++cc) {
+              if (!HI) {
+                a1[cc] = SL[lane < Hu1 ? cc * SEG + lane : ZU];
+                a2[cc] = SL[two && lane < Hu2 ? (2 + cc) * SEG + lane : ZU];
+                const int c1 = (J1 * TB + ca + cc) * 4 + jq;
+                bv1[cc] = vqf[c1];
+                br1[cc] = vqrf[c1];
+              } else {
+                a2[cc] = SL[lane < Hu2 - 64 ? cc * SEG + lane : ZU];
+              }
+              if (two) {
+                const int c2 = (J2 * TB + ca + cc) * 4 + jq;
+                bv2[cc] = vqf[c2];
+                br2[cc] = vqrf[c2];
+              }
+            }
+#pragma unroll
+            for (int k = 0; k < KT; ++k) {
+              const int rk = rb + 16 * k + bq;
+              int ix;
+              if (!HI) ix = lo ? (rk < nu1 ? jq * SEG + rk : ZU) : (two && rk < nu2 ? jq * SEG + rk : ZU);
+              else ix = !lo && rk < nu2 ? (jq - 2) * SEG + rk - 64 : ZU;
+              uu[k] = SL[ix];
+            }
+            // ---- direct products
+            if (!HI) {
+#pragma unroll
+              for (int cc = 0; cc < 2; ++cc) {
+                dE1 = __builtin_amdgcn_mfma_f32_4x4x1f32(a1[cc].x, bv1[cc], dE1, 0, 0, 0);
+                dO1 = __builtin_amdgcn_mfma_f32_4x4x1f32(a1[cc].z, bv1[cc], dO1, 0, 0, 0);
+                dE1 = __builtin_amdgcn_mfma_f32_4x4x1f32(a1[cc].y, br1[cc], dE1, 0, 0, 0);
+                dO1 = __builtin_amdgcn_mfma_f32_4x4x1f32(a1[cc].w, br1[cc], dO1, 0, 0, 0);
+              }
+            }
+            if (two) {
+#pragma unroll
+              for (int cc = 0; cc < 2; ++cc) {
+                dE2 = __builtin_amdgcn_mfma_f32_4x4x1f32(a2[cc].x, bv2[cc], dE2, 0, 0, 0);
+                dO2 = __builtin_amdgcn_mfma_f32_4x4x1f32(a2[cc].z, bv2[cc], dO2, 0, 0, 0);
+                dE2 = __builtin_amdgcn_mfma_f32_4x4x1f32(a2[cc].y, br2[cc], dE2, 0, 0, 0);
+                dO2 = __builtin_amdgcn_mfma_f32_4x4x1f32(a2[cc].w, br2[cc], dO2, 0, 0, 0);
+              }
+            }
+            if (KT > 0) {
+              // ---- transposed products
+              hz_v4f tA = z4, tB = z4;
+#pragma unroll
+              for (int k = 0; k < KT; ++k) {
+                tA = __builtin_amdgcn_mfma_f32_4x4x1f32(va[k][0], uu[k].x, tA, 0, 0, 0);
+                tB = __builtin_amdgcn_mfma_f32_4x4x1f32(vr[k][0], uu[k].y, tB, 0, 0, 0);
+                tA = __builtin_amdgcn_mfma_f32_4x4x1f32(va[k][1], uu[k].z, tA, 0, 0, 0);
+                tB = __builtin_amdgcn_mfma_f32_4x4x1f32(vr[k][1], uu[k].w, tB, 0, 0, 0);
+              }
+              // sum over the 16 blocks (lane bits 2..5): afterwards lane l holds
+              // component (l >> 2) & 3 of column c_(l & 3)
+              const T w0 = pair_step<0x124>(tA[0] + tB[0], tA[1] + tB[1], lane & 4);
+              const T w1 = pair_step<0x124>(tA[2] + tB[2], tA[3] + tB[3], lane & 4);
+              T sm = pair_step<0x128>(w0, w1, lane & 8);
+              sm += xor16_f(sm, lane);
+              sm += xor32_f(sm, lane);
+              if (lane < 16 && (jq < 2 || two)) {
+                const int col = (jq < 2 ? J1 : J2) * TB + ca + (jq & 1);
+                tdf[(q2 * n + col) * 4 + ((lane >> 2) & 3)] = sm;
+              }
             }
           }
         }
-        __builtin_amdgcn_s_waitcnt(dma::wait_vm((NS - 1) * R));
-        __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
-        dma::raw_barrier();
-        if (tid < n) {
-          U y = make_float4(0.f, 0.f, 0.f, 0.f);
+        {
+          // the tile NW stages ahead into this slot once its reads are done
+          __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
+          asm volatile("" ::: "memory");
+          const int ta = t + NW;
+          const int tt = ta % TPS;
+          const char* blk = (ta / TPS) == 0 ? b0 : b1;
+          const bool past = (ta / TPS) != 0 && st + 1 >= nsteps;
+          tile_dma(past ? reinterpret_cast<const char*>(Pk) : blk, tt, rslot, lane);
+        }
+        rslot = rslot + 1 == NW ? 0 : rslot + 1;
+        if (t == TPS - 1) {
+          {
+            // direct partials: lane 4b+j, reg i = rows 2(rb+4b+i), +1, component j
+            const hz_v4f dE = dE1 + dE2, dO = dO1 + dO2;
 #pragma unroll
-          for (int q = 0; q < NCG; ++q) {
-            const U u = part[q * n + tid];
-            y.x += u.x; y.y += u.y; y.z += u.z; y.w += u.w;
+            for (int i = 0; i < 4; ++i) {
+              const int row = 2 * (rb + 4 * bq + i);
+              if (row < n) {
+                pf[(cg * n + row) * 4 + jq] = dE[i];
+                pf[(cg * n + row + 1) * 4 + jq] = dO[i];
+              }
+            }
           }
+          // the step's line transfers precede its TPS tiles' transfers
+          __builtin_amdgcn_s_waitcnt(dma::wait_vm(TPS * LD));
+          __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
+          dma::raw_barrier();
+          // one thread per (plane, row): the plane's partial sums, the
+          // update of the next right-hand side and the stores
+          if (tid < 2 * n) {
+            const int s = tid >= n ? 1 : 0, r = tid - s * n;
+            const C* part2 = reinterpret_cast<const C*>(part);
+            const C* tdot2 = reinterpret_cast<const C*>(tdot);
+            C ys = make_float2(0.f, 0.f);
 #pragma unroll
-          for (int q = 0; q < NQ2; ++q) {
-            const U u = tdot[q * n + tid];
-            y.x += u.x; y.y += u.y; y.z += u.z; y.w += u.w;
-          }
-          const C a = sdxi[tid];
-          const double2 cf = pscal[0], b = pscal[1];
-          const double2 di =
-              make_double2((double)a.x * b.x - (double)a.y * b.y, (double)a.x * b.y + (double)a.y * b.x);
-          T w[4];
+            for (int q = 0; q < NCG; ++q) {
+              const C u = part2[(q * n + r) * 2 + s];
+              ys.x += u.x;
+              ys.y += u.y;
+            }
 #pragma unroll
-          for (int s = 0; s < 2; ++s) {
-            const C ys = s == 0 ? make_float2(y.x, y.y) : make_float2(y.z, y.w);
+            for (int q = 0; q < 2; ++q) {
+              const C u = tdot2[(q * n + r) * 2 + s];
+              ys.x += u.x;
+              ys.y += u.y;
+            }
+            const C a = sdxi[r];
+            const double2 cf = pscal[0], b = pscal[1];
+            const double2 di =
+                make_double2((double)a.x * b.x - (double)a.y * b.y, (double)a.x * b.y + (double)a.y * b.x);
             const bool wr = s == 0 || twoB;
-            C* x = xs[s];
-            if (wr && PH == 1 && (half == 0 || st > 0)) x[(size_t)jb * n + tid] = ys;  // x_jb
-            if (wr && jn == -2) scr_at(half, s)[tid] = ys;                             // y_top / y_bot
-            C nv = make_float2(0.f, 0.f);
+            C* x = xs(s);
+            if (wr && PH == 1 && (half == 0 || st > 0)) x[(size_t)jb * n + r] = ys;  // x_jb
+            if (wr && jn == -2) scr_at(half, s)[r] = ys;                             // y_top / y_bot
             if (jn >= 0) {
-              const C pre = reinterpret_cast<const C*>(pbuf + 128 * s)[tid];
+              const C pre = reinterpret_cast<const C*>(pbuf + 128 * s)[r];
+              C nv;
               nv.x = pre.x - (T)(cf.x * ys.x - cf.y * ys.y);
               nv.y = pre.y - (T)(cf.x * ys.y + cf.y * ys.x);
-              if (wr && PH == 0) x[(size_t)jn * n + tid] = nv;  // t_jn / b_jn
+              if (wr && PH == 0) x[(size_t)jn * n + r] = nv;  // t_jn / b_jn
+              const T w0 = (T)((double)nv.x * di.x - (double)nv.y * di.y);
+              const T w1 = (T)((double)nv.x * di.y + (double)nv.y * di.x);
+              reinterpret_cast<C*>(vq)[r * 2 + s] = make_float2(w0, w1);
+              reinterpret_cast<C*>(vqr)[r * 2 + s] = make_float2(-w1, w0);
             }
-            w[2 * s] = (T)((double)nv.x * di.x - (double)nv.y * di.y);
-            w[2 * s + 1] = (T)((double)nv.x * di.y + (double)nv.y * di.x);
           }
-          if (jn >= 0) {
-            vq[tid] = make_float4(w[0], w[1], w[2], w[3]);
-            vqr[tid] = make_float4(-w[1], w[0], -w[3], w[2]);
-          }
+          __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
+          dma::raw_barrier();
         }
-        __builtin_amdgcn_s_waitcnt(dma::WAIT_LGKM0);
       }
-      rslot = (rslot + 1) & (NS - 1);
     }
-  }
-  __builtin_amdgcn_s_waitcnt(dma::wait_vm(0));
+    __builtin_amdgcn_s_waitcnt(dma::wait_vm(0));
+  };
+  if (wv < 4) run(std::false_type{});
+  else run(std::true_type{});
 }
 
 // ---- complex128 form of the step-unrolled packed solve
@@ -2753,9 +2843,9 @@ bool try_solve_twist(hz_problem* p, int nrhs, void* x, hipStream_t s) {
     const int nr = std::min(TW_RHS, nrhs - r0);
     float2* xr = reinterpret_cast<float2*>(x) + (size_t)r0 * p->N;
     const dim3 grid((unsigned)(2 * p->nslot), (unsigned)nr);
-    hipLaunchKernelGGL((k_solve_twist_mx<NX, 0>), grid, dim3(dma::NT), 0, s, P, xr, p->tab(1, 0), p->tab(1, 1),
+    hipLaunchKernelGGL((k_solve_twist_pw<NX, 0>), grid, dim3(dma::NT), 0, s, P, xr, p->tab(1, 0), p->tab(1, 1),
                        p->d_symd.as<double2>(), (int)p->ny, f.tw_m, (int)p->nz, p->n2, p->N, sp, scr);
-    hipLaunchKernelGGL((k_solve_twist_mx<NX, 1>), grid, dim3(dma::NT), 0, s, P, xr, p->tab(1, 0), p->tab(1, 1),
+    hipLaunchKernelGGL((k_solve_twist_pw<NX, 1>), grid, dim3(dma::NT), 0, s, P, xr, p->tab(1, 0), p->tab(1, 1),
                        p->d_symd.as<double2>(), (int)p->ny, f.tw_m, (int)p->nz, p->n2, p->N, sp, scr);
     HZ_HIP(hipGetLastError());
   }

@@ -2034,12 +2034,6 @@ __global__ __launch_bounds__(dma::NT) void k_solve_twist_pw(const float2* __rest
       const int jb = blk_of(st), jn = next_of(st);
 #pragma unroll
       for (int t = 0; t < TPS; ++t) {
-        // the lane index, opaque to the compiler at every tile: the tile's
-        // lane-dependent offsets are formed here instead of being kept in
-        // registers across the whole step, which leaves registers for the
-        // tile's operands to be read ahead of the products
-        const int lane = tid & 63;
-        const int bq = lane >> 2, jq = lane & 3;
         // this tile's transfers are complete once at most the NW - 1 younger
         // tiles' transfers are outstanding
         __builtin_amdgcn_s_waitcnt(dma::wait_vm((NW - 1) * LD));
@@ -2083,7 +2077,7 @@ __global__ __launch_bounds__(dma::NT) void k_solve_twist_pw(const float2* __rest
           const bool two = J2 != J1;
           if (!HI || (HAS_HI && two && Hu2 > 64)) {
             constexpr int KT = HI ? KT64 : 4;
-            // ---- every operand of the tile first (one LDS round trip per tile)
+            // ---- the tile's operands
             // direct: lane = row unit rb + lane; transposed: lane (b, j) <->
             // (row unit rb + 16 k + b, column c_j)
             U a1[2], a2[2], uu[4];

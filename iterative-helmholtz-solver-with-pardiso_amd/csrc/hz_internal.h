@@ -293,20 +293,24 @@ void launch_axpy_dev(int64_t n, const double2* coef, bool neg, bool conj_coef, c
 // w -= coef[0] * x; out = y^H w (y == nullptr: out = ||w||), one pass
 void launch_axpy_dot(hz_problem* p, int64_t n, const double2* coef, const double2* x, double2* w,
                      const double2* y, double2* out, hipStream_t s);
-// out_r[q] = V_q^H w and (vj) out_l[q] = V_q^H vj for q < nv, one pass over V
-void launch_multidot(hz_problem* p, int64_t n, int nv, const double2* V, int64_t vstride, const double2* w,
-                     const double2* vj, double2* out_r, double2* out_l, hipStream_t s);
-// w -= sum_q coef[q] V_q; out_norm = ||w||, one pass over V
-void launch_multi_axpy_norm(hz_problem* p, int64_t n, int nv, const double2* V, int64_t vstride,
-                            const double2* coef, double2* w, double2* out_norm, hipStream_t s);
 // y = x / |coef| (coef holds a norm in .x)
 void launch_scale_inv_dev(int64_t n, const double2* coef, const double2* x, double2* y, hipStream_t s);
 // y = x / coef.x if coef.x > thr, else y = 0
 void launch_scale_inv_guard(int64_t n, const double2* coef, double thr, const double2* x, double2* y, hipStream_t s);
-// ICWY correction h = (I + L)^{-1} r of Arnoldi step j < ICWY_MAX on the device
-// (stores row j of L from l = V^H V_j first); one 64-lane workgroup
+// Steps j of the device ICWY correction: j < ICWY_MAX (one 64-lane wave)
 constexpr int ICWY_MAX = 64;
-void launch_icwy(int j, int ldl, const double2* r, const double2* l, double2* L, double2* h, hipStream_t s);
+// Inverse compact-WY Gram-Schmidt of Arnoldi step j < ICWY_MAX for every
+// member of mask at once (member q: basis vectors V + q n, vs apart; w + q n;
+// step memory mem + q msz holding r, l, the L rows and the Hessenberg columns
+// at offR, offL, offLg, offH), one launch per pass: r = V^H w and
+// l = V^H V_j in one pass over V, h = (I + L)^{-1} r (row j of L stored from
+// l first), then w -= V h and h_{j+1,j} = ||w|| in a second pass, and
+// v_{j+1} = w / h_{j+1,j} written to vnext + q n (zero where h_{j+1,j} <= thr).
+// The reduction workspace must hold nr members' partials (ensure_krylov
+// sizes it).
+void launch_gs_members(hz_problem* p, int64_t n, int nr, uint64_t mask, int j, int ldh, const double2* V,
+                       int64_t vs, double2* w, double2* mem, int64_t msz, int offR, int offL, int offLg, int offH,
+                       double thr, double2* vnext, hipStream_t s);
 // Complex Givens update of Arnoldi step j on the device, one thread per member
 // q of `mask` (q < 64).  Member q's scalars start at base + q * sb: column j of
 // H (leading dimension ldh) at offH + ldh * j, the cosines at offCS, sines at

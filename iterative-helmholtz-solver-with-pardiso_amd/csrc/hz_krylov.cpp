@@ -293,11 +293,11 @@ class Lockstep {
     for (int j = j0; j < j0 + K; ++j) {
       precond_apply(p_, o_.prec, R_, Vb(j, 0), Zb(j, 0), adj_, false);
       op_apply(p_, R_, Zb(j, 0), W(), adj_);
-      for (int q = 0; q < R_; ++q) {
-        if (!((mask >> q) & 1ull)) continue;
-        double2* wq = W() + q * N_;
-        double2* hc = dH(q) + (size_t)ldh * j;
-        if (seq_) {
+      if (seq_) {
+        for (int q = 0; q < R_; ++q) {
+          if (!((mask >> q) & 1ull)) continue;
+          double2* wq = W() + q * N_;
+          double2* hc = dH(q) + (size_t)ldh * j;
           // modified Gram-Schmidt in sequence: each update fused with the next
           // coefficient's dot, the last with ||w||
           launch_dot(p_, N_, Vb(0, q), wq, HZ_MASK_FULL, hc, p_->stream);
@@ -305,16 +305,15 @@ class Lockstep {
             const double2* vnext = i < j ? Vb(i + 1, q) : nullptr;
             launch_axpy_dot(p_, N_, hc + i, Vb(i, q), wq, vnext, hc + i + 1, p_->stream);
           }
-        } else {
-          // inverse compact-WY: r = V^H w and l = V^H V_j in one pass,
-          // h = (I + L)^{-1} r, then w -= V h and ||w|| in a second pass
-          launch_multidot(p_, N_, j + 1, Vb(0, q), vstride(), wq, Vb(j, q), dMem(q) + L_.offR,
-                          dMem(q) + L_.offL, p_->stream);
-          launch_icwy(j, ldh, dMem(q) + L_.offR, dMem(q) + L_.offL, dMem(q) + L_.offLg, hc, p_->stream);
-          launch_multi_axpy_norm(p_, N_, j + 1, Vb(0, q), vstride(), hc, wq, hc + j + 1, p_->stream);
+          // v_{j+1} = w / h_{j+1,j}, zero on breakdown
+          launch_scale_inv_guard(N_, hc + j + 1, o_.breakdown, wq, Vb(j + 1, q), p_->stream);
         }
-        // v_{j+1} = w / h_{j+1,j}, zero on breakdown
-        launch_scale_inv_guard(N_, hc + j + 1, o_.breakdown, wq, Vb(j + 1, q), p_->stream);
+      } else {
+        // inverse compact-WY for every member of the mask at once: r = V^H w
+        // and l = V^H V_j in one pass, h = (I + L)^{-1} r, then w -= V h and
+        // ||w|| in a second pass, v_{j+1} = w / h_{j+1,j} (zero on breakdown)
+        launch_gs_members(p_, N_, R_, mask, j, ldh, Vb(0, 0), vstride(), W(), dTop(), (int64_t)L_.size, L_.offR,
+                          L_.offL, L_.offLg, L_.offH, o_.breakdown, Vb(j + 1, 0), p_->stream);
       }
       launch_givens(j, m_, ldh, mask, dTop(), L_.size, L_.offH, L_.offCS, L_.offSN, L_.offG, dNrm() + R_, dNf(),
                     o_.tol, o_.breakdown, dEst(), p_->stream);
@@ -442,9 +441,10 @@ void ensure_krylov(hz_problem* p, int m, int R) {
   grow(p->kx, vb);
   grow(p->kaux, vb);
   grow(p->scal, std::max<size_t>(4096, scalar_slots(m, R)) * sizeof(double2));
-  // reduction partials of the largest multi-dot of the cycle (and of any dot)
+  // reduction partials of the largest multi-dot of the cycle for R members at
+  // once (and of any dot)
   const int ng = (m + 1 + MD_GROUP - 1) / MD_GROUP;
-  grow(p->red, (size_t)RED_BLOCKS * std::max(2 * MD_GROUP * ng, 4) * sizeof(double2));
+  grow(p->red, (size_t)RED_BLOCKS * std::max(2 * MD_GROUP * ng, 4) * sizeof(double2) * R);
   ensure_work(p, R);
   if (fresh) {
     // basis slots a batch member never reaches still enter the batched

@@ -80,7 +80,12 @@ __global__ __launch_bounds__(RT) void k_dot_partial(int64_t n, const double2* __
 // out[v] = sum of RB partials (mode 1: sqrt of the real part, imaginary 0)
 // sums of nb partials per vector
 __global__ __launch_bounds__(RT) void k_dot_final(const double2* __restrict__ partial, int mode,
-                                                  double2* __restrict__ out, int nb) {
+                                                  double2* __restrict__ out, int nb, int64_t pz, int64_t oz,
+                                                  uint64_t mask) {
+  // blockIdx.z: a batch member (partials pz, outputs oz apart; members not in mask skip)
+  if (!((mask >> blockIdx.z) & 1ull)) return;
+  partial += (int64_t)blockIdx.z * pz;
+  out += (int64_t)blockIdx.z * oz;
   const int v = blockIdx.x;
   double sx = 0.0, sy = 0.0;
   for (int b = threadIdx.x; b < nb; b += RT) {
@@ -132,7 +137,14 @@ __device__ __forceinline__ double wave_sum(double v) {
 __global__ __launch_bounds__(RT) void k_multidot_partial(int64_t n, int nv, const double2* __restrict__ V,
                                                          int64_t vs, const double2* __restrict__ w,
                                                          const double2* __restrict__ vj,
-                                                         double2* __restrict__ partial) {
+                                                         double2* __restrict__ partial, int64_t ms, int64_t ps,
+                                                         uint64_t mask) {
+  // blockIdx.z: a batch member (vectors ms, partials ps apart; members not in mask skip)
+  if (!((mask >> blockIdx.z) & 1ull)) return;
+  V += (int64_t)blockIdx.z * ms;
+  w += (int64_t)blockIdx.z * ms;
+  if (vj) vj += (int64_t)blockIdx.z * ms;
+  partial += (int64_t)blockIdx.z * ps;
   const int group = blockIdx.y;
   const int q0 = group * MD;
   const int nq = min(MD, nv - q0);
@@ -176,7 +188,12 @@ __global__ __launch_bounds__(RT) void k_multidot_partial(int64_t n, int nv, cons
 
 // out_r[q], out_l[q] = sums over the RB partials of vector q
 __global__ __launch_bounds__(RT) void k_multidot_final(const double2* __restrict__ partial, int nv,
-                                                       double2* __restrict__ out_r, double2* __restrict__ out_l) {
+                                                       double2* __restrict__ out_r, double2* __restrict__ out_l,
+                                                       int64_t ps, int64_t os, uint64_t mask) {
+  if (!((mask >> blockIdx.z) & 1ull)) return;
+  partial += (int64_t)blockIdx.z * ps;
+  out_r += (int64_t)blockIdx.z * os;
+  if (out_l) out_l += (int64_t)blockIdx.z * os;
   const int q = blockIdx.x;
   const int group = q / MD, g = q % MD;
   double sx = 0, sy = 0, tx = 0, ty = 0;
@@ -198,7 +215,14 @@ __global__ __launch_bounds__(RT) void k_multidot_final(const double2* __restrict
 __global__ __launch_bounds__(RT) void k_multi_axpy_norm_partial(int64_t n, int nv, const double2* __restrict__ V,
                                                                 int64_t vs, const double2* __restrict__ coef,
                                                                 double2* __restrict__ w,
-                                                                double2* __restrict__ partial) {
+                                                                double2* __restrict__ partial, int64_t ms,
+                                                                int64_t os, uint64_t mask) {
+  // blockIdx.z: a batch member (vectors ms, coefficients os, partials gridDim.x apart)
+  if (!((mask >> blockIdx.z) & 1ull)) return;
+  V += (int64_t)blockIdx.z * ms;
+  w += (int64_t)blockIdx.z * ms;
+  coef += (int64_t)blockIdx.z * os;
+  partial += (int64_t)blockIdx.z * gridDim.x;
   extern __shared__ double2 cf[];
   for (int q = threadIdx.x; q < nv; q += RT) cf[q] = coef[q];
   __syncthreads();
@@ -246,7 +270,12 @@ __global__ void k_axpy_dev(int64_t n, const double2* __restrict__ coef, double s
 // non-finite norm leaves a finite vector)
 template <bool GUARD>
 __global__ void k_scale_inv_dev(int64_t n, const double2* __restrict__ coef, double thr, const double2* __restrict__ x,
-                                double2* __restrict__ y) {
+                                double2* __restrict__ y, int64_t xs, int64_t cs, uint64_t mask) {
+  // blockIdx.z: a batch member (vectors xs, coefficients cs apart)
+  if (!((mask >> blockIdx.z) & 1ull)) return;
+  x += (int64_t)blockIdx.z * xs;
+  y += (int64_t)blockIdx.z * xs;
+  coef += (int64_t)blockIdx.z * cs;
   const double c = coef[0].x;
   const double s = GUARD ? (c > thr ? 1.0 / c : 0.0) : 1.0 / c;
   for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += (int64_t)gridDim.x * blockDim.x) {
@@ -264,7 +293,13 @@ __global__ void k_scale_inv_dev(int64_t n, const double2* __restrict__ coef, dou
 // h_c broadcast from lane c.  Needs j < 64 and (j + 1)^2 double2 of LDS.
 __global__ __launch_bounds__(64) void k_icwy(int j, int ldl, const double2* __restrict__ r,
                                              const double2* __restrict__ l, double2* __restrict__ L,
-                                             double2* __restrict__ h) {
+                                             double2* __restrict__ h, int64_t os, uint64_t mask) {
+  // blockIdx.z: a batch member (all four arrays os apart)
+  if (!((mask >> blockIdx.z) & 1ull)) return;
+  r += (int64_t)blockIdx.z * os;
+  l += (int64_t)blockIdx.z * os;
+  L += (int64_t)blockIdx.z * os;
+  h += (int64_t)blockIdx.z * os;
   extern __shared__ double2 sL[];
   const int lane = threadIdx.x;
   const int n = j + 1;
@@ -506,7 +541,7 @@ void launch_dot_multi(hz_problem* p, int64_t n, int nv, const double2* x, int64_
   Box box = make_box(p, mask);
   hipLaunchKernelGGL(k_dot_partial, dim3(RB, nv), dim3(RT), 0, s, n, x, xstride, y, ystride, box,
                      p->red.as<double2>());
-  hipLaunchKernelGGL(k_dot_final, dim3(nv), dim3(RT), 0, s, p->red.as<double2>(), y ? 0 : 1, out, RB);
+  hipLaunchKernelGGL(k_dot_final, dim3(nv), dim3(RT), 0, s, p->red.as<double2>(), y ? 0 : 1, out, RB, 0, 0, ~0ull);
   HZ_HIP(hipGetLastError());
 }
 
@@ -519,30 +554,7 @@ void launch_axpy_dot(hz_problem* p, int64_t n, const double2* coef, const double
                      const double2* y, double2* out, hipStream_t s) {
   ensure_red(p, 1);
   hipLaunchKernelGGL(k_axpy_dot_partial, dim3(RB), dim3(RT), 0, s, n, coef, x, w, y, p->red.as<double2>());
-  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RT), 0, s, p->red.as<double2>(), y ? 0 : 1, out, RB);
-  HZ_HIP(hipGetLastError());
-}
-
-void launch_multidot(hz_problem* p, int64_t n, int nv, const double2* V, int64_t vstride, const double2* w,
-                     const double2* vj, double2* out_r, double2* out_l, hipStream_t s) {
-  const int ng = (nv + MD - 1) / MD;
-  const size_t need = (size_t)ng * RB * 2 * MD * sizeof(double2);
-  if (p->red.bytes < need) {
-    p->red.release();
-    p->red.alloc(need);
-  }
-  hipLaunchKernelGGL(k_multidot_partial, dim3(RB, ng), dim3(RT), 0, s, n, nv, V, vstride, w, vj,
-                     p->red.as<double2>());
-  hipLaunchKernelGGL(k_multidot_final, dim3(nv), dim3(RT), 0, s, p->red.as<double2>(), nv, out_r, out_l);
-  HZ_HIP(hipGetLastError());
-}
-
-void launch_multi_axpy_norm(hz_problem* p, int64_t n, int nv, const double2* V, int64_t vstride,
-                            const double2* coef, double2* w, double2* out_norm, hipStream_t s) {
-  ensure_red(p, 1);
-  hipLaunchKernelGGL(k_multi_axpy_norm_partial, dim3(RBA), dim3(RT), nv * sizeof(double2), s, n, nv, V, vstride,
-                     coef, w, p->red.as<double2>());
-  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RT), 0, s, p->red.as<double2>(), 1, out_norm, RBA);
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RT), 0, s, p->red.as<double2>(), y ? 0 : 1, out, RB, 0, 0, ~0ull);
   HZ_HIP(hipGetLastError());
 }
 
@@ -553,19 +565,38 @@ void launch_axpy_dev(int64_t n, const double2* coef, bool neg, bool conj_coef, c
 }
 
 void launch_scale_inv_dev(int64_t n, const double2* coef, const double2* x, double2* y, hipStream_t s) {
-  hipLaunchKernelGGL(k_scale_inv_dev<false>, dim3(grid_for(n)), dim3(256), 0, s, n, coef, 0.0, x, y);
+  hipLaunchKernelGGL(k_scale_inv_dev<false>, dim3(grid_for(n)), dim3(256), 0, s, n, coef, 0.0, x, y, 0, 0, ~0ull);
   HZ_HIP(hipGetLastError());
 }
 
 void launch_scale_inv_guard(int64_t n, const double2* coef, double thr, const double2* x, double2* y, hipStream_t s) {
-  hipLaunchKernelGGL(k_scale_inv_dev<true>, dim3(grid_for(n)), dim3(256), 0, s, n, coef, thr, x, y);
+  hipLaunchKernelGGL(k_scale_inv_dev<true>, dim3(grid_for(n)), dim3(256), 0, s, n, coef, thr, x, y, 0, 0, ~0ull);
   HZ_HIP(hipGetLastError());
 }
 
-void launch_icwy(int j, int ldl, const double2* r, const double2* l, double2* L, double2* h, hipStream_t s) {
+void launch_gs_members(hz_problem* p, int64_t n, int nr, uint64_t mask, int j, int ldh, const double2* V,
+                       int64_t vs, double2* w, double2* mem, int64_t msz, int offR, int offL, int offLg, int offH,
+                       double thr, double2* vnext, hipStream_t s) {
   HZ_CHECK(j >= 0 && j < ICWY_MAX, HZ_ERR_ARG, "device ICWY step %d (at most %d)", j, ICWY_MAX - 1);
-  const size_t lds = (size_t)(j + 1) * (j + 1) * sizeof(double2);
-  hipLaunchKernelGGL(k_icwy, dim3(1), dim3(64), lds, s, j, ldl, r, l, L, h);
+  HZ_CHECK(nr >= 1 && nr <= 64, HZ_ERR_ARG, "%d batch members (at most 64)", nr);
+  const int nv = j + 1;
+  const int ng = (nv + MD - 1) / MD;
+  const int64_t ps = (int64_t)ng * RB * 2 * MD;  // multi-dot partials of one member
+  HZ_CHECK(p->red.bytes >= (size_t)nr * (size_t)std::max<int64_t>(ps, RBA) * sizeof(double2), HZ_ERR_STATE,
+           "reduction workspace not sized for %d members", nr);
+  double2* red = p->red.as<double2>();
+  double2* h = mem + offH + (int64_t)ldh * j;
+  hipLaunchKernelGGL(k_multidot_partial, dim3(RB, ng, nr), dim3(RT), 0, s, n, nv, V, vs, w, V + (int64_t)j * vs, red,
+                     n, ps, mask);
+  hipLaunchKernelGGL(k_multidot_final, dim3(nv, 1, nr), dim3(RT), 0, s, red, nv, mem + offR, mem + offL, ps, msz,
+                     mask);
+  hipLaunchKernelGGL(k_icwy, dim3(1, 1, nr), dim3(64), (size_t)nv * nv * sizeof(double2), s, j, ldh, mem + offR,
+                     mem + offL, mem + offLg, h, msz, mask);
+  hipLaunchKernelGGL(k_multi_axpy_norm_partial, dim3(RBA, 1, nr), dim3(RT), nv * sizeof(double2), s, n, nv, V, vs, h,
+                     w, red, n, msz, mask);
+  hipLaunchKernelGGL(k_dot_final, dim3(1, 1, nr), dim3(RT), 0, s, red, 1, h + nv, RBA, (int64_t)RBA, msz, mask);
+  hipLaunchKernelGGL(k_scale_inv_dev<true>, dim3(grid_for(n), 1, nr), dim3(256), 0, s, n, h + nv, thr, w, vnext, n, msz,
+                     mask);
   HZ_HIP(hipGetLastError());
 }
 

@@ -2000,7 +2000,7 @@ __global__ __launch_bounds__(dma::NT) void k_solve_twist_pw(const float2* __rest
     const unsigned wr_lds = ring_lds + (unsigned)wbase * 16u;
     const int ca = cg * 2;
     // the wave's transfers of tile tt of a block into ring slot sl
-    auto tile_dma = [&](const char* blk, int tt, int sl, int lane) {
+    auto tile_dma = [&](const char* blk, int tt, int sl) {
       const int J1 = tt, J2 = T_ - 1 - tt;
       const bool two = J2 != J1;
       const int Hu1 = (J1 + 1) * TB / RPL, Hu2 = (J2 + 1) * TB / RPL;
@@ -2023,7 +2023,7 @@ __global__ __launch_bounds__(dma::NT) void k_solve_twist_pw(const float2* __rest
       for (int i = 0; i < LD; ++i) dma16_untracked(base + (size_t)u[i] * 16, dst + (unsigned)(i * SEG * 16));
     };
 #pragma unroll
-    for (int g = 0; g < NW; ++g) tile_dma(blk_addr(0), g, g, lane);
+    for (int g = 0; g < NW; ++g) tile_dma(blk_addr(0), g, g);
 
     hz_v4f dE1 = z4, dO1 = z4, dE2 = z4, dO2 = z4;  // direct: even / odd rows, block columns J1 / J2
     T va[4][2], vr[4][2];                          // transposed A operands of the step
@@ -2159,7 +2159,7 @@ __global__ __launch_bounds__(dma::NT) void k_solve_twist_pw(const float2* __rest
           const int tt = ta % TPS;
           const char* blk = (ta / TPS) == 0 ? b0 : b1;
           const bool past = (ta / TPS) != 0 && st + 1 >= nsteps;
-          tile_dma(past ? reinterpret_cast<const char*>(Pk) : blk, tt, rslot, lane);
+          tile_dma(past ? reinterpret_cast<const char*>(Pk) : blk, tt, rslot);
         }
         rslot = rslot + 1 == NW ? 0 : rslot + 1;
         if (t == TPS - 1) {

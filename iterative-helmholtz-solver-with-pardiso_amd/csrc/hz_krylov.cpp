@@ -37,9 +37,10 @@ namespace {
 
 // Device scalars of one lockstep member (double2 slots), in this order:
 // H (ldh x m, column-major), y (m), r = V^H w (m + 1), l = V^H V_j (m + 1),
-// the ICWY factor L (ldh x ldh), Givens cosines (m) and sines (m), g (m + 1).
+// the ICWY factor L (ldh x ldh), Givens cosines (m) and sines (m), g (m + 1),
+// the basis scales s_k (m + 1; .x, v_k = s_k U_k for the stored vectors U_k).
 struct ScalarLayout {
-  int ldh, offH, offY, offR, offL, offLg, offCS, offSN, offG, size;
+  int ldh, offH, offY, offR, offL, offLg, offCS, offSN, offG, offS, size;
   explicit ScalarLayout(int m) {
     ldh = m + 1;
     offH = 0;
@@ -50,7 +51,8 @@ struct ScalarLayout {
     offCS = offLg + ldh * ldh;
     offSN = offCS + m;
     offG = offSN + m;
-    size = ((offG + m + 1 + 7) / 8) * 8;
+    offS = offG + m + 1;
+    size = ((offS + m + 1 + 7) / 8) * 8;
   }
 };
 
@@ -310,10 +312,11 @@ class Lockstep {
         }
       } else {
         // inverse compact-WY for every member of the mask at once: r = V^H w
-        // and l = V^H V_j in one pass, h = (I + L)^{-1} r, then w -= V h and
-        // ||w|| in a second pass, v_{j+1} = w / h_{j+1,j} (zero on breakdown)
+        // and l = V^H v_j in one pass, h = (I + L)^{-1} r, then the new basis
+        // vector w - V h and h_{j+1,j} = its norm in a second pass; the vector
+        // is stored unnormalised with its scale 1 / h_{j+1,j} (0 on breakdown)
         launch_gs_members(p_, N_, R_, mask, j, ldh, Vb(0, 0), vstride(), W(), dTop(), (int64_t)L_.size, L_.offR,
-                          L_.offL, L_.offLg, L_.offH, o_.breakdown, Vb(j + 1, 0), p_->stream);
+                          L_.offL, L_.offLg, L_.offH, L_.offS, o_.breakdown, Vb(j + 1, 0), p_->stream);
       }
       launch_givens(j, m_, ldh, mask, dTop(), L_.size, L_.offH, L_.offCS, L_.offSN, L_.offG, dNrm() + R_, dNf(),
                     o_.tol, o_.breakdown, dEst(), p_->stream);
@@ -390,7 +393,8 @@ class Lockstep {
       Member& a = M_[q];
       a.in_cycle = false;
       if (a.done || a.jj == 0) continue;
-      launch_backsub(a.jj, L_.ldh, dMem(q) + L_.offH, dMem(q) + L_.offG, dMem(q) + L_.offY, p_->stream);
+      launch_backsub(a.jj, L_.ldh, dMem(q) + L_.offH, dMem(q) + L_.offG, dMem(q) + L_.offY,
+                     seq_ ? nullptr : dMem(q) + L_.offS, p_->stream);
       launch_lincomb(N_, a.jj, dMem(q) + L_.offY, Zb(0, q), vstride(), x_ + q * N_, true, p_->stream);
       a.jj = 0;
       if (a.brk) a.done = true;

@@ -96,6 +96,26 @@ __global__ __launch_bounds__(RT) void k_dot_final(const double2* __restrict__ pa
   if (threadIdx.x == 0) out[v] = mode == 1 ? make_double2(sqrt(s.x), 0.0) : s;
 }
 
+// h = sqrt(sum of nb partials) = h_{jv+1,jv}, written to out; the next basis
+// vector's scale S[jv + 1].x = 1 / h, or 0 where h <= thr (breakdown).
+// blockIdx.z: a batch member (partials pz, out and S oz apart)
+__global__ __launch_bounds__(RT) void k_norm_scale_final(const double2* __restrict__ partial, int nb, int64_t pz,
+                                                         double2* __restrict__ out, double2* __restrict__ S,
+                                                         int64_t oz, int jv, double thr, uint64_t mask) {
+  if (!((mask >> blockIdx.z) & 1ull)) return;
+  partial += (int64_t)blockIdx.z * pz;
+  out += (int64_t)blockIdx.z * oz;
+  S += (int64_t)blockIdx.z * oz;
+  double sx = 0.0;
+  for (int b = threadIdx.x; b < nb; b += RT) sx += partial[b].x;
+  const double2 s = block_sum(make_double2(sx, 0.0));
+  if (threadIdx.x == 0) {
+    const double h = sqrt(s.x);
+    out[0] = make_double2(h, 0.0);
+    S[jv + 1] = make_double2(h > thr ? 1.0 / h : 0.0, 0.0);
+  }
+}
+
 // w -= coef * x, then partial[b] = sum conj(y) w  (y == nullptr: sum |w|^2):
 // one modified Gram-Schmidt update fused with the next coefficient's dot.
 __global__ __launch_bounds__(RT) void k_axpy_dot_partial(int64_t n, const double2* __restrict__ coef,
@@ -187,13 +207,19 @@ __global__ __launch_bounds__(RT) void k_multidot_partial(int64_t n, int nv, cons
 }
 
 // out_r[q], out_l[q] = sums over the RB partials of vector q
+__device__ __forceinline__ double basis_scale(const double2* S, int k) { return k == 0 ? 1.0 : S[k].x; }
+
+// out_r[q] = s_q s_jv (sum of vector q's r partials), out_l likewise: the
+// dots of the stored basis vectors U_q (v_q = s_q U_q, s_0 = 1, S[k].x = s_k)
 __global__ __launch_bounds__(RT) void k_multidot_final(const double2* __restrict__ partial, int nv,
                                                        double2* __restrict__ out_r, double2* __restrict__ out_l,
-                                                       int64_t ps, int64_t os, uint64_t mask) {
+                                                       int64_t ps, int64_t os, uint64_t mask,
+                                                       const double2* __restrict__ S, int jv) {
   if (!((mask >> blockIdx.z) & 1ull)) return;
   partial += (int64_t)blockIdx.z * ps;
   out_r += (int64_t)blockIdx.z * os;
   if (out_l) out_l += (int64_t)blockIdx.z * os;
+  S += (int64_t)blockIdx.z * os;
   const int q = blockIdx.x;
   const int group = q / MD, g = q % MD;
   double sx = 0, sy = 0, tx = 0, ty = 0;
@@ -206,30 +232,42 @@ __global__ __launch_bounds__(RT) void k_multidot_final(const double2* __restrict
   __syncthreads();
   const double2 t = block_sum(make_double2(tx, ty));
   if (threadIdx.x == 0) {
-    out_r[q] = s;
-    if (out_l) out_l[q] = t;
+    const double f = basis_scale(S, q) * basis_scale(S, jv);
+    out_r[q] = make_double2(s.x * f, s.y * f);
+    if (out_l) out_l[q] = make_double2(t.x * f, t.y * f);
   }
 }
 
-// w -= sum_q coef_q V_q, partial[b] = sum |w|^2
+// y = s_jv w - sum_q coef_q s_q U_q (stored basis vectors U_q, v_q = s_q U_q),
+// partial[b] = sum |y|^2
 __global__ __launch_bounds__(RT) void k_multi_axpy_norm_partial(int64_t n, int nv, const double2* __restrict__ V,
                                                                 int64_t vs, const double2* __restrict__ coef,
-                                                                double2* __restrict__ w,
+                                                                const double2* __restrict__ w,
                                                                 double2* __restrict__ partial, int64_t ms,
-                                                                int64_t os, uint64_t mask) {
+                                                                int64_t os, uint64_t mask,
+                                                                const double2* __restrict__ S, int jv,
+                                                                double2* __restrict__ y) {
   // blockIdx.z: a batch member (vectors ms, coefficients os, partials gridDim.x apart)
   if (!((mask >> blockIdx.z) & 1ull)) return;
   V += (int64_t)blockIdx.z * ms;
   w += (int64_t)blockIdx.z * ms;
+  y += (int64_t)blockIdx.z * ms;
   coef += (int64_t)blockIdx.z * os;
+  S += (int64_t)blockIdx.z * os;
   partial += (int64_t)blockIdx.z * gridDim.x;
   extern __shared__ double2 cf[];
-  for (int q = threadIdx.x; q < nv; q += RT) cf[q] = coef[q];
+  for (int q = threadIdx.x; q < nv; q += RT) {
+    const double2 c = coef[q];
+    const double sq = basis_scale(S, q);
+    cf[q] = make_double2(c.x * sq, c.y * sq);
+  }
+  const double sj = basis_scale(S, jv);
   __syncthreads();
   double acc = 0.0;
   const int64_t stride = (int64_t)gridDim.x * RT;
   for (int64_t l = (int64_t)blockIdx.x * RT + threadIdx.x; l < n; l += stride) {
-    double2 s = w[l];
+    const double2 wl = w[l];
+    double2 s = make_double2(wl.x * sj, wl.y * sj);
     int q = 0;
     for (; q + 3 < nv; q += 4) {
       const double2 a0 = ldv(V + (int64_t)q * vs + l), a1 = ldv(V + (int64_t)(q + 1) * vs + l);
@@ -245,7 +283,7 @@ __global__ __launch_bounds__(RT) void k_multi_axpy_norm_partial(int64_t n, int n
       s.x -= c.x * a.x - c.y * a.y;
       s.y -= c.x * a.y + c.y * a.x;
     }
-    w[l] = s;
+    y[l] = s;
     acc += s.x * s.x + s.y * s.y;
   }
   const double2 t = block_sum(make_double2(acc, 0.0));
@@ -409,9 +447,11 @@ __global__ __launch_bounds__(64) void k_givens(int j, int m, int ldh, uint64_t m
 }
 
 // y_i = (g_i - sum_{c > i} R_ic y_c) / R_ii for i = jj-1 .. 0 (R: the rotated
-// Hessenberg matrix, leading dimension ldh)
+// Hessenberg matrix, leading dimension ldh); with S, each y_i is then
+// multiplied by the basis scale s_i (the solution update combines the
+// preconditioned stored vectors M U_i, v_i = s_i U_i)
 __global__ void k_backsub(int jj, int ldh, const double2* __restrict__ H, const double2* __restrict__ g,
-                          double2* __restrict__ y) {
+                          double2* __restrict__ y, const double2* __restrict__ S) {
   if (threadIdx.x != 0) return;
   for (int i = jj - 1; i >= 0; --i) {
     double2 s = g[i];
@@ -422,6 +462,11 @@ __global__ void k_backsub(int jj, int ldh, const double2* __restrict__ H, const 
     }
     y[i] = zdiv(s, H[i + (int64_t)ldh * i]);
   }
+  if (S)
+    for (int i = 0; i < jj; ++i) {
+      const double f = basis_scale(S, i);
+      y[i] = make_double2(y[i].x * f, y[i].y * f);
+    }
 }
 
 __global__ void k_axpby(int64_t n, double2 a, const double2* __restrict__ x, double2 b, double2* __restrict__ y) {
@@ -575,8 +620,8 @@ void launch_scale_inv_guard(int64_t n, const double2* coef, double thr, const do
 }
 
 void launch_gs_members(hz_problem* p, int64_t n, int nr, uint64_t mask, int j, int ldh, const double2* V,
-                       int64_t vs, double2* w, double2* mem, int64_t msz, int offR, int offL, int offLg, int offH,
-                       double thr, double2* vnext, hipStream_t s) {
+                       int64_t vs, const double2* w, double2* mem, int64_t msz, int offR, int offL, int offLg,
+                       int offH, int offS, double thr, double2* vnext, hipStream_t s) {
   HZ_CHECK(j >= 0 && j < ICWY_MAX, HZ_ERR_ARG, "device ICWY step %d (at most %d)", j, ICWY_MAX - 1);
   HZ_CHECK(nr >= 1 && nr <= 64, HZ_ERR_ARG, "%d batch members (at most 64)", nr);
   const int nv = j + 1;
@@ -589,14 +634,13 @@ void launch_gs_members(hz_problem* p, int64_t n, int nr, uint64_t mask, int j, i
   hipLaunchKernelGGL(k_multidot_partial, dim3(RB, ng, nr), dim3(RT), 0, s, n, nv, V, vs, w, V + (int64_t)j * vs, red,
                      n, ps, mask);
   hipLaunchKernelGGL(k_multidot_final, dim3(nv, 1, nr), dim3(RT), 0, s, red, nv, mem + offR, mem + offL, ps, msz,
-                     mask);
+                     mask, mem + offS, j);
   hipLaunchKernelGGL(k_icwy, dim3(1, 1, nr), dim3(64), (size_t)nv * nv * sizeof(double2), s, j, ldh, mem + offR,
                      mem + offL, mem + offLg, h, msz, mask);
   hipLaunchKernelGGL(k_multi_axpy_norm_partial, dim3(RBA, 1, nr), dim3(RT), nv * sizeof(double2), s, n, nv, V, vs, h,
-                     w, red, n, msz, mask);
-  hipLaunchKernelGGL(k_dot_final, dim3(1, 1, nr), dim3(RT), 0, s, red, 1, h + nv, RBA, (int64_t)RBA, msz, mask);
-  hipLaunchKernelGGL(k_scale_inv_dev<true>, dim3(grid_for(n), 1, nr), dim3(256), 0, s, n, h + nv, thr, w, vnext, n, msz,
-                     mask);
+                     w, red, n, msz, mask, mem + offS, j, vnext);
+  hipLaunchKernelGGL(k_norm_scale_final, dim3(1, 1, nr), dim3(RT), 0, s, red, RBA, (int64_t)RBA, h + nv, mem + offS,
+                     msz, j, thr, mask);
   HZ_HIP(hipGetLastError());
 }
 
@@ -609,9 +653,10 @@ void launch_givens(int j, int m, int ldh, uint64_t mask, double2* base, int64_t 
   HZ_HIP(hipGetLastError());
 }
 
-void launch_backsub(int jj, int ldh, const double2* H, const double2* g, double2* y, hipStream_t s) {
+void launch_backsub(int jj, int ldh, const double2* H, const double2* g, double2* y, const double2* S,
+                    hipStream_t s) {
   HZ_CHECK(jj >= 1 && jj < ldh, HZ_ERR_ARG, "back substitution over %d columns (ldh %d)", jj, ldh);
-  hipLaunchKernelGGL(k_backsub, dim3(1), dim3(64), 0, s, jj, ldh, H, g, y);
+  hipLaunchKernelGGL(k_backsub, dim3(1), dim3(64), 0, s, jj, ldh, H, g, y, S);
   HZ_HIP(hipGetLastError());
 }
 

@@ -299,22 +299,26 @@ void launch_scale_inv_dev(int64_t n, const double2* coef, const double2* x, doub
 void launch_scale_inv_guard(int64_t n, const double2* coef, double thr, const double2* x, double2* y, hipStream_t s);
 // Steps j of the device ICWY correction: j < ICWY_MAX (one 64-lane wave)
 constexpr int ICWY_MAX = 64;
-// Inverse compact-WY Gram-Schmidt of Arnoldi step j < ICWY_MAX for every
-// member of mask at once, one launch per pass, on stored basis vectors U_k
-// with v_k = s_k U_k (U_0 = v_0, s_0 = 1; s_k for k >= 1 in the step memory
-// at offS).  Member q: basis vectors V + q n, vs apart; w = A M U_j at w + q n;
-// step memory mem + q msz holding r, l, the L rows, the Hessenberg columns and
-// the scales at offR, offL, offLg, offH, offS.  r = V^H w and l = V^H v_j in
-// one pass over the basis (scaled by s_i s_j), h = (I + L)^{-1} r (row j of L
-// stored from l first), then U_{j+1} = s_j w - sum h_i s_i U_i written to
-// vnext + q n with h_{j+1,j} = ||U_{j+1}|| in a second pass, and
-// s_{j+1} = 1 / h_{j+1,j} (0 where h_{j+1,j} <= thr).  No pass normalises the
-// new vector: its scale is folded into the next step's dots and coefficients
-// and into the solution update (launch_backsub).  The reduction workspace
-// must hold nr members' partials (ensure_krylov sizes it).
-void launch_gs_members(hz_problem* p, int64_t n, int nr, uint64_t mask, int j, int ldh, const double2* V,
+// Arnoldi step j < min(ICWY_MAX, m) after w = A M U_j, for every member of
+// mask at once, on stored basis vectors U_k with v_k = s_k U_k (U_0 = v_0,
+// s_0 = 1; s_k for k >= 1 in the step memory at offS).  Member q: basis
+// vectors V + q n, vs apart; w at w + q n; step memory base mem + q msz holding
+// r, l, the L rows, the Hessenberg columns, the scales, the Givens cosines and
+// sines and g at offR, offL, offLg, offH, offS, offCS, offSN, offG.  Five
+// launches: r = V^H w and l = V^H v_j in one pass over the basis; their sums
+// (scaled by s_i s_j), one block per vector; h = (I + L)^{-1} r (row j of L
+// stored from l), one wave per member; U_{j+1} = s_j w - sum h_i s_i U_i
+// written to vnext + q n with the partial sums of ||U_{j+1}||^2 in a second
+// pass; then, one block per member, h_{j+1,j} = ||U_{j+1}||, s_{j+1} =
+// 1 / h_{j+1,j} (0 where h_{j+1,j} <= brk) and the Givens update of
+// launch_givens (beta, nf, tol, est as there).  No pass normalises the new
+// vector: its scale is folded into the next step's dots and coefficients and
+// into the solution update (launch_backsub).  The reduction workspace must
+// hold nr members' partials (ensure_krylov sizes it).
+void launch_gs_members(hz_problem* p, int64_t n, int nr, uint64_t mask, int j, int m, int ldh, const double2* V,
                        int64_t vs, const double2* w, double2* mem, int64_t msz, int offR, int offL, int offLg,
-                       int offH, int offS, double thr, double2* vnext, hipStream_t s);
+                       int offH, int offS, int offCS, int offSN, int offG, const double2* beta, const double2* nf,
+                       double tol, double brk, double2* est, double2* vnext, hipStream_t s);
 // Complex Givens update of Arnoldi step j on the device, one thread per member
 // q of `mask` (q < 64).  Member q's scalars start at base + q * sb: column j of
 // H (leading dimension ldh) at offH + ldh * j, the cosines at offCS, sines at
@@ -324,7 +328,8 @@ void launch_gs_members(hz_problem* p, int64_t n, int nr, uint64_t mask, int j, i
 void launch_givens(int j, int m, int ldh, uint64_t mask, double2* base, int64_t sb, int offH, int offCS, int offSN,
                    int offG, const double2* beta, const double2* nf, double tol, double brk, double2* est,
                    hipStream_t s);
-// y = R^{-1} g over the leading jj x jj triangle of the rotated H (one thread)
+// y = R^{-1} g over the leading jj x jj triangle of the rotated H (one thread;
+// R and y in LDS for jj <= 63)
 // (S != nullptr: y_i *= s_i afterwards, S[i].x = s_i for i >= 1, s_0 = 1)
 void launch_backsub(int jj, int ldh, const double2* H, const double2* g, double2* y, const double2* S,
                     hipStream_t s);

@@ -310,16 +310,18 @@ class Lockstep {
           // v_{j+1} = w / h_{j+1,j}, zero on breakdown
           launch_scale_inv_guard(N_, hc + j + 1, o_.breakdown, wq, Vb(j + 1, q), p_->stream);
         }
+        launch_givens(j, m_, ldh, mask, dTop(), L_.size, L_.offH, L_.offCS, L_.offSN, L_.offG, dNrm() + R_,
+                      dNf(), o_.tol, o_.breakdown, dEst(), p_->stream);
       } else {
         // inverse compact-WY for every member of the mask at once: r = V^H w
         // and l = V^H v_j in one pass, h = (I + L)^{-1} r, then the new basis
         // vector w - V h and h_{j+1,j} = its norm in a second pass; the vector
-        // is stored unnormalised with its scale 1 / h_{j+1,j} (0 on breakdown)
-        launch_gs_members(p_, N_, R_, mask, j, ldh, Vb(0, 0), vstride(), W(), dTop(), (int64_t)L_.size, L_.offR,
-                          L_.offL, L_.offLg, L_.offH, L_.offS, o_.breakdown, Vb(j + 1, 0), p_->stream);
+        // is stored unnormalised with its scale 1 / h_{j+1,j} (0 on breakdown);
+        // the Givens update closes the step
+        launch_gs_members(p_, N_, R_, mask, j, m_, ldh, Vb(0, 0), vstride(), W(), dTop(), (int64_t)L_.size,
+                          L_.offR, L_.offL, L_.offLg, L_.offH, L_.offS, L_.offCS, L_.offSN, L_.offG, dNrm() + R_,
+                          dNf(), o_.tol, o_.breakdown, dEst(), Vb(j + 1, 0), p_->stream);
       }
-      launch_givens(j, m_, ldh, mask, dTop(), L_.size, L_.offH, L_.offCS, L_.offSN, L_.offG, dNrm() + R_, dNf(),
-                    o_.tol, o_.breakdown, dEst(), p_->stream);
     }
   }
 

@@ -96,26 +96,6 @@ __global__ __launch_bounds__(RT) void k_dot_final(const double2* __restrict__ pa
   if (threadIdx.x == 0) out[v] = mode == 1 ? make_double2(sqrt(s.x), 0.0) : s;
 }
 
-// h = sqrt(sum of nb partials) = h_{jv+1,jv}, written to out; the next basis
-// vector's scale S[jv + 1].x = 1 / h, or 0 where h <= thr (breakdown).
-// blockIdx.z: a batch member (partials pz, out and S oz apart)
-__global__ __launch_bounds__(RT) void k_norm_scale_final(const double2* __restrict__ partial, int nb, int64_t pz,
-                                                         double2* __restrict__ out, double2* __restrict__ S,
-                                                         int64_t oz, int jv, double thr, uint64_t mask) {
-  if (!((mask >> blockIdx.z) & 1ull)) return;
-  partial += (int64_t)blockIdx.z * pz;
-  out += (int64_t)blockIdx.z * oz;
-  S += (int64_t)blockIdx.z * oz;
-  double sx = 0.0;
-  for (int b = threadIdx.x; b < nb; b += RT) sx += partial[b].x;
-  const double2 s = block_sum(make_double2(sx, 0.0));
-  if (threadIdx.x == 0) {
-    const double h = sqrt(s.x);
-    out[0] = make_double2(h, 0.0);
-    S[jv + 1] = make_double2(h > thr ? 1.0 / h : 0.0, 0.0);
-  }
-}
-
 // w -= coef * x, then partial[b] = sum conj(y) w  (y == nullptr: sum |w|^2):
 // one modified Gram-Schmidt update fused with the next coefficient's dot.
 __global__ __launch_bounds__(RT) void k_axpy_dot_partial(int64_t n, const double2* __restrict__ coef,
@@ -206,7 +186,6 @@ __global__ __launch_bounds__(RT) void k_multidot_partial(int64_t n, int nv, cons
   }
 }
 
-// out_r[q], out_l[q] = sums over the RB partials of vector q
 __device__ __forceinline__ double basis_scale(const double2* S, int k) { return k == 0 ? 1.0 : S[k].x; }
 
 // out_r[q] = s_q s_jv (sum of vector q's r partials), out_l likewise: the
@@ -388,17 +367,13 @@ __device__ __forceinline__ double2 zdiv(double2 a, double2 b) {
   return make_double2((a.x * r + a.y) / d, (a.y * r - a.x) / d);
 }
 
-// Complex Givens least-squares update of Arnoldi step j (the per-iteration
-// rotation of FGMRES's Hessenberg column and of g = ||r|| e_1), one thread per
-// member of `mask`.  The earlier rotations (c_i real, s_i complex) act on
-// rows (i, i+1) as [c s; -conj(s) c]; the new one zeroes h_{j+1,j}.
-__global__ __launch_bounds__(64) void k_givens(int j, int m, int ldh, uint64_t mask, double2* __restrict__ base,
-                                              int64_t sb, int offH, int offCS, int offSN, int offG,
-                                              const double2* __restrict__ beta, const double2* __restrict__ nf,
-                                              double tol, double brk, double2* __restrict__ est) {
-  const int q = threadIdx.x;
-  if (q >= 64 || !((mask >> q) & 1ull)) return;
-  double2* mb = base + sb * q;
+// Complex Givens least-squares update of Arnoldi step j of member q (the
+// per-iteration rotation of FGMRES's Hessenberg column and of g = ||r|| e_1);
+// mb: the member's scalars.  The earlier rotations (c_i real, s_i complex) act
+// on rows (i, i+1) as [c s; -conj(s) c]; the new one zeroes h_{j+1,j}.
+__device__ void givens_member(int q, int j, int m, int ldh, double2* mb, int offH, int offCS, int offSN, int offG,
+                              const double2* __restrict__ beta, const double2* __restrict__ nf, double tol,
+                              double brk, double2* __restrict__ est) {
   double2* H = mb + offH + (int64_t)ldh * j;
   double2* cs = mb + offCS;
   double2* sn = mb + offSN;
@@ -446,27 +421,76 @@ __global__ __launch_bounds__(64) void k_givens(int j, int m, int ldh, uint64_t m
   est[(int64_t)q * m + j] = make_double2(e, status);
 }
 
+// The Givens update of step j, one thread per member q of `mask` (member q's
+// scalars at base + q sb)
+__global__ __launch_bounds__(64) void k_givens(int j, int m, int ldh, uint64_t mask, double2* __restrict__ base,
+                                              int64_t sb, int offH, int offCS, int offSN, int offG,
+                                              const double2* __restrict__ beta, const double2* __restrict__ nf,
+                                              double tol, double brk, double2* __restrict__ est) {
+  const int q = threadIdx.x;
+  if (q >= 64 || !((mask >> q) & 1ull)) return;
+  givens_member(q, j, m, ldh, base + sb * q, offH, offCS, offSN, offG, beta, nf, tol, brk, est);
+}
+
+// The end of Arnoldi step j in one block per member q = blockIdx.z of `mask`:
+// h_{j+1,j} = sqrt(sum of the nb update-pass partials, partial + q nb), the
+// next basis vector's scale S[j + 1].x = 1 / h_{j+1,j} (0 where h_{j+1,j} <=
+// brk, a breakdown), then the member's Givens update
+__global__ __launch_bounds__(RT) void k_norm_givens(const double2* __restrict__ partial, int nb, int j, int m,
+                                                    int ldh, uint64_t mask, double2* __restrict__ base, int64_t sb,
+                                                    int offH, int offS, int offCS, int offSN, int offG,
+                                                    const double2* __restrict__ beta, const double2* __restrict__ nf,
+                                                    double tol, double brk, double2* __restrict__ est) {
+  const int q = blockIdx.z;
+  if (!((mask >> q) & 1ull)) return;
+  partial += (int64_t)q * nb;
+  double sx = 0.0;
+  for (int b = threadIdx.x; b < nb; b += RT) sx += partial[b].x;
+  const double2 s = block_sum(make_double2(sx, 0.0));
+  if (threadIdx.x != 0) return;
+  double2* mb = base + sb * q;
+  const double hn = sqrt(s.x);
+  mb[offH + (int64_t)ldh * j + j + 1] = make_double2(hn, 0.0);
+  mb[offS + j + 1] = make_double2(hn > brk ? 1.0 / hn : 0.0, 0.0);
+  givens_member(q, j, m, ldh, mb, offH, offCS, offSN, offG, beta, nf, tol, brk, est);
+}
+
 // y_i = (g_i - sum_{c > i} R_ic y_c) / R_ii for i = jj-1 .. 0 (R: the rotated
 // Hessenberg matrix, leading dimension ldh); with S, each y_i is then
 // multiplied by the basis scale s_i (the solution update combines the
 // preconditioned stored vectors M U_i, v_i = s_i U_i)
 __global__ void k_backsub(int jj, int ldh, const double2* __restrict__ H, const double2* __restrict__ g,
-                          double2* __restrict__ y, const double2* __restrict__ S) {
+                          double2* __restrict__ y, const double2* __restrict__ S, int stage) {
+  // stage: the block's threads copy R (jj x jj) into LDS first, and y is
+  // formed there, so the serial loop waits on LDS rather than on memory
+  extern __shared__ double2 sm[];
+  const double2* R = H;
+  double2* yy = y;
+  int ld = ldh;
+  if (stage) {
+    for (int e = threadIdx.x; e < jj * jj; e += blockDim.x) {
+      const int c = e / jj, i = e - c * jj;
+      sm[e] = H[i + (int64_t)ldh * c];
+    }
+    __syncthreads();
+    R = sm;
+    yy = sm + jj * jj;
+    ld = jj;
+  }
   if (threadIdx.x != 0) return;
   for (int i = jj - 1; i >= 0; --i) {
     double2 s = g[i];
     for (int c = i + 1; c < jj; ++c) {
-      const double2 t = zmul(H[i + (int64_t)ldh * c], y[c]);
+      const double2 t = zmul(R[i + (int64_t)ld * c], yy[c]);
       s.x -= t.x;
       s.y -= t.y;
     }
-    y[i] = zdiv(s, H[i + (int64_t)ldh * i]);
+    yy[i] = zdiv(s, R[i + (int64_t)ld * i]);
   }
-  if (S)
-    for (int i = 0; i < jj; ++i) {
-      const double f = basis_scale(S, i);
-      y[i] = make_double2(y[i].x * f, y[i].y * f);
-    }
+  for (int i = 0; i < jj; ++i) {
+    const double f = S ? basis_scale(S, i) : 1.0;
+    y[i] = S ? make_double2(yy[i].x * f, yy[i].y * f) : yy[i];
+  }
 }
 
 __global__ void k_axpby(int64_t n, double2 a, const double2* __restrict__ x, double2 b, double2* __restrict__ y) {
@@ -619,10 +643,12 @@ void launch_scale_inv_guard(int64_t n, const double2* coef, double thr, const do
   HZ_HIP(hipGetLastError());
 }
 
-void launch_gs_members(hz_problem* p, int64_t n, int nr, uint64_t mask, int j, int ldh, const double2* V,
+void launch_gs_members(hz_problem* p, int64_t n, int nr, uint64_t mask, int j, int m, int ldh, const double2* V,
                        int64_t vs, const double2* w, double2* mem, int64_t msz, int offR, int offL, int offLg,
-                       int offH, int offS, double thr, double2* vnext, hipStream_t s) {
-  HZ_CHECK(j >= 0 && j < ICWY_MAX, HZ_ERR_ARG, "device ICWY step %d (at most %d)", j, ICWY_MAX - 1);
+                       int offH, int offS, int offCS, int offSN, int offG, const double2* beta, const double2* nf,
+                       double tol, double brk, double2* est, double2* vnext, hipStream_t s) {
+  HZ_CHECK(j >= 0 && j < ICWY_MAX && j < m, HZ_ERR_ARG, "device ICWY step %d (at most %d, m %d)", j,
+           ICWY_MAX - 1, m);
   HZ_CHECK(nr >= 1 && nr <= 64, HZ_ERR_ARG, "%d batch members (at most 64)", nr);
   const int nv = j + 1;
   const int ng = (nv + MD - 1) / MD;
@@ -639,8 +665,8 @@ void launch_gs_members(hz_problem* p, int64_t n, int nr, uint64_t mask, int j, i
                      mem + offL, mem + offLg, h, msz, mask);
   hipLaunchKernelGGL(k_multi_axpy_norm_partial, dim3(RBA, 1, nr), dim3(RT), nv * sizeof(double2), s, n, nv, V, vs, h,
                      w, red, n, msz, mask, mem + offS, j, vnext);
-  hipLaunchKernelGGL(k_norm_scale_final, dim3(1, 1, nr), dim3(RT), 0, s, red, RBA, (int64_t)RBA, h + nv, mem + offS,
-                     msz, j, thr, mask);
+  hipLaunchKernelGGL(k_norm_givens, dim3(1, 1, nr), dim3(RT), 0, s, red, RBA, j, m, ldh, mask, mem, msz, offH, offS,
+                     offCS, offSN, offG, beta, nf, tol, brk, est);
   HZ_HIP(hipGetLastError());
 }
 
@@ -656,7 +682,10 @@ void launch_givens(int j, int m, int ldh, uint64_t mask, double2* base, int64_t 
 void launch_backsub(int jj, int ldh, const double2* H, const double2* g, double2* y, const double2* S,
                     hipStream_t s) {
   HZ_CHECK(jj >= 1 && jj < ldh, HZ_ERR_ARG, "back substitution over %d columns (ldh %d)", jj, ldh);
-  hipLaunchKernelGGL(k_backsub, dim3(1), dim3(64), 0, s, jj, ldh, H, g, y, S);
+  // R and y staged in LDS up to 64 KB (jj <= 63)
+  const size_t lds = (size_t)(jj * jj + jj) * sizeof(double2);
+  const int stage = lds <= 64 * 1024 ? 1 : 0;
+  hipLaunchKernelGGL(k_backsub, dim3(1), dim3(256), stage ? lds : 0, s, jj, ldh, H, g, y, S, stage);
   HZ_HIP(hipGetLastError());
 }
 

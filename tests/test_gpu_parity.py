@@ -689,6 +689,26 @@ def test_fgmres_fp64_and_mixed_vs_oracle(hz, oracle):
         assert np.linalg.norm(r) / np.linalg.norm(f) < 1.5e-10
 
 
+def test_fgmres_restart_beyond_64_vs_oracle(hz, oracle):
+    """FGMRES(80): restarts longer than 64 take the sequential Gram-Schmidt
+    form, and a cycle of more than 63 steps back-substitutes from the
+    Hessenberg matrix in memory (shorter cycles stage it in LDS).  The first
+    cycle runs all 80 steps, the second converges; iterations and solution
+    match the oracle's FGMRES(80)."""
+    print("case FGMRES(80)", flush=True)
+    d, c = s7_case(hz, n=(20, 18, 22), npml=4, smooth=True)
+    S, P = hz.Solver(d, c), oracle.Problem(d, c)
+    S.factor(hz.abi.PREC_FP64)
+    f = hz.problem.point_source(d, (9, 8, 11), amplitude=1.0)
+    u_o, st_o = P.fgmres(f, m=80, rest=4, tol=1e-12)
+    u, st = S.fgmres(_dev(f), m=80, restarts=4, tol=1e-12, prec=hz.abi.PREC_FP64)
+    print("iters oracle %d device %d" % (st_o.iters, st.iters))
+    assert st_o.converged and st.converged
+    assert st_o.iters > 80 and abs(st.iters - st_o.iters) <= 1
+    assert np.allclose(st.hist[:20], st_o.hist[:20], rtol=1e-8)
+    assert _rel(_host(u), u_o) < 1e-9
+
+
 @pytest.mark.parametrize("m", [64, 70])
 def test_fgmres_icwy_device_and_host_corrections(hz, oracle, m):
     """Restarts up to 64 correct the one-pass Gram-Schmidt coefficients on the

@@ -477,14 +477,17 @@ def bench_config2(a, hz, np, torch, steps, warmup):
     10 points per minimum wavelength, one centre source, FP64 preconditioner
     and FGMRES to relative residual 1e-8 (the value); the mixed-precision
     solve and its error against the FP64 one beside it.  The true residual of
-    the FP64 solution is recomputed with the 7-point operator."""
+    the FP64 solution is recomputed with the 7-point operator.  Unless
+    --skip-cpu: the reference's CPU path at the same size (cpu_baseline, to
+    the FP64 run's iteration count) and the GPU's speed-up over it."""
     pr, A = hz.problem, hz.abi
     n, npml = 128, 16
     h = 1.0 / n
     c = pr.random_velocity(pr.cube_geometry(n, npml, 1.0), seed=a.seed)
     omega = pr.omega_for_ppw(float(c.min()), h, 10.0)
     d = pr.cube_geometry(n, npml, omega, beta=a.beta, pml_c=pml_strength(npml, h))
-    f = torch.from_numpy(pr.point_source(d, (n // 2 - 1,) * 3, amplitude=1.0 / h ** 3)).cuda()
+    f_host = pr.point_source(d, (n // 2 - 1,) * 3, amplitude=1.0 / h ** 3)
+    f = torch.from_numpy(f_host).cuda()
     S = hz.Solver(d, c)
     kw = dict(m=a.m, restarts=max(1, a.max_iters // a.m), tol=a.tol)
     res = {}
@@ -518,6 +521,11 @@ def bench_config2(a, hz, np, torch, steps, warmup):
         "err_mixed_vs_fp64": (torch.linalg.vector_norm(u32 - u64) / torch.linalg.vector_norm(u64)).item(),
     }
     S.close()
+    if not a.skip_cpu:
+        note("config2 cpu baseline sample")
+        cpu = cpu_baseline(d, c, f_host, st64.iters, a.m, a.cpu_threads)
+        out["cpu_baseline"] = cpu
+        out["speedup_fp64_gpu_vs_cpu"] = round(cpu["value"] / t64, 1)
     return out
 
 

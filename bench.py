@@ -6,12 +6,15 @@ the complex64 FFT-in-z preconditioner and the FP64 7-point operator until the
 true relative residual is below 1e-8.  Inputs are resident in HBM before the
 timed region.  Also reported: the same solve with the FP64 preconditioner, the
 error of the mixed solution against it, the live roofline of the dominant
-kernel (the block-tridiagonal plane solve) and the CPU baseline (the FP64
-oracle on the host cores, on a bounded sample of the same workload).
+kernel (the block-tridiagonal plane solve) and the CPU baseline (the
+reference's CPU path with MKL PARDISO plane solves on the host cores, or the
+FP64 oracle where MKL is absent: the factorisation and one restart cycle at
+full size, extrapolated to the GPU run's iteration count).
 
 The default run adds the other configurations of the metric as keys of the
 same JSON line, each with its own small step count (--extra-steps,
---extra-warmup): "config2" (128^3 incl. 16-point PML, FP64 preconditioner),
+--extra-warmup): "config2" (128^3 incl. 16-point PML, FP64 preconditioner,
+with the CPU baseline at that size),
 "batched" (four frequencies on 96^3..192^3 grids, 8 sources each, right-hand
 sides per second) and "adjoint" (128^3 16-source misfit gradients per second,
 checked against the FP64 gradient and central finite differences).

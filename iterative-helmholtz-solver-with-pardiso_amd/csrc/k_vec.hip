@@ -369,15 +369,13 @@ __device__ __forceinline__ double2 zdiv(double2 a, double2 b) {
 
 // Complex Givens least-squares update of Arnoldi step j of member q (the
 // per-iteration rotation of FGMRES's Hessenberg column and of g = ||r|| e_1);
-// mb: the member's scalars.  The earlier rotations (c_i real, s_i complex) act
-// on rows (i, i+1) as [c s; -conj(s) c]; the new one zeroes h_{j+1,j}.
-__device__ void givens_member(int q, int j, int m, int ldh, double2* mb, int offH, int offCS, int offSN, int offG,
+// H: column j of the Hessenberg matrix, cs / sn: the cosines and sines, g:
+// the least-squares right-hand side (global memory or LDS copies).  The
+// earlier rotations (c_i real, s_i complex) act on rows (i, i+1) as
+// [c s; -conj(s) c]; the new one zeroes h_{j+1,j}.
+__device__ void givens_member(int q, int j, int m, double2* H, double2* cs, double2* sn, double2* g,
                               const double2* __restrict__ beta, const double2* __restrict__ nf, double tol,
                               double brk, double2* __restrict__ est) {
-  double2* H = mb + offH + (int64_t)ldh * j;
-  double2* cs = mb + offCS;
-  double2* sn = mb + offSN;
-  double2* g = mb + offG;
   if (j == 0) {
     g[0] = make_double2(beta[q].x, 0.0);
     for (int i = 1; i <= m; ++i) g[i] = make_double2(0.0, 0.0);
@@ -429,30 +427,42 @@ __global__ __launch_bounds__(64) void k_givens(int j, int m, int ldh, uint64_t m
                                               double tol, double brk, double2* __restrict__ est) {
   const int q = threadIdx.x;
   if (q >= 64 || !((mask >> q) & 1ull)) return;
-  givens_member(q, j, m, ldh, base + sb * q, offH, offCS, offSN, offG, beta, nf, tol, brk, est);
+  double2* mb = base + sb * q;
+  givens_member(q, j, m, mb + offH + (int64_t)ldh * j, mb + offCS, mb + offSN, mb + offG, beta, nf, tol, brk, est);
 }
 
 // The end of Arnoldi step j in one block per member q = blockIdx.z of `mask`:
 // h_{j+1,j} = sqrt(sum of the nb update-pass partials, partial + q nb), the
 // next basis vector's scale S[j + 1].x = 1 / h_{j+1,j} (0 where h_{j+1,j} <=
-// brk, a breakdown), then the member's Givens update
+// brk, a breakdown), then the member's Givens update on LDS copies of column j
+// of H and of the rotations, so its serial loop waits on LDS (j < ICWY_MAX)
 __global__ __launch_bounds__(RT) void k_norm_givens(const double2* __restrict__ partial, int nb, int j, int m,
                                                     int ldh, uint64_t mask, double2* __restrict__ base, int64_t sb,
                                                     int offH, int offS, int offCS, int offSN, int offG,
                                                     const double2* __restrict__ beta, const double2* __restrict__ nf,
                                                     double tol, double brk, double2* __restrict__ est) {
+  __shared__ double2 sh[ICWY_MAX + 1], scs[ICWY_MAX], ssn[ICWY_MAX];
   const int q = blockIdx.z;
   if (!((mask >> q) & 1ull)) return;
   partial += (int64_t)q * nb;
+  double2* mb = base + sb * q;
+  double2* Hc = mb + offH + (int64_t)ldh * j;
+  for (int i = threadIdx.x; i <= j; i += RT) sh[i] = Hc[i];
+  for (int i = threadIdx.x; i < j; i += RT) {
+    scs[i] = mb[offCS + i];
+    ssn[i] = mb[offSN + i];
+  }
   double sx = 0.0;
   for (int b = threadIdx.x; b < nb; b += RT) sx += partial[b].x;
-  const double2 s = block_sum(make_double2(sx, 0.0));
+  const double2 s = block_sum(make_double2(sx, 0.0));  // its barrier also publishes the copies
   if (threadIdx.x != 0) return;
-  double2* mb = base + sb * q;
   const double hn = sqrt(s.x);
-  mb[offH + (int64_t)ldh * j + j + 1] = make_double2(hn, 0.0);
+  sh[j + 1] = make_double2(hn, 0.0);
   mb[offS + j + 1] = make_double2(hn > brk ? 1.0 / hn : 0.0, 0.0);
-  givens_member(q, j, m, ldh, mb, offH, offCS, offSN, offG, beta, nf, tol, brk, est);
+  givens_member(q, j, m, sh, scs, ssn, mb + offG, beta, nf, tol, brk, est);
+  for (int i = 0; i <= j + 1; ++i) Hc[i] = sh[i];
+  mb[offCS + j] = scs[j];
+  mb[offSN + j] = ssn[j];
 }
 
 // y_i = (g_i - sum_{c > i} R_ic y_c) / R_ii for i = jj-1 .. 0 (R: the rotated
